@@ -1,0 +1,178 @@
+/*
+ * rwnmr.h -- C ABI of the MI355X random-walk NMR hot path.
+ *
+ * Plain C types only (no HIP or torch types): an FFI (ctypes, cgo, JNI, ...)
+ * binds these symbols directly.  Every call returns an rwnmr_status (0 = OK)
+ * and never exits the process; rwnmr_last_error() explains a failure.
+ *
+ * Ownership: the caller owns every input and output array; a context owns its
+ * device memory, HIP stream and launch-shape cache.  A context is not
+ * thread-safe (one host thread drives it), like the reference's single
+ * mutex-guarded inverter (B/Laplace/src/nmrinv.cpp:4-5).
+ *
+ * Host vs device pointers: rwnmr_walkers.on_device = 0 means every array of
+ * the call is host memory (the call copies in, runs, copies out and returns
+ * when done); 1 means every array is device memory already resident in HBM
+ * (the call only enqueues work on the context stream; rwnmr_ctx_synchronize
+ * waits and reports kernel-side errors).
+ *
+ * Reference paths below are under src/backend/ of RandomWalk_NMR.
+ */
+#ifndef RWNMR_H
+#define RWNMR_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rwnmr_status;
+#define RWNMR_OK 0
+#define RWNMR_ERR_INVALID 1     /* bad argument (sizes, null pointers, ranges) */
+#define RWNMR_ERR_HIP 2         /* HIP runtime error */
+#define RWNMR_ERR_ALLOC 3       /* device or host allocation failed */
+#define RWNMR_ERR_STATE 4       /* call out of order (e.g. lambda not set) */
+#define RWNMR_ERR_WALKER 5      /* a walker started outside the image (noflux) */
+#define RWNMR_ERR_NUMERIC 6     /* solver failed to converge */
+
+/* Boundary conditions: rwnmr_config BC key (B/ConfigFiles/rwnmr_config.cpp:235-246). */
+#define RWNMR_BC_NOFLUX 0
+#define RWNMR_BC_PERIODIC 1
+#define RWNMR_BC_MIRROR 2
+
+typedef struct rwnmr_ctx rwnmr_ctx;
+typedef struct rwnmr_image rwnmr_image;
+
+/* Library version string. */
+const char *rwnmr_version(void);
+
+/* Context on HIP device `device`: stream, scratch, launch-shape cache.
+ * Replaces the per-call cudaMalloc/cudaDeviceReset life cycle of
+ * B/NMR_Simulation/NMR_cpmg_cuda.cpp:929-1774 (no device reset, ever). */
+rwnmr_status rwnmr_ctx_create(rwnmr_ctx **out, int32_t device);
+void rwnmr_ctx_destroy(rwnmr_ctx *ctx);
+const char *rwnmr_last_error(const rwnmr_ctx *ctx);
+/* Wait for the context stream; returns RWNMR_ERR_WALKER if a kernel saw a
+ * walker start outside the image since the last check. */
+rwnmr_status rwnmr_ctx_synchronize(rwnmr_ctx *ctx);
+/* The context's hipStream_t, as an opaque pointer (for event timing). */
+void *rwnmr_ctx_stream(rwnmr_ctx *ctx);
+/* Launch shape for walk kernels: threads per block (64..1024, multiple of
+ * 64) and walkers per lane (1, 2 or 4).  0 selects the built-in default. */
+rwnmr_status rwnmr_ctx_set_launch(rwnmr_ctx *ctx, uint32_t threads_per_block, uint32_t walkers_per_lane);
+rwnmr_status rwnmr_ctx_get_launch(const rwnmr_ctx *ctx, uint32_t *threads_per_block, uint32_t *walkers_per_lane);
+
+/* Upload a packed image in the reference bitBlock layout: block index
+ * (x>>2) + (y>>2)*bC + (z>>2)*bC*bR, bit (x&3) + 4*(y&3) + 16*(z&3),
+ * 1 = solid, bC = ceil(cols/4), bR = ceil(rows/4), bD = ceil(depth/4)
+ * (B/BitBlock/bitBlock.h:113-139, B/BitBlock/bitBlock.cpp:138-221; the
+ * H2D copy of NMR_cpmg_cuda.cpp:1044-1048).  `blocks` is host memory of
+ * bC*bR*bD words.  solid_labels (nullable) is a cols*rows*depth byte volume
+ * (x fastest) of solid-voxel labels in [0, n_labels), n_labels <= 8, used by
+ * the per-label relaxivity path (no reference counterpart: the reference has
+ * one relaxivity per walker, B/Walker/walker.h:198-209). */
+rwnmr_status rwnmr_image_upload(rwnmr_ctx *ctx, const uint64_t *blocks, int32_t cols, int32_t rows,
+                                int32_t depth, const uint8_t *solid_labels, int32_t n_labels,
+                                rwnmr_image **out);
+void rwnmr_image_destroy(rwnmr_image *img);
+/* Pack a host voxel volume (x fastest, nonzero = solid) into bitBlock words
+ * (bC*bR*bD entries), as B/BitBlock/bitBlock.cpp:168-221 does. */
+rwnmr_status rwnmr_bitblock_pack(const uint8_t *solid, int32_t cols, int32_t rows, int32_t depth,
+                                 uint64_t *blocks_out);
+
+/* Walker set, structure of arrays (the SoA staging of
+ * B/NMR_Simulation/NMR_cpmg_cuda.cpp:1110-1170).  x/y/z are current
+ * positions (unfolded for periodic/mirror), seed the current xorshift64
+ * state.  energy (nullable): magnetisation, in/out (NULL: starts at 1.0).
+ * hits (nullable): wall-hit counter, incremented in place.  penalty
+ * (nullable): per-walker delta = 1 - (2/3) rho eps / (D0 1e3)
+ * (B/Walker/walker.h:203-209); NULL with no label path uses delta_uniform. */
+typedef struct {
+    int32_t *x, *y, *z;
+    uint64_t *seed;
+    double *energy;
+    uint32_t *hits;
+    const double *penalty;
+    double delta_uniform;
+    uint64_t n;
+    int32_t on_device;
+} rwnmr_walkers;
+
+/* Map simulation: advance every walker `steps` steps counting wall hits
+ * (map_3D_{noflux,periodic,mirror}, B/NMR_Simulation/NMR_Simulation_cuda.cpp:418-721,
+ * driven by mapSimulation_CUDA_3D_histograms :726-1270).  shift = log2 of
+ * the voxel division (0 for none). */
+rwnmr_status rwnmr_map_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint64_t steps,
+                           int32_t bc, uint32_t shift);
+
+/* CPMG: `echoes` x `steps_per_echo` steps; every wall hit multiplies the
+ * walker's energy by its delta (CPMG_walk_*, NMR_cpmg_cuda.cpp:37-139,
+ * 300-402, 574-676).  echo_sums (echoes+1 doubles): [0] = sum of initial
+ * energies, [e] = sum after echo e (image_simulation_cuda,
+ * NMR_cpmg_cuda.cpp:929-1774, before bulk and normalisation).  Sums are
+ * FP64 with a fixed reduction order (deterministic). */
+rwnmr_status rwnmr_cpmg_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                            uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums);
+
+/* CPMG with a relaxivity per solid label: a hit on a voxel of label l
+ * multiplies by label_delta[l] (host or device like the walkers, n_labels of
+ * the image).  label_hits (nullable, [n_labels][n] uint32, in/out) counts hits
+ * per label.  No reference counterpart (new data for the relaxivity fit). */
+rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w,
+                                   uint32_t echoes, uint32_t steps_per_echo, int32_t bc, uint32_t shift,
+                                   const double *label_delta, uint32_t *label_hits, double *echo_sums);
+
+/* PFGSE, reference GPU semantics (NMR_PFGSE::simulation_cuda,
+ * B/NMR_Simulation/NMR_pfgse_cuda.cpp:536-1348): for each diffusion time i
+ * in order, walk steps_inc[i] more steps (PFG_map_*, :34-339), multiply the
+ * energy by pow(delta, hits of this increment) (PFG_evaluate_energy,
+ * :342-354) and record M[i][k] = sum_w E_w cos((k . (r_w - r0_w)) * voxel)
+ * for every k (PFG_measure + PFG_reduce, :358-389, 504-532, one pass for
+ * all k instead of a launch per k).  x0/y0/z0 are the walkers' initial
+ * positions.  n_sets == 0: per-walker delta (w->penalty / delta_uniform),
+ * M is [n_delta][n_k].  n_sets > 0: set_delta[n_sets] uniform deltas are
+ * evaluated on the same trajectories (relaxation does not change them) and
+ * M is [n_sets][n_delta][n_k]; w->energy then returns set 0's energy.
+ * Host/device placement of every array follows w->on_device. */
+rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,
+                             const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc,
+                             uint32_t n_delta, const double *kx, const double *ky, const double *kz,
+                             uint32_t n_k, double voxel, int32_t bc, uint32_t shift, uint32_t n_sets,
+                             const double *set_delta, double *M);
+
+/* Laplace inversion, same names and semantics as the reference's C API
+ * (B/Laplace/include/nmrinv.h:13-39, B/Laplace/src/nmrinv.cpp): one global
+ * inverter guarded by a mutex; FP64 throughout. */
+void initialize_inverter(int num_raw_bins, double *raw_bins, double min_t2, double max_t2, int num_t2_bins,
+                         int num_lambdas, int prune_num, double noise_amp);
+void initialize_inverter_with_t2_bins(int num_raw_bins, double *raw_bins, int num_t2_bins, double *t2_bins,
+                                      int num_lambdas, int prune_num, double noise_amp);
+double find_best_lambda(int num_raw_amps, double *raw_amps);
+void get_processed_raw_amps(int data_size, double *raw_amps, double *processed_raw_amps);
+bool invert(int data_size, double *raw_amps, double *out_result);
+bool invert_with_multiple_noises(const int raw_amps_size, double *raw_amps_ptr, const int num_inversions,
+                                  double *raw_noise_ptr, double *t2_amps_ptr);
+bool get_lcurve_values(double *solution_norms_ptr, double *residual_norms_ptr, double *lambdas_ptr,
+                        double *curvature_ptr);
+double get_tikhonov_lambda(void);
+void set_lambda(double tik_lambda);
+bool get_noise(double *out_noise_data);
+void set_noise(double noise_amp, double *in_noise_data);
+void get_used_raw_bins(double *out_used_raw_bins);
+void get_used_t2_bins(double *out_used_t2_bins);
+/* Sizes of the arrays the getters above fill (the reference leaves these to
+ * the caller's knowledge of its config). */
+int rwnmr_inverter_num_used_raw_bins(void);
+int rwnmr_inverter_num_t2_bins(void);
+int rwnmr_inverter_num_lambdas(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RWNMR_H */

@@ -1,0 +1,11 @@
+"""MI355X-native random-walk NMR hot path (RandomWalk_NMR drop-in for the
+CPMG / PFGSE walker kernels, echo reduction, Laplace inversion and the
+relaxivity fit).  Compute runs in librwnmr_amd.so (HIP kernels for gfx950
+behind the C ABI in include/rwnmr/); this package is the Python host mirror.
+"""
+from ._lib import RwnmrError, load  # noqa: F401
+from .core import (BC, Context, Image, bc_code, bitblock_dims, cpmg_run, map_run,  # noqa: F401
+                   pack_bitblocks, penalty, pfgse_run, seed_and_place, sphere_pack, time_framework,
+                   time_step, voronoi_labels)
+
+__version__ = "0.1.0"

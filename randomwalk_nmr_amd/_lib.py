@@ -1,0 +1,128 @@
+"""ctypes binding of librwnmr_amd.so (the C ABI in include/rwnmr/*.h).
+
+There is no fallback: if the library is missing the import of any compute
+entry point raises, so a GPU run can never silently use another path.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librwnmr_amd.so")
+
+P = ct.c_void_p
+I32 = ct.c_int32
+U32 = ct.c_uint32
+U64 = ct.c_uint64
+I64 = ct.c_int64
+D = ct.c_double
+CI = ct.c_int
+B = ct.c_bool
+
+RWNMR_OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "ALLOC", 4: "STATE", 5: "WALKER", 6: "NUMERIC"}
+
+
+class RwnmrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rwnmr error {STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Walkers(ct.Structure):
+    """rwnmr_walkers (include/rwnmr/rwnmr.h)."""
+    _fields_ = [("x", P), ("y", P), ("z", P), ("seed", P), ("energy", P), ("hits", P),
+                ("penalty", P), ("delta_uniform", D), ("n", U64), ("on_device", I32)]
+
+
+class CpmgInstance(ct.Structure):
+    """rwnmr_cpmg_instance (include/rwnmr/rwnmr.h)."""
+    _fields_ = [("image", P), ("walkers", Walkers), ("echoes", U32), ("steps_per_echo", U32),
+                ("bc", I32), ("echo_sums", P)]
+
+
+# (name, restype, argtypes) of every exported symbol the package binds.
+SIGNATURES = {
+    "rwnmr_version": (ct.c_char_p, []),
+    "rwnmr_ctx_create": (I32, [ct.POINTER(P), I32]),
+    "rwnmr_ctx_destroy": (None, [P]),
+    "rwnmr_last_error": (ct.c_char_p, [P]),
+    "rwnmr_ctx_synchronize": (I32, [P]),
+    "rwnmr_ctx_stream": (P, [P]),
+    "rwnmr_ctx_set_launch": (I32, [P, U32, U32]),
+    "rwnmr_ctx_get_launch": (I32, [P, ct.POINTER(U32), ct.POINTER(U32)]),
+    "rwnmr_image_upload": (I32, [P, P, I32, I32, I32, P, I32, ct.POINTER(P)]),
+    "rwnmr_image_destroy": (None, [P]),
+    "rwnmr_bitblock_pack": (I32, [P, I32, I32, I32, P]),
+    "rwnmr_map_run": (I32, [P, P, ct.POINTER(Walkers), U64, I32, U32]),
+    "rwnmr_cpmg_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
+    "rwnmr_cpmg_run_labels": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P, P]),
+    "rwnmr_pfgse_run": (I32, [P, P, ct.POINTER(Walkers), P, P, P, P, U32, P, P, P, U32, D, I32, U32,
+                              U32, P, P]),
+    "rwnmr_cpmg_batch_run": (I32, [P, ct.POINTER(CpmgInstance), U32, U32]),
+    "rwnmr_cpmg_forward_ckpt": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, P, U32, P, P]),
+    "rwnmr_cpmg_backward": (I32, [P, P, P, U64, U32, U32, I32, P, U32, P, P, P]),
+    "rwnmr_ckpt_bytes": (U64, [U64, U32, U32]),
+    # host set-up (include/rwnmr/rwnmr_host.h)
+    "rwnmr_synth_sphere_pack": (I32, [I32, I32, I32, D, D, D, U64, P, ct.POINTER(D)]),
+    "rwnmr_synth_voronoi_labels": (I32, [I32, I32, I32, I32, I32, U64, P, P]),
+    "rwnmr_count_pores": (I64, [P, I32, I32, I32]),
+    "rwnmr_seed_and_place": (I32, [P, I32, I32, I32, U64, U64, P, P, P, P]),
+    "rwnmr_time_step": (D, [D, D]),
+    "rwnmr_time_framework": (I32, [D, D, D, U32, ct.POINTER(U32), ct.POINTER(U64)]),
+    "rwnmr_penalty": (D, [D, D, D]),
+    # Laplace inversion, reference C API names (B/Laplace/include/nmrinv.h)
+    "initialize_inverter": (None, [CI, P, D, D, CI, CI, CI, D]),
+    "initialize_inverter_with_t2_bins": (None, [CI, P, CI, P, CI, CI, D]),
+    "find_best_lambda": (D, [CI, P]),
+    "get_processed_raw_amps": (None, [CI, P, P]),
+    "invert": (B, [CI, P, P]),
+    "invert_with_multiple_noises": (B, [CI, P, CI, P, P]),
+    "get_lcurve_values": (B, [P, P, P, P]),
+    "get_tikhonov_lambda": (D, []),
+    "set_lambda": (None, [D]),
+    "get_noise": (B, [P]),
+    "set_noise": (None, [D, P]),
+    "get_used_raw_bins": (None, [P]),
+    "get_used_t2_bins": (None, [P]),
+    "rwnmr_inverter_num_used_raw_bins": (CI, []),
+    "rwnmr_inverter_num_t2_bins": (CI, []),
+    "rwnmr_inverter_num_lambdas": (CI, []),
+    "rwnmr_inverter_set_lambda_range": (None, [D, D]),
+    "rwnmr_laplace_nnls": (I32, [CI, CI, P, P, P, D, D, CI]),
+}
+
+_lib = None
+
+
+def load():
+    """Load librwnmr_amd.so and bind every signature; raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m randomwalk_nmr_amd.build` "
+                          "(there is no CPU fallback)")
+    L = ct.CDLL(LIB_PATH)
+    missing = []
+    for name, (res, args) in SIGNATURES.items():
+        try:
+            f = getattr(L, name)
+        except AttributeError:
+            missing.append(name)
+            continue
+        f.restype = res
+        f.argtypes = args
+    L._rwnmr_missing = missing
+    _lib = L
+    return L
+
+
+def check(code, ctx=None):
+    if code != RWNMR_OK:
+        msg = ""
+        if ctx is not None:
+            m = load().rwnmr_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise RwnmrError(code, msg)

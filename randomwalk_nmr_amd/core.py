@@ -1,0 +1,327 @@
+"""Python host API over the C ABI: contexts, images, walkers and the
+CPMG / map / PFGSE runs.
+
+Arrays may be numpy arrays (host memory: the call copies in and out and
+returns when done) or torch CUDA tensors (device memory already resident in
+HBM: the call enqueues on the context's stream; ``sync=True`` waits).
+Reference paths are under src/backend/ of RandomWalk_NMR.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib
+from ._lib import Walkers, check, load
+
+BC = {"noflux": 0, "periodic": 1, "mirror": 2}
+
+
+def bc_code(bc) -> int:
+    if isinstance(bc, str):
+        if bc not in BC:
+            raise ValueError(f"unknown boundary condition {bc!r}")
+        return BC[bc]
+    return int(bc)
+
+
+def _is_dev(a) -> bool:
+    return a is not None and hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return ct.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _host(a, dtype, n=None, copy=True):
+    if a is None:
+        return None
+    out = np.array(a, dtype=dtype, copy=copy, order="C") if copy else np.ascontiguousarray(a, dtype=dtype)
+    if n is not None and out.shape != (n,):
+        out = np.ascontiguousarray(np.broadcast_to(out, (n,)), dtype=dtype)
+    return out
+
+
+class Context:
+    """rwnmr_ctx: one device, one stream, grow-only device scratch."""
+
+    def __init__(self, device: int = 0, threads_per_block: int = 0, walkers_per_lane: int = 0):
+        L = load()
+        h = ct.c_void_p()
+        code = L.rwnmr_ctx_create(ct.byref(h), device)
+        if code != 0:
+            raise _lib.RwnmrError(code, f"rwnmr_ctx_create(device={device}) failed (no GPU?)")
+        self.handle = h
+        self.device = device
+        if threads_per_block or walkers_per_lane:
+            self.set_launch(threads_per_block, walkers_per_lane)
+
+    def set_launch(self, threads_per_block: int = 0, walkers_per_lane: int = 0):
+        check(load().rwnmr_ctx_set_launch(self.handle, threads_per_block, walkers_per_lane), self.handle)
+
+    def get_launch(self):
+        t, k = ct.c_uint32(), ct.c_uint32()
+        check(load().rwnmr_ctx_get_launch(self.handle, ct.byref(t), ct.byref(k)), self.handle)
+        return t.value, k.value
+
+    @property
+    def stream(self) -> int:
+        return load().rwnmr_ctx_stream(self.handle) or 0
+
+    def synchronize(self):
+        check(load().rwnmr_ctx_synchronize(self.handle), self.handle)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().rwnmr_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+# ------------------------------------------------------------------ images
+
+def bitblock_dims(W, H, D):
+    """ceil(dim / 4) per axis (B/BitBlock/bitBlock.cpp:138-166)."""
+    return (W + 3) // 4, (H + 3) // 4, (D + 3) // 4
+
+
+def pack_bitblocks(solid: np.ndarray) -> np.ndarray:
+    """Pack a [z, y, x] volume (nonzero = solid) into bitBlock words
+    (B/BitBlock/bitBlock.cpp:168-221)."""
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    D, H, W = solid.shape
+    bC, bR, bD = bitblock_dims(W, H, D)
+    out = np.zeros(bC * bR * bD, np.uint64)
+    check(load().rwnmr_bitblock_pack(_ptr(solid), W, H, D, _ptr(out)))
+    return out
+
+
+class Image:
+    """rwnmr_image: a bitBlock image (and optional solid labels) in HBM."""
+
+    def __init__(self, ctx: Context, blocks: np.ndarray, dims, labels: np.ndarray | None = None,
+                 n_labels: int = 0):
+        W, H, D = (int(v) for v in dims)
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint64)
+        bC, bR, bD = bitblock_dims(W, H, D)
+        if blocks.size != bC * bR * bD:
+            raise ValueError(f"expected {bC * bR * bD} bitBlock words, got {blocks.size}")
+        lab = None
+        if labels is not None:
+            lab = np.ascontiguousarray(labels, dtype=np.uint8)
+            if lab.size != W * H * D:
+                raise ValueError("labels must have W*H*D entries")
+            if not n_labels:
+                n_labels = int(lab.max()) + 1
+        h = ct.c_void_p()
+        check(load().rwnmr_image_upload(ctx.handle, _ptr(blocks), W, H, D, _ptr(lab), int(n_labels),
+                                        ct.byref(h)), ctx.handle)
+        self.ctx = ctx
+        self.handle = h
+        self.dims = (W, H, D)
+        self.n_labels = int(n_labels) if lab is not None else 0
+
+    @classmethod
+    def from_solid(cls, ctx: Context, solid: np.ndarray, labels=None, n_labels: int = 0):
+        D, H, W = solid.shape
+        return cls(ctx, pack_bitblocks(solid), (W, H, D), labels, n_labels)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().rwnmr_image_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------- synthetic inputs
+
+def sphere_pack(W, H, D, r_min, r_max, porosity, seed):
+    """Seeded overlapping solid spheres until the pore fraction <= porosity.
+    Returns (solid [D, H, W] uint8, achieved porosity)."""
+    solid = np.zeros((D, H, W), np.uint8)
+    phi = ct.c_double()
+    check(load().rwnmr_synth_sphere_pack(W, H, D, float(r_min), float(r_max), float(porosity), int(seed),
+                                         _ptr(solid), ct.byref(phi)))
+    return solid, phi.value
+
+
+def voronoi_labels(solid, n_sites, n_labels, seed):
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    D, H, W = solid.shape
+    lab = np.zeros_like(solid)
+    check(load().rwnmr_synth_voronoi_labels(W, H, D, n_sites, n_labels, int(seed), _ptr(solid), _ptr(lab)))
+    return lab
+
+
+def seed_and_place(solid, master_seed, n):
+    """Reference seeding and uniform placement (include/rwnmr/rwnmr_host.h)."""
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    D, H, W = solid.shape
+    seeds = np.zeros(n, np.uint64)
+    x = np.zeros(n, np.int32)
+    y = np.zeros(n, np.int32)
+    z = np.zeros(n, np.int32)
+    check(load().rwnmr_seed_and_place(_ptr(solid), W, H, D, int(master_seed), int(n), _ptr(seeds), _ptr(x),
+                                      _ptr(y), _ptr(z)))
+    return seeds, x, y, z
+
+
+def time_step(voxel, D0):
+    return load().rwnmr_time_step(float(voxel), float(D0))
+
+
+def time_framework(time_ms, voxel, D0, steps_per_echo):
+    e, s = ct.c_uint32(), ct.c_uint64()
+    check(load().rwnmr_time_framework(float(time_ms), float(voxel), float(D0), int(steps_per_echo),
+                                      ct.byref(e), ct.byref(s)))
+    return e.value, s.value
+
+
+def penalty(rho, voxel, D0):
+    return load().rwnmr_penalty(float(rho), float(voxel), float(D0))
+
+
+# ---------------------------------------------------------------- walkers
+
+def _walker_struct(x, y, z, seed, energy, hits, pen, delta, n, on_device):
+    w = Walkers()
+    w.x, w.y, w.z, w.seed = _ptr(x), _ptr(y), _ptr(z), _ptr(seed)
+    w.energy, w.hits, w.penalty = _ptr(energy), _ptr(hits), _ptr(pen)
+    w.delta_uniform = float(delta)
+    w.n = int(n)
+    w.on_device = int(on_device)
+    return w
+
+
+def _torch_sync():
+    import torch
+    torch.cuda.current_stream().synchronize()
+
+
+def cpmg_run(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo, bc="mirror", shift=0,
+             penalty=None, delta=1.0, energy=None, hits=None, echo_sums=True, label_delta=None,
+             label_hits=None, sync=True):
+    """CPMG walk (rwnmr_cpmg_run / rwnmr_cpmg_run_labels).
+
+    Host (numpy) inputs are copied and the results returned in a dict
+    (x, y, z, seed, energy, hits, echo_sums, label_hits).  Device (torch CUDA)
+    inputs are updated in place; echo_sums must then be a device tensor or
+    None."""
+    dev = _is_dev(x)
+    n = len(x)
+    L = load()
+    if dev:
+        _torch_sync()
+        es = echo_sums if (echo_sums is not None and echo_sums is not True and echo_sums is not False) else None
+        w = _walker_struct(x, y, z, seed, energy, hits, penalty, delta, n, 1)
+        if label_delta is not None:
+            code = L.rwnmr_cpmg_run_labels(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo,
+                                           bc_code(bc), shift, _ptr(label_delta), _ptr(label_hits), _ptr(es))
+        else:
+            code = L.rwnmr_cpmg_run(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo, bc_code(bc),
+                                    shift, _ptr(es))
+        check(code, ctx.handle)
+        if sync:
+            ctx.synchronize()
+        return dict(x=x, y=y, z=z, seed=seed, energy=energy, hits=hits, echo_sums=es, label_hits=label_hits)
+    x, y, z = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+    seed = _host(seed, np.uint64)
+    energy = np.ones(n) if energy is None else _host(energy, np.float64, n)
+    hits = np.zeros(n, np.uint32) if hits is None else _host(hits, np.uint32, n)
+    pen = None if penalty is None else _host(penalty, np.float64, n)
+    es = np.zeros(echoes + 1) if echo_sums else None
+    w = _walker_struct(x, y, z, seed, energy, hits, pen, delta, n, 0)
+    lh = None
+    if label_delta is not None:
+        ld = _host(label_delta, np.float64)
+        if label_hits is not None:
+            lh = _host(label_hits, np.uint32).reshape(-1, n) if not isinstance(label_hits, bool) else \
+                np.zeros((img.n_labels, n), np.uint32)
+        code = L.rwnmr_cpmg_run_labels(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo, bc_code(bc),
+                                       shift, _ptr(ld), _ptr(lh), _ptr(es))
+    else:
+        code = L.rwnmr_cpmg_run(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo, bc_code(bc), shift,
+                                _ptr(es))
+    check(code, ctx.handle)
+    return dict(x=x, y=y, z=z, seed=seed, energy=energy, hits=hits, echo_sums=es, label_hits=lh)
+
+
+def map_run(ctx: Context, img: Image, x, y, z, seed, *, steps, bc="mirror", shift=0, hits=None, sync=True):
+    """Hit-count walk (rwnmr_map_run)."""
+    dev = _is_dev(x)
+    n = len(x)
+    L = load()
+    if dev:
+        _torch_sync()
+        w = _walker_struct(x, y, z, seed, None, hits, None, 1.0, n, 1)
+        check(L.rwnmr_map_run(ctx.handle, img.handle, ct.byref(w), int(steps), bc_code(bc), shift), ctx.handle)
+        if sync:
+            ctx.synchronize()
+        return dict(x=x, y=y, z=z, seed=seed, hits=hits)
+    x, y, z = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+    seed = _host(seed, np.uint64)
+    hits = np.zeros(n, np.uint32) if hits is None else _host(hits, np.uint32, n)
+    w = _walker_struct(x, y, z, seed, None, hits, None, 1.0, n, 0)
+    check(L.rwnmr_map_run(ctx.handle, img.handle, ct.byref(w), int(steps), bc_code(bc), shift), ctx.handle)
+    return dict(x=x, y=y, z=z, seed=seed, hits=hits)
+
+
+def pfgse_run(ctx: Context, img: Image, x, y, z, seed, *, steps_inc, kx, ky, kz, voxel, bc="mirror",
+              x0=None, y0=None, z0=None, penalty=None, delta=1.0, set_delta=None, energy=None, hits=None,
+              M=None, sync=True):
+    """PFGSE walk + all-k measurement (rwnmr_pfgse_run).  Returns dict with
+    M shaped [n_sets or 1, n_delta, n_k] plus the walker state."""
+    dev = _is_dev(x)
+    n = len(x)
+    L = load()
+    nd, nk = len(steps_inc), len(kx)
+    ns = 0 if set_delta is None else len(set_delta)
+    if dev:
+        _torch_sync()
+        w = _walker_struct(x, y, z, seed, energy, hits, penalty, delta, n, 1)
+        code = L.rwnmr_pfgse_run(ctx.handle, img.handle, ct.byref(w), _ptr(x0), _ptr(y0), _ptr(z0),
+                                 _ptr(steps_inc), nd, _ptr(kx), _ptr(ky), _ptr(kz), nk, float(voxel),
+                                 bc_code(bc), 0, ns, _ptr(set_delta), _ptr(M))
+        check(code, ctx.handle)
+        if sync:
+            ctx.synchronize()
+        return dict(M=M, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
+    x0 = _host(x if x0 is None else x0, np.int32)
+    y0 = _host(y if y0 is None else y0, np.int32)
+    z0 = _host(z if z0 is None else z0, np.int32)
+    x, y, z = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+    seed = _host(seed, np.uint64)
+    energy = np.ones(n) if energy is None else _host(energy, np.float64, n)
+    hits = np.zeros(n, np.uint32) if hits is None else _host(hits, np.uint32, n)
+    pen = None if penalty is None else _host(penalty, np.float64, n)
+    si = _host(steps_inc, np.uint32)
+    kx, ky, kz = (_host(a, np.float64) for a in (kx, ky, kz))
+    sd = None if set_delta is None else _host(set_delta, np.float64)
+    M = np.zeros((max(ns, 1), nd, nk))
+    w = _walker_struct(x, y, z, seed, energy, hits, pen, delta, n, 0)
+    code = L.rwnmr_pfgse_run(ctx.handle, img.handle, ct.byref(w), _ptr(x0), _ptr(y0), _ptr(z0), _ptr(si), nd,
+                             _ptr(kx), _ptr(ky), _ptr(kz), nk, float(voxel), bc_code(bc), 0, ns, _ptr(sd),
+                             _ptr(M))
+    check(code, ctx.handle)
+    return dict(M=M, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)

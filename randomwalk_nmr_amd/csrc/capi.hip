@@ -1,0 +1,692 @@
+// capi.hip -- C ABI entry points of the MI355X random-walk NMR hot path
+// (declared in include/rwnmr/rwnmr.h).
+//
+// A context owns one HIP stream and grow-only device buffers, so repeated
+// calls (fit iterations, PFGSE diffusion times, benchmarks) allocate nothing
+// after the first call of a given size.  Reference paths under src/backend/.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/rwnmr/rwnmr.h"
+#include "walk_kernels.h"
+
+using rwnmr::LaunchShape;
+using rwnmr::PfgseArgs;
+using rwnmr::WalkArgs;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t want)
+    {
+        if (want <= bytes)
+            return hipSuccess;
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess)
+            bytes = want;
+        return e;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+__global__ void fill_f64_kernel(double *p, double v, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+}  // namespace
+
+struct rwnmr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    LaunchShape shape{256, 1};
+    // walker staging for host-pointer calls
+    DevBuf x, y, z, seed, energy, hits, penalty, x0, y0, z0;
+    // per-call small arrays and reductions
+    DevBuf partials, sums, label_delta, label_hits, steps_inc, kvec, set_delta;
+    DevBuf flag;  // uint32 error word
+    uint32_t *host_flag = nullptr;  // pinned readback of flag
+};
+
+struct rwnmr_image {
+    rwnmr_ctx *ctx = nullptr;
+    int32_t W = 0, H = 0, D = 0;
+    int32_t bC = 0, bR = 0, bD = 0;
+    uint64_t nblocks = 0;
+    uint64_t *blocks = nullptr;        // device
+    uint8_t *labels_blocked = nullptr; // device, 64 bytes per block
+    int32_t n_labels = 0;
+};
+
+namespace {
+
+rwnmr_status fail(rwnmr_ctx *c, rwnmr_status code, const std::string &msg)
+{
+    if (c)
+        c->err = msg;
+    return code;
+}
+
+#define RW_HIP(ctx, call)                                                                                   \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? RWNMR_ERR_ALLOC : RWNMR_ERR_HIP,                 \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                                 \
+    } while (0)
+
+rwnmr_status clear_flag(rwnmr_ctx *c)
+{
+    RW_HIP(c, c->flag.reserve(sizeof(uint32_t)));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c->stream, c->flag.as<uint32_t>(), 0u, 1ull);
+    RW_HIP(c, hipGetLastError());
+    return RWNMR_OK;
+}
+
+// Synchronise and turn a raised kernel flag into RWNMR_ERR_WALKER.
+rwnmr_status sync_and_check(rwnmr_ctx *c)
+{
+    if (!c->flag.p) {
+        RW_HIP(c, hipStreamSynchronize(c->stream));
+        return RWNMR_OK;
+    }
+    RW_HIP(c, hipMemcpyAsync(c->host_flag, c->flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    RW_HIP(c, hipStreamSynchronize(c->stream));
+    if (*c->host_flag)
+        return fail(c, RWNMR_ERR_WALKER, "a walker started outside the image (noflux boundary)");
+    return RWNMR_OK;
+}
+
+rwnmr_status check_image(rwnmr_ctx *c, const rwnmr_image *img)
+{
+    if (!img || !img->blocks)
+        return fail(c, RWNMR_ERR_INVALID, "image is null");
+    if (img->ctx != c)
+        return fail(c, RWNMR_ERR_INVALID, "image belongs to another context");
+    return RWNMR_OK;
+}
+
+rwnmr_status check_walkers(rwnmr_ctx *c, const rwnmr_image *img, const rwnmr_walkers *w, int32_t bc,
+                           uint32_t shift)
+{
+    if (!w)
+        return fail(c, RWNMR_ERR_INVALID, "walkers is null");
+    if (w->n && (!w->x || !w->y || !w->z || !w->seed))
+        return fail(c, RWNMR_ERR_INVALID, "walker position/seed arrays must be non-null");
+    if (bc < RWNMR_BC_NOFLUX || bc > RWNMR_BC_MIRROR)
+        return fail(c, RWNMR_ERR_INVALID, "unknown boundary condition");
+    if (shift > 16)
+        return fail(c, RWNMR_ERR_INVALID, "voxel-division shift above 16");
+    if (bc == RWNMR_BC_NOFLUX && (img->W < 2 || img->H < 2 || img->D < 2))
+        return fail(c, RWNMR_ERR_INVALID, "noflux needs every image dimension >= 2");
+    if (w->n > (1ull << 40))
+        return fail(c, RWNMR_ERR_INVALID, "too many walkers");
+    if (!w->on_device && bc == RWNMR_BC_NOFLUX) {
+        for (uint64_t i = 0; i < w->n; ++i) {
+            const int gx = w->x[i] >> shift, gy = w->y[i] >> shift, gz = w->z[i] >> shift;
+            if ((unsigned)gx >= (unsigned)img->W || (unsigned)gy >= (unsigned)img->H ||
+                (unsigned)gz >= (unsigned)img->D)
+                return fail(c, RWNMR_ERR_WALKER, "walker " + std::to_string(i) + " starts outside the image");
+        }
+    }
+    return RWNMR_OK;
+}
+
+// Device views of the walker arrays: the caller's pointers (on_device) or
+// the context's staging copies (host pointers, copied in here).
+struct WalkerView {
+    int32_t *x, *y, *z;
+    uint64_t *seed;
+    double *energy;
+    uint32_t *hits;
+    const double *penalty;
+};
+
+rwnmr_status stage_in(rwnmr_ctx *c, const rwnmr_walkers *w, WalkerView &v)
+{
+    if (w->on_device) {
+        v = {w->x, w->y, w->z, w->seed, w->energy, w->hits, w->penalty};
+        return RWNMR_OK;
+    }
+    const uint64_t n = w->n;
+    RW_HIP(c, c->x.reserve(n * 4 + 4));
+    RW_HIP(c, c->y.reserve(n * 4 + 4));
+    RW_HIP(c, c->z.reserve(n * 4 + 4));
+    RW_HIP(c, c->seed.reserve(n * 8 + 8));
+    v.x = c->x.as<int32_t>();
+    v.y = c->y.as<int32_t>();
+    v.z = c->z.as<int32_t>();
+    v.seed = c->seed.as<uint64_t>();
+    v.energy = nullptr;
+    v.hits = nullptr;
+    v.penalty = nullptr;
+    RW_HIP(c, hipMemcpyAsync(v.x, w->x, n * 4, hipMemcpyHostToDevice, c->stream));
+    RW_HIP(c, hipMemcpyAsync(v.y, w->y, n * 4, hipMemcpyHostToDevice, c->stream));
+    RW_HIP(c, hipMemcpyAsync(v.z, w->z, n * 4, hipMemcpyHostToDevice, c->stream));
+    RW_HIP(c, hipMemcpyAsync(v.seed, w->seed, n * 8, hipMemcpyHostToDevice, c->stream));
+    if (w->energy) {
+        RW_HIP(c, c->energy.reserve(n * 8 + 8));
+        v.energy = c->energy.as<double>();
+        RW_HIP(c, hipMemcpyAsync(v.energy, w->energy, n * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    if (w->hits) {
+        RW_HIP(c, c->hits.reserve(n * 4 + 4));
+        v.hits = c->hits.as<uint32_t>();
+        RW_HIP(c, hipMemcpyAsync(v.hits, w->hits, n * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (w->penalty) {
+        RW_HIP(c, c->penalty.reserve(n * 8 + 8));
+        RW_HIP(c, hipMemcpyAsync(c->penalty.p, w->penalty, n * 8, hipMemcpyHostToDevice, c->stream));
+        v.penalty = c->penalty.as<double>();
+    }
+    return RWNMR_OK;
+}
+
+rwnmr_status stage_out(rwnmr_ctx *c, rwnmr_walkers *w, const WalkerView &v)
+{
+    if (w->on_device)
+        return RWNMR_OK;
+    const uint64_t n = w->n;
+    RW_HIP(c, hipMemcpyAsync(w->x, v.x, n * 4, hipMemcpyDeviceToHost, c->stream));
+    RW_HIP(c, hipMemcpyAsync(w->y, v.y, n * 4, hipMemcpyDeviceToHost, c->stream));
+    RW_HIP(c, hipMemcpyAsync(w->z, v.z, n * 4, hipMemcpyDeviceToHost, c->stream));
+    RW_HIP(c, hipMemcpyAsync(w->seed, v.seed, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (w->energy)
+        RW_HIP(c, hipMemcpyAsync(w->energy, v.energy, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (w->hits)
+        RW_HIP(c, hipMemcpyAsync(w->hits, v.hits, n * 4, hipMemcpyDeviceToHost, c->stream));
+    return RWNMR_OK;
+}
+
+WalkArgs base_args(const rwnmr_image *img, const WalkerView &v, const rwnmr_walkers *w, uint32_t shift,
+                   uint32_t *flag)
+{
+    WalkArgs a;
+    memset(&a, 0, sizeof(a));
+    a.blocks = img->blocks;
+    a.labels_blocked = img->labels_blocked;
+    a.W = img->W;
+    a.H = img->H;
+    a.D = img->D;
+    a.bC = img->bC;
+    a.bCR = img->bC * img->bR;
+    a.shift = (int32_t)shift;
+    a.x = v.x;
+    a.y = v.y;
+    a.z = v.z;
+    a.seed = v.seed;
+    a.energy = v.energy;
+    a.hits = v.hits;
+    a.penalty = v.penalty;
+    a.delta_uniform = w->delta_uniform;
+    a.n = w->n;
+    a.error_flag = flag;
+    return a;
+}
+
+// Upper bound on the partial-sum scratch of one walk launch; longer CPMG runs
+// are split into several launches of fewer echoes.
+constexpr uint64_t kMaxPartialBytes = 1ull << 30;
+
+// Shared CPMG driver: emode 1 (per-walker delta) or 2 (per-label delta).
+rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                         uint32_t steps_per_echo, int32_t bc, uint32_t shift, int emode,
+                         const double *label_delta, uint32_t *label_hits, double *echo_sums)
+{
+    rwnmr_status s;
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if ((s = check_walkers(c, img, w, bc, shift)) != RWNMR_OK)
+        return s;
+    if (emode == 2) {
+        if (!img->labels_blocked || img->n_labels <= 0)
+            return fail(c, RWNMR_ERR_INVALID, "image has no solid labels");
+        if (!label_delta)
+            return fail(c, RWNMR_ERR_INVALID, "label_delta is null");
+        if (shift != 0 && label_hits)
+            return fail(c, RWNMR_ERR_INVALID, "per-label hit counts need shift == 0");
+    }
+    WalkerView v;
+    if ((s = clear_flag(c)) != RWNMR_OK)
+        return s;
+    if ((s = stage_in(c, w, v)) != RWNMR_OK)
+        return s;
+    WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
+    a.n_labels = img->n_labels;
+    if (emode == 2) {
+        if (w->on_device) {
+            a.label_delta = label_delta;
+            a.label_hits = label_hits;
+        } else {
+            RW_HIP(c, c->label_delta.reserve(sizeof(double) * img->n_labels));
+            RW_HIP(c, hipMemcpyAsync(c->label_delta.p, label_delta, sizeof(double) * img->n_labels,
+                                     hipMemcpyHostToDevice, c->stream));
+            a.label_delta = c->label_delta.as<double>();
+            if (label_hits) {
+                const size_t lb = sizeof(uint32_t) * (size_t)img->n_labels * w->n;
+                RW_HIP(c, c->label_hits.reserve(lb + 4));
+                RW_HIP(c, hipMemcpyAsync(c->label_hits.p, label_hits, lb, hipMemcpyHostToDevice, c->stream));
+                a.label_hits = c->label_hits.as<uint32_t>();
+            }
+        }
+    }
+    const uint32_t blocks = rwnmr::walk_grid_blocks(w->n, c->shape, (int)shift);
+    a.partial_stride = blocks;
+    double *sums = nullptr;
+    if (echo_sums) {
+        if (w->on_device) {
+            sums = echo_sums;
+        } else {
+            RW_HIP(c, c->sums.reserve(sizeof(double) * ((size_t)echoes + 1)));
+            sums = c->sums.as<double>();
+        }
+    }
+    // echoes per launch so that the partial buffer stays bounded
+    uint64_t chunk = echoes ? echoes : 1;
+    if (echo_sums && blocks) {
+        const uint64_t max_rows = std::max<uint64_t>(2, kMaxPartialBytes / (8ull * blocks));
+        chunk = std::min<uint64_t>(chunk, max_rows - 1);
+    }
+    if (echo_sums)
+        RW_HIP(c, c->partials.reserve(sizeof(double) * (size_t)blocks * (size_t)(chunk + 1) + 8));
+    if (a.energy == nullptr && chunk < echoes && w->n) {
+        // energies must persist between launches: keep them in context memory
+        RW_HIP(c, c->energy.reserve(w->n * 8 + 8));
+        a.energy = c->energy.as<double>();
+        const uint64_t nb = (w->n + 255) / 256;
+        hipLaunchKernelGGL(fill_f64_kernel, dim3((uint32_t)nb), dim3(256), 0, c->stream, a.energy, 1.0, w->n);
+        RW_HIP(c, hipGetLastError());
+    }
+    a.steps_per_echo = steps_per_echo;
+    uint32_t done = 0;
+    do {
+        const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
+        a.echoes = ne;
+        a.partials = echo_sums ? c->partials.as<double>() : nullptr;
+        RW_HIP(c, rwnmr::launch_walk(a, bc, emode, c->shape, c->stream));
+        if (echo_sums) {
+            // rows 0..ne of this launch are echoes done..done+ne (row 0 repeats
+            // the previous launch's last echo with identical bits)
+            RW_HIP(c, rwnmr::launch_reduce_rows(a.partials, (uint64_t)ne + 1, blocks, sums + done, c->stream));
+        }
+        done += ne;
+    } while (done < echoes);
+    if (!w->on_device) {
+        if (echo_sums)
+            RW_HIP(c, hipMemcpyAsync(echo_sums, sums, sizeof(double) * ((size_t)echoes + 1), hipMemcpyDeviceToHost,
+                                     c->stream));
+        if (emode == 2 && label_hits)
+            RW_HIP(c, hipMemcpyAsync(label_hits, a.label_hits, sizeof(uint32_t) * (size_t)img->n_labels * w->n,
+                                     hipMemcpyDeviceToHost, c->stream));
+        if ((s = stage_out(c, w, v)) != RWNMR_OK)
+            return s;
+        return sync_and_check(c);
+    }
+    return RWNMR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rwnmr_version(void) { return "rwnmr-mi355x 0.1 (gfx950)"; }
+
+rwnmr_status rwnmr_ctx_create(rwnmr_ctx **out, int32_t device)
+{
+    if (!out)
+        return RWNMR_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return RWNMR_ERR_HIP;
+    if (device < 0 || device >= count)
+        return RWNMR_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess)
+        return RWNMR_ERR_HIP;
+    rwnmr_ctx *c = new (std::nothrow) rwnmr_ctx();
+    if (!c)
+        return RWNMR_ERR_ALLOC;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RWNMR_ERR_HIP;
+    }
+    if (hipHostMalloc((void **)&c->host_flag, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return RWNMR_ERR_ALLOC;
+    }
+    *c->host_flag = 0;
+    *out = c;
+    return RWNMR_OK;
+}
+
+void rwnmr_ctx_destroy(rwnmr_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf *b : {&c->x, &c->y, &c->z, &c->seed, &c->energy, &c->hits, &c->penalty, &c->x0, &c->y0, &c->z0,
+                      &c->partials, &c->sums, &c->label_delta, &c->label_hits, &c->steps_inc, &c->kvec,
+                      &c->set_delta, &c->flag})
+        b->release();
+    if (c->host_flag)
+        (void)hipHostFree(c->host_flag);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *rwnmr_last_error(const rwnmr_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+rwnmr_status rwnmr_ctx_synchronize(rwnmr_ctx *c)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    return sync_and_check(c);
+}
+
+void *rwnmr_ctx_stream(rwnmr_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+rwnmr_status rwnmr_ctx_set_launch(rwnmr_ctx *c, uint32_t tpb, uint32_t k)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if (tpb == 0)
+        tpb = 256;
+    if (k == 0)
+        k = 1;
+    if (tpb % 64 != 0 || tpb > 1024 || !(k == 1 || k == 2 || k == 4))
+        return fail(c, RWNMR_ERR_INVALID, "threads_per_block must be a multiple of 64 <= 1024, k in {1,2,4}");
+    c->shape.threads = tpb;
+    c->shape.k = k;
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_ctx_get_launch(const rwnmr_ctx *c, uint32_t *tpb, uint32_t *k)
+{
+    if (!c || !tpb || !k)
+        return RWNMR_ERR_INVALID;
+    *tpb = c->shape.threads;
+    *k = c->shape.k;
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_bitblock_pack(const uint8_t *solid, int32_t W, int32_t H, int32_t D, uint64_t *blocks)
+{
+    if (!solid || !blocks || W <= 0 || H <= 0 || D <= 0)
+        return RWNMR_ERR_INVALID;
+    const int32_t bC = (W + 3) / 4, bR = (H + 3) / 4, bD = (D + 3) / 4;
+    const size_t nb = (size_t)bC * bR * bD;
+    memset(blocks, 0, nb * sizeof(uint64_t));
+    for (int32_t z = 0; z < D; ++z)
+        for (int32_t y = 0; y < H; ++y) {
+            const uint8_t *row = solid + ((size_t)z * H + y) * W;
+            const size_t brow = (size_t)(y >> 2) * bC + (size_t)(z >> 2) * bC * bR;
+            const int bitrow = ((y & 3) << 2) | ((z & 3) << 4);
+            for (int32_t x = 0; x < W; ++x)
+                if (row[x])
+                    blocks[brow + (x >> 2)] |= 1ull << (bitrow | (x & 3));
+        }
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_image_upload(rwnmr_ctx *c, const uint64_t *blocks, int32_t W, int32_t H, int32_t D,
+                                const uint8_t *labels, int32_t n_labels, rwnmr_image **out)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if (!out || !blocks || W <= 0 || H <= 0 || D <= 0)
+        return fail(c, RWNMR_ERR_INVALID, "image_upload: null pointer or non-positive dimension");
+    if (labels && (n_labels <= 0 || n_labels > rwnmr::kMaxLabels))
+        return fail(c, RWNMR_ERR_INVALID, "image_upload: n_labels must be in [1, 8]");
+    *out = nullptr;
+    rwnmr_image *img = new (std::nothrow) rwnmr_image();
+    if (!img)
+        return fail(c, RWNMR_ERR_ALLOC, "image_upload: host allocation");
+    img->ctx = c;
+    img->W = W;
+    img->H = H;
+    img->D = D;
+    img->bC = (W + 3) / 4;
+    img->bR = (H + 3) / 4;
+    img->bD = (D + 3) / 4;
+    img->nblocks = (uint64_t)img->bC * img->bR * img->bD;
+    if (img->nblocks > 0xFFFFFFFFull) {
+        delete img;
+        return fail(c, RWNMR_ERR_INVALID, "image_upload: more than 2^32 blocks");
+    }
+    hipError_t e = hipMalloc((void **)&img->blocks, img->nblocks * 8);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(img->blocks, blocks, img->nblocks * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && labels) {
+        for (size_t i = 0; i < (size_t)W * H * D; ++i)
+            if (labels[i] >= (uint8_t)n_labels) {
+                (void)hipFree(img->blocks);
+                delete img;
+                return fail(c, RWNMR_ERR_INVALID, "image_upload: label out of range");
+            }
+        std::vector<uint8_t> lb(img->nblocks * 64, 0);
+        for (int32_t z = 0; z < D; ++z)
+            for (int32_t y = 0; y < H; ++y)
+                for (int32_t x = 0; x < W; ++x) {
+                    const size_t b = (size_t)(x >> 2) + (size_t)(y >> 2) * img->bC +
+                                     (size_t)(z >> 2) * img->bC * img->bR;
+                    const int bit = (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
+                    lb[b * 64 + bit] = labels[((size_t)z * H + y) * W + x];
+                }
+        e = hipMalloc((void **)&img->labels_blocked, lb.size());
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(img->labels_blocked, lb.data(), lb.size(), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        img->n_labels = n_labels;
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        if (img->blocks)
+            (void)hipFree(img->blocks);
+        if (img->labels_blocked)
+            (void)hipFree(img->labels_blocked);
+        delete img;
+        return fail(c, e == hipErrorOutOfMemory ? RWNMR_ERR_ALLOC : RWNMR_ERR_HIP,
+                    std::string("image_upload: ") + hipGetErrorString(e));
+    }
+    *out = img;
+    return RWNMR_OK;
+}
+
+void rwnmr_image_destroy(rwnmr_image *img)
+{
+    if (!img)
+        return;
+    if (img->ctx)
+        (void)hipStreamSynchronize(img->ctx->stream);
+    if (img->blocks)
+        (void)hipFree(img->blocks);
+    if (img->labels_blocked)
+        (void)hipFree(img->labels_blocked);
+    delete img;
+}
+
+rwnmr_status rwnmr_map_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint64_t steps, int32_t bc,
+                           uint32_t shift)
+{
+    rwnmr_status s;
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if ((s = check_walkers(c, img, w, bc, shift)) != RWNMR_OK)
+        return s;
+    if (!w->hits)
+        return fail(c, RWNMR_ERR_INVALID, "map_run needs a hits array");
+    WalkerView v;
+    if ((s = clear_flag(c)) != RWNMR_OK)
+        return s;
+    if ((s = stage_in(c, w, v)) != RWNMR_OK)
+        return s;
+    WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
+    a.energy = nullptr;  // the map walk never touches energies
+    a.echoes = 1;
+    uint64_t left = steps;
+    while (left > 0) {
+        const uint32_t now = (uint32_t)std::min<uint64_t>(left, 1ull << 30);
+        a.steps_per_echo = now;
+        RW_HIP(c, rwnmr::launch_walk(a, bc, 0, c->shape, c->stream));
+        left -= now;
+    }
+    if (!w->on_device) {
+        double *saved = w->energy;
+        w->energy = nullptr;
+        s = stage_out(c, w, v);
+        w->energy = saved;
+        if (s != RWNMR_OK)
+            return s;
+        return sync_and_check(c);
+    }
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_cpmg_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                            uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums)
+{
+    return cpmg_common(c, img, w, echoes, steps_per_echo, bc, shift, 1, nullptr, nullptr, echo_sums);
+}
+
+rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                   uint32_t steps_per_echo, int32_t bc, uint32_t shift, const double *label_delta,
+                                   uint32_t *label_hits, double *echo_sums)
+{
+    return cpmg_common(c, img, w, echoes, steps_per_echo, bc, shift, 2, label_delta, label_hits, echo_sums);
+}
+
+rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,
+                             const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc, uint32_t n_delta,
+                             const double *kx, const double *ky, const double *kz, uint32_t n_k, double voxel,
+                             int32_t bc, uint32_t shift, uint32_t n_sets, const double *set_delta, double *M)
+{
+    rwnmr_status s;
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if ((s = check_walkers(c, img, w, bc, shift)) != RWNMR_OK)
+        return s;
+    if (shift != 0)
+        return fail(c, RWNMR_ERR_INVALID, "pfgse_run supports shift == 0 only");
+    if (n_sets > (uint32_t)rwnmr::kMaxSets)
+        return fail(c, RWNMR_ERR_INVALID, "pfgse_run: at most 8 energy sets");
+    if (!M || (n_delta && !steps_inc) || (n_k && (!kx || !ky || !kz)) || (n_sets && !set_delta) ||
+        (w->n && (!x0 || !y0 || !z0)))
+        return fail(c, RWNMR_ERR_INVALID, "pfgse_run: null array");
+    const uint32_t threads = c->shape.threads;
+    const uint32_t blocks = (uint32_t)((w->n + threads - 1) / threads);
+    const uint64_t rows = (uint64_t)(n_sets ? n_sets : 1) * n_delta * n_k;
+    if (rows * blocks * 8ull > (8ull << 30))
+        return fail(c, RWNMR_ERR_INVALID, "pfgse_run: partial buffer above 8 GiB; split the k or delta set");
+    WalkerView v;
+    if ((s = clear_flag(c)) != RWNMR_OK)
+        return s;
+    if ((s = stage_in(c, w, v)) != RWNMR_OK)
+        return s;
+    PfgseArgs p;
+    memset(&p, 0, sizeof(p));
+    p.n_delta = n_delta;
+    p.n_k = n_k;
+    p.voxel = voxel;
+    p.n_sets = n_sets;
+    if (w->on_device) {
+        p.x0 = x0;
+        p.y0 = y0;
+        p.z0 = z0;
+        p.steps_inc = steps_inc;
+        p.kx = kx;
+        p.ky = ky;
+        p.kz = kz;
+        p.set_delta = set_delta;
+    } else {
+        const uint64_t n = w->n;
+        RW_HIP(c, c->x0.reserve(n * 4 + 4));
+        RW_HIP(c, c->y0.reserve(n * 4 + 4));
+        RW_HIP(c, c->z0.reserve(n * 4 + 4));
+        RW_HIP(c, hipMemcpyAsync(c->x0.p, x0, n * 4, hipMemcpyHostToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(c->y0.p, y0, n * 4, hipMemcpyHostToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(c->z0.p, z0, n * 4, hipMemcpyHostToDevice, c->stream));
+        p.x0 = c->x0.as<int32_t>();
+        p.y0 = c->y0.as<int32_t>();
+        p.z0 = c->z0.as<int32_t>();
+        RW_HIP(c, c->steps_inc.reserve(4ull * n_delta + 4));
+        RW_HIP(c, hipMemcpyAsync(c->steps_inc.p, steps_inc, 4ull * n_delta, hipMemcpyHostToDevice, c->stream));
+        p.steps_inc = c->steps_inc.as<uint32_t>();
+        RW_HIP(c, c->kvec.reserve(24ull * n_k + 24));
+        double *kv = c->kvec.as<double>();
+        RW_HIP(c, hipMemcpyAsync(kv, kx, 8ull * n_k, hipMemcpyHostToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(kv + n_k, ky, 8ull * n_k, hipMemcpyHostToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(kv + 2ull * n_k, kz, 8ull * n_k, hipMemcpyHostToDevice, c->stream));
+        p.kx = kv;
+        p.ky = kv + n_k;
+        p.kz = kv + 2ull * n_k;
+        if (n_sets) {
+            RW_HIP(c, c->set_delta.reserve(8ull * n_sets));
+            RW_HIP(c, hipMemcpyAsync(c->set_delta.p, set_delta, 8ull * n_sets, hipMemcpyHostToDevice, c->stream));
+            p.set_delta = c->set_delta.as<double>();
+        }
+    }
+    RW_HIP(c, c->partials.reserve(8ull * rows * blocks + 8));
+    p.partials = c->partials.as<double>();
+    p.partial_stride = blocks;
+    WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
+    double *Mdev = M;
+    if (!w->on_device) {
+        RW_HIP(c, c->sums.reserve(8ull * rows + 8));
+        Mdev = c->sums.as<double>();
+    }
+    RW_HIP(c, rwnmr::launch_pfgse(a, p, bc, threads, c->stream));
+    if (w->n)
+        RW_HIP(c, rwnmr::launch_reduce_rows(p.partials, rows, blocks, Mdev, c->stream));
+    else if (rows)
+        return fail(c, RWNMR_ERR_INVALID, "pfgse_run: no walkers");
+    if (!w->on_device) {
+        RW_HIP(c, hipMemcpyAsync(M, Mdev, 8ull * rows, hipMemcpyDeviceToHost, c->stream));
+        if ((s = stage_out(c, w, v)) != RWNMR_OK)
+            return s;
+        return sync_and_check(c);
+    }
+    return RWNMR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// walk_kernels.h -- host-side view of the gfx950 walk kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rwnmr {
+
+static constexpr int kMaxLabels = 8;   // solid labels per image (per-label relaxivity)
+
+// Everything one walk launch reads or writes.  Walker arrays are device
+// pointers of length n (structure of arrays).
+struct WalkArgs {
+    const uint64_t *blocks;          // bitBlock words, reference layout
+    const uint8_t *labels_blocked;   // optional: 64 label bytes per block (bit order)
+    int32_t W, H, D;                 // image columns, rows, depth
+    int32_t bC, bCR;                 // block columns, block columns * block rows
+    int32_t shift;                   // log2(voxel division)
+    int32_t *x, *y, *z;
+    uint64_t *seed;
+    double *energy;                  // nullable: start at 1.0, not stored
+    uint32_t *hits;                  // nullable
+    const double *penalty;           // nullable: delta_uniform for every walker
+    double delta_uniform;
+    const double *label_delta;       // [n_labels] device
+    int32_t n_labels;
+    uint32_t *label_hits;            // nullable: [n_labels][n]
+    uint64_t n;
+    uint32_t echoes;
+    uint32_t steps_per_echo;
+    double *partials;                // nullable: [(echoes + 1)][partial_stride]
+    uint32_t partial_stride;         // = grid blocks of the launch
+    uint32_t *error_flag;            // device word, bit 0: walker started outside the image
+};
+
+struct LaunchShape {
+    uint32_t threads;   // per block, multiple of 64, <= 1024
+    uint32_t k;         // walkers per lane: 1, 2 or 4
+};
+
+static constexpr int kMaxSets = 8;       // energy sets evaluated on one PFGSE trajectory
+static constexpr int kPfgseKChunk = 64;  // k values staged in LDS per block sync
+
+// PFGSE-specific arguments (device pointers).  The shift > 0 voxel division
+// is not supported by the PFGSE kernel.
+struct PfgseArgs {
+    const int32_t *x0, *y0, *z0;     // initial positions (displacement origin)
+    const uint32_t *steps_inc;       // [n_delta] steps between consecutive diffusion times
+    uint32_t n_delta;
+    const double *kx, *ky, *kz;      // [n_k] wave vectors
+    uint32_t n_k;
+    double voxel;                    // voxel size (phase = k . dr * voxel)
+    uint32_t n_sets;                 // 0: per-walker delta; else uniform deltas below
+    const double *set_delta;         // [n_sets]
+    double *partials;                // [(n_sets|1) * n_delta * n_k][partial_stride]
+    uint32_t partial_stride;         // = grid blocks
+};
+
+uint32_t walk_grid_blocks(uint64_t n, const LaunchShape &ls, int shift);
+hipError_t launch_pfgse(const WalkArgs &a, const PfgseArgs &p, int bc, uint32_t threads, hipStream_t st);
+hipError_t launch_walk(const WalkArgs &a, int bc, int emode, const LaunchShape &ls, hipStream_t st);
+hipError_t launch_reduce_rows(const double *partials, uint64_t rows, uint32_t cols, double *out,
+                              hipStream_t st);
+
+}  // namespace rwnmr

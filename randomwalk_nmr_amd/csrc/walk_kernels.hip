@@ -1,0 +1,537 @@
+// walk_kernels.hip -- gfx950 kernels of the random-walk NMR hot path.
+//
+// One lane owns K walkers (structure-of-arrays in HBM, walker k of a lane at
+// index base + k * blockDim.x so every load/store is coalesced).  Each step:
+// xorshift64 in registers, direction = state % 6 (32-bit popcount form),
+// image fold kept incrementally in registers (no integer division in the
+// loop), one 8-byte bitBlock gather, then move or wall hit (hit count +1 and
+// energy *= delta).  Per-echo magnetisation is reduced in-wave (cross-lane
+// shuffles), staged per block in LDS for 32 echoes at a time, and written as
+// one FP64 partial per (echo, block); a second kernel sums the partials of
+// each echo in a fixed order, so echo sums are deterministic run to run.
+//
+// Reference semantics (paths under src/backend/ of RandomWalk_NMR):
+//   CPMG_walk_{noflux,periodic,mirror}  B/NMR_Simulation/NMR_cpmg_cuda.cpp:37-139, 300-402, 574-676
+//   map_3D_{noflux,periodic,mirror}     B/NMR_Simulation/NMR_Simulation_cuda.cpp:418-721
+//   CPMG_energyReduce + host sum        B/NMR_Simulation/NMR_cpmg_cuda.cpp:888-925, 1780-1801
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "walk_common.h"
+#include "walk_kernels.h"
+
+namespace rwnmr {
+
+static constexpr int kWave = 64;
+static constexpr int kEchoChunk = 32;   // echoes staged in LDS between block syncs
+static constexpr int kMaxWaves = 16;    // up to 1024 threads per block
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Registers of one walker.  ix/iy/iz are the image-voxel (folded)
+// coordinates, mx/my/mz the mirror reflection flags.
+struct Walker {
+    int x, y, z;
+    int ix, iy, iz;
+    int mx, my, mz;
+    uint64_t s;
+    double e;
+    double delta;
+    uint32_t h;
+    int active;
+};
+
+template <int BC>
+__device__ __forceinline__ void init_fold(const WalkArgs &a, Walker &w)
+{
+    if (BC == kMirror) {
+        w.ix = fold_mirror(w.x, a.W);
+        w.iy = fold_mirror(w.y, a.H);
+        w.iz = fold_mirror(w.z, a.D);
+        w.mx = mirror_flag(w.x, a.W);
+        w.my = mirror_flag(w.y, a.H);
+        w.mz = mirror_flag(w.z, a.D);
+    } else if (BC == kPeriodic) {
+        w.ix = fold_periodic(w.x, a.W);
+        w.iy = fold_periodic(w.y, a.H);
+        w.iz = fold_periodic(w.z, a.D);
+        w.mx = w.my = w.mz = 0;
+    } else {
+        w.ix = w.x;
+        w.iy = w.y;
+        w.iz = w.z;
+        w.mx = w.my = w.mz = 0;
+    }
+}
+
+// Candidate image voxel of the next move plus the move itself.
+struct Move {
+    int ax;     // axis 0/1/2
+    int sg;     // +-1
+    int nix, niy, niz;
+    int nm;     // new reflection flag of the moving axis
+    uint32_t blk, bit;
+};
+
+template <int BC>
+__device__ __forceinline__ Move propose(const WalkArgs &a, Walker &w)
+{
+    Move mv;
+    w.s = xorshift64(w.s);
+    const uint32_t r = mod6(w.s);
+    mv.ax = move_axis(r);
+    int sg = move_sign(r);
+    const int i = mv.ax == 0 ? w.ix : (mv.ax == 1 ? w.iy : w.iz);
+    const int d = mv.ax == 0 ? a.W : (mv.ax == 1 ? a.H : a.D);
+    int ni, nm = 0;
+    if (BC == kMirror) {
+        const int m = mv.ax == 0 ? w.mx : (mv.ax == 1 ? w.my : w.mz);
+        mirror_step(i, m, sg, d, ni, nm);
+    } else if (BC == kPeriodic) {
+        ni = periodic_step(i, sg, d);
+    } else {
+        // noflux: an outward move at a face turns around before the wall test
+        if ((sg < 0 && i == 0) || (sg > 0 && i == d - 1))
+            sg = -sg;
+        ni = i + sg;
+    }
+    mv.sg = sg;
+    mv.nm = nm;
+    mv.nix = mv.ax == 0 ? ni : w.ix;
+    mv.niy = mv.ax == 1 ? ni : w.iy;
+    mv.niz = mv.ax == 2 ? ni : w.iz;
+    mv.blk = block_index(mv.nix, mv.niy, mv.niz, a.bC, a.bCR);
+    mv.bit = bit_index(mv.nix, mv.niy, mv.niz);
+    return mv;
+}
+
+template <int BC>
+__device__ __forceinline__ void commit(Walker &w, const Move &mv)
+{
+    w.x += mv.ax == 0 ? mv.sg : 0;
+    w.y += mv.ax == 1 ? mv.sg : 0;
+    w.z += mv.ax == 2 ? mv.sg : 0;
+    w.ix = mv.nix;
+    w.iy = mv.niy;
+    w.iz = mv.niz;
+    if (BC == kMirror) {
+        w.mx = mv.ax == 0 ? mv.nm : w.mx;
+        w.my = mv.ax == 1 ? mv.nm : w.my;
+        w.mz = mv.ax == 2 ? mv.nm : w.mz;
+    }
+}
+
+// Validity of a walker's start position: noflux walkers must start inside
+// the image (the fold keeps periodic/mirror lookups in range for any start).
+template <int BC>
+__device__ __forceinline__ int start_ok(const WalkArgs &a, const Walker &w)
+{
+    if (BC != kNoflux)
+        return 1;
+    return (unsigned)w.x < (unsigned)a.W && (unsigned)w.y < (unsigned)a.H && (unsigned)w.z < (unsigned)a.D;
+}
+
+// Stage one echo's block sum; every kEchoChunk echoes (and at the last echo)
+// the block writes its partials.  All threads of the block must call this.
+__device__ __forceinline__ void stage_echo(double (*lds)[kMaxWaves], const WalkArgs &a, uint32_t echo,
+                                           double v)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    v = wave_sum(v);
+    const int slot = echo % kEchoChunk;
+    if (lane == 0)
+        lds[slot][wave] = v;
+    if (slot == kEchoChunk - 1 || echo == a.echoes) {
+        __syncthreads();
+        const int cnt = slot + 1;
+        if ((int)threadIdx.x < cnt) {
+            double s = 0.0;
+            for (int q = 0; q < nw; ++q)
+                s += lds[threadIdx.x][q];
+            const uint64_t row = (uint64_t)(echo - slot) + threadIdx.x;
+            a.partials[row * a.partial_stride + blockIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// EMODE: 0 = hit counting only (map), 1 = per-walker delta, 2 = per-label delta.
+template <int BC, int EMODE, int K>
+__global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a)
+{
+    __shared__ double lds[kEchoChunk][kMaxWaves];
+    const uint64_t base = (uint64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
+    Walker w[K];
+    uint32_t lhits[K][kMaxLabels];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t id = base + (uint64_t)k * blockDim.x;
+        w[k].active = id < a.n;
+        const uint64_t j = w[k].active ? id : 0;
+        w[k].x = w[k].active ? a.x[j] : 0;
+        w[k].y = w[k].active ? a.y[j] : 0;
+        w[k].z = w[k].active ? a.z[j] : 0;
+        w[k].s = w[k].active ? a.seed[j] : 1;
+        w[k].e = (w[k].active && a.energy) ? a.energy[j] : 1.0;
+        w[k].h = (w[k].active && a.hits) ? a.hits[j] : 0u;
+        w[k].delta = EMODE == 1 ? ((w[k].active && a.penalty) ? a.penalty[j] : a.delta_uniform) : 1.0;
+        if (w[k].active && !start_ok<BC>(a, w[k])) {
+            w[k].active = 0;
+            atomicOr(a.error_flag, 1u);
+        }
+        if (!w[k].active) {
+            w[k].x = w[k].y = w[k].z = 0;
+            w[k].e = 0.0;
+        }
+        init_fold<BC>(a, w[k]);
+#pragma unroll
+        for (int l = 0; l < kMaxLabels; ++l)
+            lhits[k][l] = (EMODE == 2 && a.label_hits && w[k].active && l < a.n_labels)
+                              ? a.label_hits[(uint64_t)l * a.n + j]
+                              : 0u;
+    }
+
+    if (a.partials) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            v += w[k].active ? w[k].e : 0.0;
+        stage_echo(lds, a, 0, v);
+    }
+
+    for (uint32_t echo = 1; echo <= a.echoes; ++echo) {
+        for (uint32_t st = 0; st < a.steps_per_echo; ++st) {
+            Move mv[K];
+            uint64_t word[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                mv[k] = propose<BC>(a, w[k]);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                word[k] = w[k].active ? a.blocks[mv[k].blk] : 0ull;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int wall = (int)((word[k] >> mv[k].bit) & 1ull);
+                if (!wall) {
+                    commit<BC>(w[k], mv[k]);
+                } else {
+                    w[k].h += 1u;
+                    if (EMODE == 1) {
+                        w[k].e *= w[k].delta;
+                    } else if (EMODE == 2) {
+                        const uint32_t lab = a.labels_blocked[(uint64_t)mv[k].blk * 64u + mv[k].bit];
+                        w[k].e *= a.label_delta[lab];
+#pragma unroll
+                        for (int l = 0; l < kMaxLabels; ++l)
+                            lhits[k][l] += (lab == (uint32_t)l) ? 1u : 0u;
+                    }
+                }
+            }
+        }
+        if (a.partials) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                v += w[k].active ? w[k].e : 0.0;
+            stage_echo(lds, a, echo, v);
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!w[k].active)
+            continue;
+        const uint64_t id = base + (uint64_t)k * blockDim.x;
+        a.x[id] = w[k].x;
+        a.y[id] = w[k].y;
+        a.z[id] = w[k].z;
+        a.seed[id] = w[k].s;
+        if (a.energy)
+            a.energy[id] = w[k].e;
+        if (a.hits)
+            a.hits[id] = w[k].h;
+        if (EMODE == 2 && a.label_hits)
+#pragma unroll
+            for (int l = 0; l < kMaxLabels; ++l)
+                if (l < a.n_labels)
+                    a.label_hits[(uint64_t)l * a.n + id] = lhits[k][l];
+    }
+}
+
+// Voxel-division path (shift > 0): walker coordinates are sub-voxel units
+// and the image voxel is (p >> shift); folds recomputed each step.
+template <int BC, int EMODE>
+__global__ __launch_bounds__(1024) void walk_kernel_shift(const WalkArgs a)
+{
+    __shared__ double lds[kEchoChunk][kMaxWaves];
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int active = id < a.n;
+    const uint64_t j = active ? id : 0;
+    int x = active ? a.x[j] : 0, y = active ? a.y[j] : 0, z = active ? a.z[j] : 0;
+    uint64_t s = active ? a.seed[j] : 1;
+    double e = (active && a.energy) ? a.energy[j] : 1.0;
+    uint32_t h = (active && a.hits) ? a.hits[j] : 0u;
+    const double delta = EMODE == 1 ? ((active && a.penalty) ? a.penalty[j] : a.delta_uniform) : 1.0;
+    const int sh = a.shift;
+    if (active && BC == kNoflux &&
+        !((unsigned)(x >> sh) < (unsigned)a.W && (unsigned)(y >> sh) < (unsigned)a.H &&
+          (unsigned)(z >> sh) < (unsigned)a.D)) {
+        active = 0;
+        atomicOr(a.error_flag, 1u);
+    }
+    if (!active)
+        e = 0.0;
+    if (a.partials)
+        stage_echo(lds, a, 0, active ? e : 0.0);
+    for (uint32_t echo = 1; echo <= a.echoes; ++echo) {
+        for (uint32_t st = 0; st < a.steps_per_echo; ++st) {
+            s = xorshift64(s);
+            const uint32_t r = mod6(s);
+            const int ax = move_axis(r);
+            int sg = move_sign(r);
+            if (BC == kNoflux) {
+                const int g = (ax == 0 ? x : (ax == 1 ? y : z)) >> sh;
+                const int d = ax == 0 ? a.W : (ax == 1 ? a.H : a.D);
+                if ((sg < 0 && g == 0) || (sg > 0 && g == d - 1))
+                    sg = -sg;
+            }
+            const int nx = x + (ax == 0 ? sg : 0);
+            const int ny = y + (ax == 1 ? sg : 0);
+            const int nz = z + (ax == 2 ? sg : 0);
+            int ix, iy, iz;
+            if (BC == kMirror) {
+                ix = fold_mirror(nx >> sh, a.W);
+                iy = fold_mirror(ny >> sh, a.H);
+                iz = fold_mirror(nz >> sh, a.D);
+            } else if (BC == kPeriodic) {
+                ix = fold_periodic(nx >> sh, a.W);
+                iy = fold_periodic(ny >> sh, a.H);
+                iz = fold_periodic(nz >> sh, a.D);
+            } else {
+                ix = nx >> sh;
+                iy = ny >> sh;
+                iz = nz >> sh;
+            }
+            const uint32_t blk = block_index(ix, iy, iz, a.bC, a.bCR);
+            const uint32_t bit = bit_index(ix, iy, iz);
+            const uint64_t word = active ? a.blocks[blk] : 0ull;
+            if (!((word >> bit) & 1ull)) {
+                x = nx;
+                y = ny;
+                z = nz;
+            } else {
+                h += 1u;
+                if (EMODE == 1)
+                    e *= delta;
+                else if (EMODE == 2)
+                    e *= a.label_delta[a.labels_blocked[(uint64_t)blk * 64u + bit]];
+            }
+        }
+        if (a.partials)
+            stage_echo(lds, a, echo, active ? e : 0.0);
+    }
+    if (active) {
+        a.x[id] = x;
+        a.y[id] = y;
+        a.z[id] = z;
+        a.seed[id] = s;
+        if (a.energy)
+            a.energy[id] = e;
+        if (a.hits)
+            a.hits[id] = h;
+    }
+}
+
+// PFGSE: the walk between diffusion times counts hits only (relaxation never
+// changes a trajectory), then every energy set is updated once with
+// pow(delta, hits of the increment) and the block sums of E cos(phase) for all
+// k are staged through LDS.  Partials: [(set * n_delta + i) * n_k + k][block].
+template <int BC>
+__global__ __launch_bounds__(1024) void pfgse_kernel(const WalkArgs a, const PfgseArgs p)
+{
+    __shared__ double lds[kPfgseKChunk][kMaxWaves];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Walker w;
+    w.active = id < a.n;
+    const uint64_t j = w.active ? id : 0;
+    w.x = w.active ? a.x[j] : 0;
+    w.y = w.active ? a.y[j] : 0;
+    w.z = w.active ? a.z[j] : 0;
+    w.s = w.active ? a.seed[j] : 1;
+    w.h = (w.active && a.hits) ? a.hits[j] : 0u;
+    const int x0 = w.active ? p.x0[j] : 0;
+    const int y0 = w.active ? p.y0[j] : 0;
+    const int z0 = w.active ? p.z0[j] : 0;
+    const double own_delta = (w.active && a.penalty) ? a.penalty[j] : a.delta_uniform;
+    double e[kMaxSets];
+    const int nsets = p.n_sets > 0 ? (int)p.n_sets : 1;
+#pragma unroll
+    for (int q = 0; q < kMaxSets; ++q)
+        e[q] = (q == 0 && w.active && a.energy) ? a.energy[j] : 1.0;
+    if (w.active && !start_ok<BC>(a, w)) {
+        w.active = 0;
+        atomicOr(a.error_flag, 1u);
+    }
+    if (!w.active) {
+        w.x = w.y = w.z = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxSets; ++q)
+            e[q] = 0.0;
+    }
+    init_fold<BC>(a, w);
+
+    for (uint32_t i = 0; i < p.n_delta; ++i) {
+        uint32_t h = 0;
+        const uint32_t steps = p.steps_inc[i];
+        for (uint32_t st = 0; st < steps; ++st) {
+            Move mv = propose<BC>(a, w);
+            const uint64_t word = w.active ? a.blocks[mv.blk] : 0ull;
+            if (!((word >> mv.bit) & 1ull))
+                commit<BC>(w, mv);
+            else
+                h += 1u;
+        }
+        w.h += h;
+#pragma unroll
+        for (int q = 0; q < kMaxSets; ++q)
+            if (q < nsets)
+                e[q] *= pow(p.n_sets > 0 ? p.set_delta[q] : own_delta, (double)h);
+        const double dX = (double)(w.x - x0);
+        const double dY = (double)(w.y - y0);
+        const double dZ = (double)(w.z - z0);
+        for (uint32_t k0 = 0; k0 < p.n_k; k0 += kPfgseKChunk) {
+            const uint32_t kc = p.n_k - k0 < (uint32_t)kPfgseKChunk ? p.n_k - k0 : (uint32_t)kPfgseKChunk;
+            for (int q = 0; q < nsets; ++q) {
+                for (uint32_t kk = 0; kk < kc; ++kk) {
+                    const uint32_t k = k0 + kk;
+                    const double ph = (p.kx[k] * dX + p.ky[k] * dY + p.kz[k] * dZ) * p.voxel;
+                    double v = w.active ? cos(ph) * e[q] : 0.0;
+                    v = wave_sum(v);
+                    if (lane == 0)
+                        lds[kk][wave] = v;
+                }
+                __syncthreads();
+                if (threadIdx.x < kc) {
+                    double s = 0.0;
+                    for (int qq = 0; qq < nw; ++qq)
+                        s += lds[threadIdx.x][qq];
+                    const uint64_t row = ((uint64_t)q * p.n_delta + i) * p.n_k + k0 + threadIdx.x;
+                    p.partials[row * p.partial_stride + blockIdx.x] = s;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (w.active) {
+        a.x[id] = w.x;
+        a.y[id] = w.y;
+        a.z[id] = w.z;
+        a.seed[id] = w.s;
+        if (a.energy)
+            a.energy[id] = e[0];
+        if (a.hits)
+            a.hits[id] = w.h;
+    }
+}
+
+hipError_t launch_pfgse(const WalkArgs &a, const PfgseArgs &p, int bc, uint32_t threads, hipStream_t st)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    const uint32_t blocks = (uint32_t)((a.n + threads - 1) / threads);
+    switch (bc) {
+    case kNoflux: hipLaunchKernelGGL(pfgse_kernel<kNoflux>, dim3(blocks), dim3(threads), 0, st, a, p); break;
+    case kPeriodic: hipLaunchKernelGGL(pfgse_kernel<kPeriodic>, dim3(blocks), dim3(threads), 0, st, a, p); break;
+    case kMirror: hipLaunchKernelGGL(pfgse_kernel<kMirror>, dim3(blocks), dim3(threads), 0, st, a, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// out[r] = sum_c partials[r * cols + c], fixed summation order per row.
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const double *partials, uint64_t rows, uint32_t cols,
+                                                          double *out)
+{
+    __shared__ double red[256];
+    for (uint64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        double s = 0.0;
+        for (uint32_t c = threadIdx.x; c < cols; c += blockDim.x)
+            s += partials[r * cols + c];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off)
+                red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            out[r] = red[0];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- launch
+
+template <int BC, int EMODE>
+static hipError_t launch_bc_emode(const WalkArgs &a, const LaunchShape &ls, hipStream_t st)
+{
+    if (a.shift != 0) {
+        const uint32_t tpb = ls.threads;
+        const uint64_t blocks = (a.n + tpb - 1) / tpb;
+        hipLaunchKernelGGL((walk_kernel_shift<BC, EMODE>), dim3((uint32_t)blocks), dim3(tpb), 0, st, a);
+        return hipGetLastError();
+    }
+    const uint64_t per_block = (uint64_t)ls.threads * ls.k;
+    const uint32_t blocks = (uint32_t)((a.n + per_block - 1) / per_block);
+    switch (ls.k) {
+    case 1: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 1>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 2>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 4>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+uint32_t walk_grid_blocks(uint64_t n, const LaunchShape &ls, int shift)
+{
+    const uint64_t per_block = shift ? (uint64_t)ls.threads : (uint64_t)ls.threads * ls.k;
+    return (uint32_t)((n + per_block - 1) / per_block);
+}
+
+hipError_t launch_walk(const WalkArgs &a, int bc, int emode, const LaunchShape &ls, hipStream_t st)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    if (walk_grid_blocks(a.n, ls, a.shift) > 0x7FFFFFFFu)
+        return hipErrorInvalidValue;
+#define RW_CASE(B, E)                                                                                   \
+    if (bc == B && emode == E)                                                                            \
+        return launch_bc_emode<B, E>(a, ls, st);
+    RW_CASE(kNoflux, 0) RW_CASE(kNoflux, 1) RW_CASE(kNoflux, 2)
+    RW_CASE(kPeriodic, 0) RW_CASE(kPeriodic, 1) RW_CASE(kPeriodic, 2)
+    RW_CASE(kMirror, 0) RW_CASE(kMirror, 1) RW_CASE(kMirror, 2)
+#undef RW_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_reduce_rows(const double *partials, uint64_t rows, uint32_t cols, double *out, hipStream_t st)
+{
+    if (rows == 0)
+        return hipSuccess;
+    const uint32_t grid = (uint32_t)(rows < 65536 ? rows : 65536);
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(grid), dim3(256), 0, st, partials, rows, cols, out);
+    return hipGetLastError();
+}
+
+}  // namespace rwnmr

@@ -1,0 +1,162 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (GPU tests).
+
+Bar: walker positions, seeds, hit counts and per-walker energies bit-exact
+(energy is a sequential product of the same doubles in the same order);
+echo sums within 1e-12 relative of the oracle's compensated long-double sum;
+PFGSE M(k, Delta) within 1e-11 of M(0, Delta) (GPU and libm cos/pow may
+differ in the last ulp per term).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import randomwalk_nmr_amd as R
+
+pytestmark = pytest.mark.gpu
+
+ECHO_RTOL = 1e-12
+
+
+def _case(W, H, D, r, phi, n, seed, rho_spread=True):
+    solid, _ = R.sphere_pack(W, H, D, r, r, phi, seed)
+    seeds, x, y, z = R.seed_and_place(solid, seed + 7, n)
+    rng = np.random.default_rng(seed)
+    pen = 1.0 - rng.uniform(0.001, 0.02, n) if rho_spread else np.full(n, 0.99)
+    return solid, seeds, x, y, z, pen
+
+
+def _assert_walkers(g, o):
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(np.asarray(g[k]), np.asarray(o[k])), k
+    if g.get("energy") is not None and o.get("energy") is not None:
+        assert np.array_equal(np.asarray(g["energy"]).view(np.uint64),
+                              np.asarray(o["energy"]).view(np.uint64)), "energy bits"
+
+
+def _assert_echoes(ge, oe):
+    ge, oe = np.asarray(ge), np.asarray(oe)
+    assert ge.shape == oe.shape
+    assert np.all(np.abs(ge - oe) <= ECHO_RTOL * np.abs(oe) + 1e-300), np.max(np.abs(ge - oe) / np.abs(oe))
+
+
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
+@pytest.mark.parametrize("shape", [(256, 1), (64, 2), (1024, 4), (128, 4)])
+def test_cpmg_matches_oracle(ctx, bc, shape):
+    print(f"[test] cpmg bc={bc} launch={shape}", flush=True)
+    solid, seeds, x, y, z, pen = _case(30, 21, 17, 3.5, 0.35, 1037, 11)
+    img = R.Image.from_solid(ctx, solid)
+    ctx.set_launch(*shape)
+    try:
+        g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=40, steps_per_echo=3, bc=bc, penalty=pen)
+    finally:
+        ctx.set_launch(256, 1)
+    o = O.walk(O.bitblock_pack(solid), (30, 21, 17), x, y, z, seeds, echoes=40, steps_per_echo=3, bc=bc,
+               penalty=pen)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+
+
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
+def test_map_matches_oracle(ctx, bc):
+    print(f"[test] map bc={bc}", flush=True)
+    solid, seeds, x, y, z, _ = _case(24, 24, 24, 4, 0.3, 777, 5)
+    img = R.Image.from_solid(ctx, solid)
+    g = R.map_run(ctx, img, x, y, z, seeds, steps=501, bc=bc)
+    o = O.walk(O.bitblock_pack(solid), (24, 24, 24), x, y, z, seeds, echoes=1, steps_per_echo=501, bc=bc,
+               echo_sums=False)
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(g[k], o[k]), k
+
+
+def test_map_hits_equal_cpmg_wall_events(ctx):
+    """The map replay and the CPMG run share trajectories (same seeds), so
+    the hit counts agree and E == delta^hits by sequential product."""
+    print("[test] map vs cpmg hits", flush=True)
+    solid, seeds, x, y, z, _ = _case(32, 32, 32, 4, 0.3, 500, 9)
+    img = R.Image.from_solid(ctx, solid)
+    d = 0.995
+    c = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=30, steps_per_echo=4, bc="mirror", delta=d)
+    m = R.map_run(ctx, img, x, y, z, seeds, steps=120, bc="mirror")
+    assert np.array_equal(c["hits"], m["hits"])
+    table = np.ones(int(m["hits"].max()) + 1)
+    for i in range(1, len(table)):
+        table[i] = table[i - 1] * d
+    assert np.array_equal(c["energy"], table[m["hits"]])
+
+
+def test_cpmg_labels_match_oracle(ctx):
+    print("[test] cpmg labels", flush=True)
+    solid, seeds, x, y, z, _ = _case(28, 28, 28, 4, 0.3, 900, 3)
+    labels = R.voronoi_labels(solid, 16, 8, 99)
+    img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
+    ld = np.linspace(0.97, 0.999, 8)
+    g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=25, steps_per_echo=4, bc="mirror", label_delta=ld,
+                   label_hits=True)
+    o = O.walk(O.bitblock_pack(solid), (28, 28, 28), x, y, z, seeds, echoes=25, steps_per_echo=4, bc="mirror",
+               labels=labels, label_delta=ld, label_hits=True)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+    assert np.array_equal(g["label_hits"], o["label_hits"].T)
+
+
+@pytest.mark.parametrize("bc", ["noflux", "mirror"])
+def test_pfgse_matches_oracle(ctx, bc):
+    print(f"[test] pfgse bc={bc}", flush=True)
+    solid, seeds, x, y, z, pen = _case(32, 32, 32, 5, 0.35, 1500, 21)
+    img = R.Image.from_solid(ctx, solid)
+    steps_inc = np.array([3, 5, 9, 20, 40], np.int64)
+    kmax = 1.3
+    kx = np.array([j * kmax / 15 for j in range(16)])
+    ky = np.zeros(16)
+    kz = 0.05 * np.arange(16)
+    g = R.pfgse_run(ctx, img, x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=ky, kz=kz, voxel=0.7, bc=bc,
+                    penalty=pen)
+    o = O.pfgse(O.bitblock_pack(solid), (32, 32, 32), x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=ky, kz=kz,
+                voxel=0.7, bc=bc, penalty=pen)
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.allclose(g["energy"], o["energy"], rtol=1e-13, atol=0)
+    M0 = o["M"][..., :1]
+    assert np.all(np.abs(g["M"] - o["M"]) <= 1e-11 * np.abs(M0))
+
+
+def test_pfgse_energy_sets(ctx):
+    """Several uniform deltas on one trajectory equal separate runs."""
+    print("[test] pfgse sets", flush=True)
+    solid, seeds, x, y, z, _ = _case(24, 24, 24, 4, 0.3, 800, 4)
+    img = R.Image.from_solid(ctx, solid)
+    si = np.array([4, 8, 16], np.int64)
+    kx = np.linspace(0, 1.0, 8)
+    zeros = np.zeros(8)
+    sets = np.array([1.0, 0.99, 0.97])
+    g = R.pfgse_run(ctx, img, x, y, z, seeds, steps_inc=si, kx=kx, ky=zeros, kz=zeros, voxel=1.0,
+                    set_delta=sets)
+    for q, d in enumerate(sets):
+        o = O.pfgse(O.bitblock_pack(solid), (24, 24, 24), x, y, z, seeds, steps_inc=si, kx=kx, ky=zeros, kz=zeros,
+                    voxel=1.0, penalty=d)
+        assert np.all(np.abs(g["M"][q] - o["M"][0]) <= 1e-11 * np.abs(o["M"][0][:, :1]))
+
+
+def test_noflux_walker_outside_image_is_an_error(ctx):
+    print("[test] noflux outside", flush=True)
+    solid, seeds, x, y, z, _ = _case(16, 16, 16, 3, 0.4, 64, 2)
+    img = R.Image.from_solid(ctx, solid)
+    x = x.copy()
+    x[5] = 16
+    with pytest.raises(R.RwnmrError):
+        R.cpmg_run(ctx, img, x, y, z, seeds, echoes=2, steps_per_echo=1, bc="noflux")
+
+
+def test_config1_small_cpmg(ctx):
+    """Plain CPMG, small: 128^3 sphere pack r=8 porosity 0.25, 10,000 walkers,
+    eps 1 um, D0 2.5, rho 10 um/s, TE 0.2 ms (3 steps), 2000 echoes, mirror."""
+    print("[test] config1 128^3", flush=True)
+    solid, phi = R.sphere_pack(128, 128, 128, 8, 8, 0.25, 2024)
+    seeds, x, y, z = R.seed_and_place(solid, 2024, 10_000)
+    img = R.Image.from_solid(ctx, solid)
+    d = R.penalty(10.0, 1.0, 2.5)
+    g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=2000, steps_per_echo=3, bc="mirror", delta=d)
+    o = O.walk(O.bitblock_pack(solid), (128, 128, 128), x, y, z, seeds, echoes=2000, steps_per_echo=3,
+               bc="mirror", penalty=d)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
