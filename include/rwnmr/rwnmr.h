@@ -66,6 +66,12 @@ void *rwnmr_ctx_stream(rwnmr_ctx *ctx);
  * 64) and walkers per lane (1, 2 or 4).  0 selects the built-in default. */
 rwnmr_status rwnmr_ctx_set_launch(rwnmr_ctx *ctx, uint32_t threads_per_block, uint32_t walkers_per_lane);
 rwnmr_status rwnmr_ctx_get_launch(const rwnmr_ctx *ctx, uint32_t *threads_per_block, uint32_t *walkers_per_lane);
+/* HIP-event timing of every walk / PFGSE kernel launched on the context
+ * stream (replaces the cudaEvent + omp_get_wtime buckets of
+ * NMR_cpmg_cuda.cpp:934-989, 1756-1773).  get_timing synchronises and
+ * returns the accumulated kernel milliseconds and launch count. */
+rwnmr_status rwnmr_ctx_set_timing(rwnmr_ctx *ctx, int32_t enable);
+rwnmr_status rwnmr_ctx_get_timing(rwnmr_ctx *ctx, double *kernel_ms, uint64_t *launches, int32_t reset);
 
 /* Upload a packed image in the reference bitBlock layout: block index
  * (x>>2) + (y>>2)*bC + (z>>2)*bC*bR, bit (x&3) + 4*(y&3) + 16*(z&3),
@@ -102,6 +108,15 @@ typedef struct {
     uint64_t n;
     int32_t on_device;
 } rwnmr_walkers;
+
+/* Autotuner: time a trial_echoes-long CPMG run of copies of the walkers for
+ * every launch shape (threads per block 64..1024 x walkers per lane 1/2/4)
+ * and keep the fastest in the context.  The reference fixes its launch shape
+ * in its config (BLOCKS/THREADSPERBLOCK/ECHOESPERKERNEL,
+ * config/default/rwnmr.config:43-47). */
+rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *ctx, const rwnmr_image *img, const rwnmr_walkers *w,
+                                uint32_t steps_per_echo, int32_t bc, uint32_t trial_echoes,
+                                uint32_t *best_threads, uint32_t *best_k, double *best_ms);
 
 /* Map simulation: advance every walker `steps` steps counting wall hits
  * (map_3D_{noflux,periodic,mirror}, B/NMR_Simulation/NMR_Simulation_cuda.cpp:418-721,

@@ -69,6 +69,31 @@ class Context:
         check(load().rwnmr_ctx_get_launch(self.handle, ct.byref(t), ct.byref(k)), self.handle)
         return t.value, k.value
 
+    def set_timing(self, enable: bool = True):
+        check(load().rwnmr_ctx_set_timing(self.handle, int(enable)), self.handle)
+
+    def get_timing(self, reset: bool = False):
+        """(accumulated walk/PFGSE kernel milliseconds, launches), HIP events."""
+        ms, n = ct.c_double(), ct.c_uint64()
+        check(load().rwnmr_ctx_get_timing(self.handle, ct.byref(ms), ct.byref(n), int(reset)), self.handle)
+        return ms.value, n.value
+
+    def autotune(self, img, x, y, z, seed, *, steps_per_echo, bc="mirror", trial_echoes=64, penalty=None,
+                 delta=1.0):
+        """Pick the fastest launch shape on copies of these walkers
+        (rwnmr_ctx_autotune).  Returns (threads_per_block, walkers_per_lane, ms)."""
+        dev = _is_dev(x)
+        if dev:
+            _torch_sync()
+        else:
+            x, y, z = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+            seed = _host(seed, np.uint64)
+        w = _walker_struct(x, y, z, seed, None, None, penalty if dev else None, delta, len(x), int(dev))
+        t, k, ms = ct.c_uint32(), ct.c_uint32(), ct.c_double()
+        check(load().rwnmr_ctx_autotune(self.handle, img.handle, ct.byref(w), int(steps_per_echo), bc_code(bc),
+                                        int(trial_echoes), ct.byref(t), ct.byref(k), ct.byref(ms)), self.handle)
+        return t.value, k.value, ms.value
+
     @property
     def stream(self) -> int:
         return load().rwnmr_ctx_stream(self.handle) or 0

@@ -76,6 +76,12 @@ struct rwnmr_ctx {
     DevBuf partials, sums, label_delta, label_hits, steps_inc, kvec, set_delta;
     DevBuf flag;  // uint32 error word
     uint32_t *host_flag = nullptr;  // pinned readback of flag
+    // optional HIP-event timing of the walk / PFGSE kernels on this stream
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> event_pool;
+    double kernel_ms = 0.0;
+    uint64_t kernel_launches = 0;
 };
 
 struct rwnmr_image {
@@ -105,6 +111,56 @@ rwnmr_status fail(rwnmr_ctx *c, rwnmr_status code, const std::string &msg)
                         std::string(#call) + ": " + hipGetErrorString(e_));                                 \
     } while (0)
 
+hipEvent_t take_event(rwnmr_ctx *c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+// Record the start of a timed kernel; returns the event (or null when off).
+hipEvent_t timing_begin(rwnmr_ctx *c)
+{
+    if (!c->timing)
+        return nullptr;
+    hipEvent_t e = take_event(c);
+    if (e)
+        (void)hipEventRecord(e, c->stream);
+    return e;
+}
+
+void timing_end(rwnmr_ctx *c, hipEvent_t start)
+{
+    if (!c->timing || !start)
+        return;
+    hipEvent_t e = take_event(c);
+    if (!e)
+        return;
+    (void)hipEventRecord(e, c->stream);
+    c->pending.emplace_back(start, e);
+}
+
+// Fold finished event pairs into the kernel-time counters (stream idle).
+void timing_collect(rwnmr_ctx *c)
+{
+    for (auto &pr : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+            c->kernel_ms += ms;
+            c->kernel_launches += 1;
+        }
+        c->event_pool.push_back(pr.first);
+        c->event_pool.push_back(pr.second);
+    }
+    c->pending.clear();
+}
+
 rwnmr_status clear_flag(rwnmr_ctx *c)
 {
     RW_HIP(c, c->flag.reserve(sizeof(uint32_t)));
@@ -118,10 +174,12 @@ rwnmr_status sync_and_check(rwnmr_ctx *c)
 {
     if (!c->flag.p) {
         RW_HIP(c, hipStreamSynchronize(c->stream));
+        timing_collect(c);
         return RWNMR_OK;
     }
     RW_HIP(c, hipMemcpyAsync(c->host_flag, c->flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     RW_HIP(c, hipStreamSynchronize(c->stream));
+    timing_collect(c);
     if (*c->host_flag)
         return fail(c, RWNMR_ERR_WALKER, "a walker started outside the image (noflux boundary)");
     return RWNMR_OK;
@@ -335,7 +393,9 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
         a.echoes = ne;
         a.partials = echo_sums ? c->partials.as<double>() : nullptr;
+        hipEvent_t t0 = timing_begin(c);
         RW_HIP(c, rwnmr::launch_walk(a, bc, emode, c->shape, c->stream));
+        timing_end(c, t0);
         if (echo_sums) {
             // rows 0..ne of this launch are echoes done..done+ne (row 0 repeats
             // the previous launch's last echo with identical bits)
@@ -405,8 +465,34 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
         b->release();
     if (c->host_flag)
         (void)hipHostFree(c->host_flag);
+    timing_collect(c);
+    for (hipEvent_t e : c->event_pool)
+        (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+rwnmr_status rwnmr_ctx_set_timing(rwnmr_ctx *c, int32_t enable)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    c->timing = enable != 0;
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_ctx_get_timing(rwnmr_ctx *c, double *kernel_ms, uint64_t *launches, int32_t reset)
+{
+    if (!c || !kernel_ms || !launches)
+        return RWNMR_ERR_INVALID;
+    RW_HIP(c, hipStreamSynchronize(c->stream));
+    timing_collect(c);
+    *kernel_ms = c->kernel_ms;
+    *launches = c->kernel_launches;
+    if (reset) {
+        c->kernel_ms = 0.0;
+        c->kernel_launches = 0;
+    }
+    return RWNMR_OK;
 }
 
 const char *rwnmr_last_error(const rwnmr_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -566,7 +652,9 @@ rwnmr_status rwnmr_map_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *
     while (left > 0) {
         const uint32_t now = (uint32_t)std::min<uint64_t>(left, 1ull << 30);
         a.steps_per_echo = now;
+        hipEvent_t t0 = timing_begin(c);
         RW_HIP(c, rwnmr::launch_walk(a, bc, 0, c->shape, c->stream));
+        timing_end(c, t0);
         left -= now;
     }
     if (!w->on_device) {
@@ -578,6 +666,105 @@ rwnmr_status rwnmr_map_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *
             return s;
         return sync_and_check(c);
     }
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *c, const rwnmr_image *img, const rwnmr_walkers *w,
+                                uint32_t steps_per_echo, int32_t bc, uint32_t trial_echoes, uint32_t *best_threads,
+                                uint32_t *best_k, double *best_ms)
+{
+    rwnmr_status s;
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if ((s = check_walkers(c, img, w, bc, 0)) != RWNMR_OK)
+        return s;
+    if (w->n == 0 || trial_echoes == 0 || steps_per_echo == 0)
+        return fail(c, RWNMR_ERR_INVALID, "autotune needs walkers, echoes and steps");
+    // trial copies of the walkers, so the caller's state is untouched
+    const uint64_t n = w->n;
+    DevBuf tx, ty, tz, ts, te, tsum;
+    auto cleanup = [&]() {
+        for (DevBuf *b : {&tx, &ty, &tz, &ts, &te, &tsum})
+            b->release();
+    };
+    const hipMemcpyKind kind = w->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (tx.reserve(n * 4) != hipSuccess || ty.reserve(n * 4) != hipSuccess || tz.reserve(n * 4) != hipSuccess ||
+        ts.reserve(n * 8) != hipSuccess || te.reserve(n * 8) != hipSuccess ||
+        tsum.reserve(8ull * (trial_echoes + 1)) != hipSuccess) {
+        cleanup();
+        return fail(c, RWNMR_ERR_ALLOC, "autotune: trial buffers");
+    }
+    const bool was_timing = c->timing;
+    const double saved_ms = c->kernel_ms;
+    const uint64_t saved_launches = c->kernel_launches;
+    const LaunchShape saved = c->shape;
+    LaunchShape best = saved;
+    double best_t = 1e300;
+    const uint32_t threads[] = {64, 128, 256, 512, 1024};
+    const uint32_t ks[] = {1, 2, 4};
+    c->timing = true;
+    (void)hipStreamSynchronize(c->stream);
+    timing_collect(c);
+    for (uint32_t t : threads)
+        for (uint32_t k : ks) {
+            rwnmr_walkers tw = *w;
+            tw.x = tx.as<int32_t>();
+            tw.y = ty.as<int32_t>();
+            tw.z = tz.as<int32_t>();
+            tw.seed = ts.as<uint64_t>();
+            tw.energy = te.as<double>();
+            tw.hits = nullptr;
+            tw.on_device = 1;
+            hipError_t e = hipMemcpyAsync(tw.x, w->x, n * 4, kind, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(tw.y, w->y, n * 4, kind, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(tw.z, w->z, n * 4, kind, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(tw.seed, w->seed, n * 8, kind, c->stream);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(fill_f64_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, c->stream,
+                                   tw.energy, 1.0, n);
+                e = hipGetLastError();
+            }
+            if (tw.penalty && !w->on_device)
+                tw.penalty = nullptr;  // host penalties: time with the uniform delta instead
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) {
+                c->timing = was_timing;
+                c->shape = saved;
+                cleanup();
+                return fail(c, RWNMR_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+            }
+            c->kernel_ms = 0.0;
+            c->kernel_launches = 0;
+            c->shape = LaunchShape{t, k};
+            s = cpmg_common(c, img, &tw, trial_echoes, steps_per_echo, bc, 0, 1, nullptr, nullptr,
+                            tsum.as<double>());
+            if (s == RWNMR_OK)
+                s = sync_and_check(c);
+            if (s != RWNMR_OK) {
+                c->timing = was_timing;
+                c->shape = saved;
+                cleanup();
+                return s;
+            }
+            if (c->kernel_ms < best_t) {
+                best_t = c->kernel_ms;
+                best = LaunchShape{t, k};
+            }
+        }
+    c->timing = was_timing;
+    c->kernel_ms = saved_ms;
+    c->kernel_launches = saved_launches;
+    c->shape = best;
+    cleanup();
+    if (best_threads)
+        *best_threads = best.threads;
+    if (best_k)
+        *best_k = best.k;
+    if (best_ms)
+        *best_ms = best_t;
     return RWNMR_OK;
 }
 
@@ -675,7 +862,9 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
         RW_HIP(c, c->sums.reserve(8ull * rows + 8));
         Mdev = c->sums.as<double>();
     }
+    hipEvent_t t0 = timing_begin(c);
     RW_HIP(c, rwnmr::launch_pfgse(a, p, bc, threads, c->stream));
+    timing_end(c, t0);
     if (w->n)
         RW_HIP(c, rwnmr::launch_reduce_rows(p.partials, rows, blocks, Mdev, c->stream));
     else if (rows)
