@@ -74,7 +74,7 @@ def lib():
                                               _I64, _I64, _P, _P, _P, _P, _I]
         L.orc_pfgse.restype = _I
         L.orc_pfgse.argtypes = [_P, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                _I, _P, _I, _P, _I, _P, _P, _P, _D, _P, _I]
+                                _I, _P, _I, _P, _I, _I, _P, _P, _P, _D, _P, _I]
         L.orc_mt_state_size.restype = _ct.c_size_t
         L.orc_mt_seed.restype = None
         L.orc_mt_seed.argtypes = [_P, _U32]
@@ -202,17 +202,20 @@ def pfgse(blocks, dims, x, y, z, seed, *, steps_inc, kx, ky, kz, voxel, bc="mirr
     energy = _np.ones(n) if energy is None else _c(energy, _np.float64).copy()
     hits = _np.zeros(n, _np.uint32) if hits is None else _c(hits, _np.uint32).copy()
     steps_inc = _c(steps_inc, _np.int64)
-    kx, ky, kz = (_c(a, _np.float64) for a in (kx, ky, kz))
+    kshape = _np.shape(kx)
+    per_delta = len(kshape) == 2
+    kx, ky, kz = (_c(_np.broadcast_to(_np.asarray(a, _np.float64), kshape), _np.float64) for a in (kx, ky, kz))
+    nk = kshape[-1]
     nsets = 0 if set_delta is None else len(set_delta)
     sd = None if set_delta is None else _c(set_delta, _np.float64)
     pen = None if penalty is None else _c(_np.broadcast_to(penalty, (n,)), _np.float64)
     if nsets == 0 and pen is None:
         pen = _np.ones(n)
-    M = _np.zeros((max(nsets, 1), len(steps_inc), len(kx)))
+    M = _np.zeros((max(nsets, 1), len(steps_inc), nk))
     blocks = _c(blocks, _np.uint64)
     rc = lib().orc_pfgse(_ptr(blocks), W, H, D, BC_CODES.get(bc, bc), shift, n, _ptr(x0), _ptr(y0),
                          _ptr(z0), _ptr(x), _ptr(y), _ptr(z), _ptr(seed), _ptr(energy), _ptr(hits),
-                         _ptr(pen), nsets, _ptr(sd), len(steps_inc), _ptr(steps_inc), len(kx),
+                         _ptr(pen), nsets, _ptr(sd), len(steps_inc), _ptr(steps_inc), nk, int(per_delta),
                          _ptr(kx), _ptr(ky), _ptr(kz), float(voxel), _ptr(M), threads)
     if rc:
         raise ValueError(f"orc_pfgse failed ({rc})")

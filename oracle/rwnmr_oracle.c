@@ -456,14 +456,15 @@ double orc_cpmg_reference_loop(const uint64_t *blocks, int W, int H, int D, int 
  * penalty: per-walker delta (n), or, when n_sets > 0, set_delta[n_sets] gives
  * one uniform delta per set and M is laid out [set][i][k].  energy (n) holds
  * the walkers' energy for set 0 / per-walker mode on return (may be NULL).
+ * k_per_delta != 0: kx/ky/kz hold n_delta x n_k values (a k set per time).
  * Sums are compensated long double; cos is libm's double cos.
  */
 int orc_pfgse(const uint64_t *blocks, int W, int H, int D, int bc, int shift, int64_t n,
               const int32_t *x0, const int32_t *y0, const int32_t *z0, int32_t *x, int32_t *y,
               int32_t *z, uint64_t *seed, double *energy, uint32_t *hits, const double *penalty,
               int n_sets, const double *set_delta, int n_delta, const int64_t *steps_inc, int n_k,
-              const double *kx, const double *ky, const double *kz, double voxel, double *M,
-              int threads)
+              int k_per_delta, const double *kx, const double *ky, const double *kz, double voxel,
+              double *M, int threads)
 {
     if (n < 0 || n_delta < 0 || n_k < 0)
         return 1;
@@ -516,7 +517,8 @@ int orc_pfgse(const uint64_t *blocks, int W, int H, int D, int bc, int shift, in
                 double dY = (double)(py - y0[w]);
                 double dZ = (double)(pz - z0[w]);
                 for (int k = 0; k < n_k; ++k) {
-                    double c = cos((kx[k] * dX + ky[k] * dY + kz[k] * dZ) * voxel);
+                    const size_t kk = (k_per_delta ? (size_t)i * n_k : 0) + (size_t)k;
+                    double c = cos((kx[kk] * dX + ky[kk] * dY + kz[kk] * dZ) * voxel);
                     for (int q = 0; q < ns; ++q)
                         orc_acc_add(&my[((size_t)q * n_delta + i) * n_k + k], c * e[q]);
                 }

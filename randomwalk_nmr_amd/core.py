@@ -315,19 +315,22 @@ def map_run(ctx: Context, img: Image, x, y, z, seed, *, steps, bc="mirror", shif
 def pfgse_run(ctx: Context, img: Image, x, y, z, seed, *, steps_inc, kx, ky, kz, voxel, bc="mirror",
               x0=None, y0=None, z0=None, penalty=None, delta=1.0, set_delta=None, energy=None, hits=None,
               M=None, sync=True):
-    """PFGSE walk + all-k measurement (rwnmr_pfgse_run).  Returns dict with
-    M shaped [n_sets or 1, n_delta, n_k] plus the walker state."""
+    """PFGSE walk + all-k measurement (rwnmr_pfgse_run).  kx/ky/kz are 1-D
+    (one k set) or 2-D [n_delta, n_k] (a k set per diffusion time).  Returns
+    dict with M shaped [n_sets or 1, n_delta, n_k] plus the walker state."""
     dev = _is_dev(x)
     n = len(x)
     L = load()
-    nd, nk = len(steps_inc), len(kx)
+    nd = len(steps_inc)
+    per_delta = len(kx.shape) == 2
+    nk = kx.shape[-1]
     ns = 0 if set_delta is None else len(set_delta)
     if dev:
         _torch_sync()
         w = _walker_struct(x, y, z, seed, energy, hits, penalty, delta, n, 1)
         code = L.rwnmr_pfgse_run(ctx.handle, img.handle, ct.byref(w), _ptr(x0), _ptr(y0), _ptr(z0),
-                                 _ptr(steps_inc), nd, _ptr(kx), _ptr(ky), _ptr(kz), nk, float(voxel),
-                                 bc_code(bc), 0, ns, _ptr(set_delta), _ptr(M))
+                                 _ptr(steps_inc), nd, _ptr(kx), _ptr(ky), _ptr(kz), nk, int(per_delta),
+                                 float(voxel), bc_code(bc), 0, ns, _ptr(set_delta), _ptr(M))
         check(code, ctx.handle)
         if sync:
             ctx.synchronize()
@@ -341,12 +344,13 @@ def pfgse_run(ctx: Context, img: Image, x, y, z, seed, *, steps_inc, kx, ky, kz,
     hits = np.zeros(n, np.uint32) if hits is None else _host(hits, np.uint32, n)
     pen = None if penalty is None else _host(penalty, np.float64, n)
     si = _host(steps_inc, np.uint32)
-    kx, ky, kz = (_host(a, np.float64) for a in (kx, ky, kz))
+    kx, ky, kz = (np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), np.shape(kx)))
+                  for a in (kx, ky, kz))
     sd = None if set_delta is None else _host(set_delta, np.float64)
     M = np.zeros((max(ns, 1), nd, nk))
     w = _walker_struct(x, y, z, seed, energy, hits, pen, delta, n, 0)
     code = L.rwnmr_pfgse_run(ctx.handle, img.handle, ct.byref(w), _ptr(x0), _ptr(y0), _ptr(z0), _ptr(si), nd,
-                             _ptr(kx), _ptr(ky), _ptr(kz), nk, float(voxel), bc_code(bc), 0, ns, _ptr(sd),
-                             _ptr(M))
+                             _ptr(kx), _ptr(ky), _ptr(kz), nk, int(per_delta), float(voxel), bc_code(bc), 0, ns,
+                             _ptr(sd), _ptr(M))
     check(code, ctx.handle)
     return dict(M=M, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)

@@ -783,8 +783,9 @@ rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_w
 
 rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,
                              const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc, uint32_t n_delta,
-                             const double *kx, const double *ky, const double *kz, uint32_t n_k, double voxel,
-                             int32_t bc, uint32_t shift, uint32_t n_sets, const double *set_delta, double *M)
+                             const double *kx, const double *ky, const double *kz, uint32_t n_k, int32_t k_per_delta,
+                             double voxel, int32_t bc, uint32_t shift, uint32_t n_sets, const double *set_delta,
+                             double *M)
 {
     rwnmr_status s;
     if (!c)
@@ -814,6 +815,7 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
     memset(&p, 0, sizeof(p));
     p.n_delta = n_delta;
     p.n_k = n_k;
+    p.k_delta_stride = k_per_delta ? n_k : 0u;
     p.voxel = voxel;
     p.n_sets = n_sets;
     if (w->on_device) {
@@ -839,14 +841,15 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
         RW_HIP(c, c->steps_inc.reserve(4ull * n_delta + 4));
         RW_HIP(c, hipMemcpyAsync(c->steps_inc.p, steps_inc, 4ull * n_delta, hipMemcpyHostToDevice, c->stream));
         p.steps_inc = c->steps_inc.as<uint32_t>();
-        RW_HIP(c, c->kvec.reserve(24ull * n_k + 24));
+        const uint64_t nkv = (uint64_t)n_k * (k_per_delta ? n_delta : 1u);
+        RW_HIP(c, c->kvec.reserve(24ull * nkv + 24));
         double *kv = c->kvec.as<double>();
-        RW_HIP(c, hipMemcpyAsync(kv, kx, 8ull * n_k, hipMemcpyHostToDevice, c->stream));
-        RW_HIP(c, hipMemcpyAsync(kv + n_k, ky, 8ull * n_k, hipMemcpyHostToDevice, c->stream));
-        RW_HIP(c, hipMemcpyAsync(kv + 2ull * n_k, kz, 8ull * n_k, hipMemcpyHostToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(kv, kx, 8ull * nkv, hipMemcpyHostToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(kv + nkv, ky, 8ull * nkv, hipMemcpyHostToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(kv + 2ull * nkv, kz, 8ull * nkv, hipMemcpyHostToDevice, c->stream));
         p.kx = kv;
-        p.ky = kv + n_k;
-        p.kz = kv + 2ull * n_k;
+        p.ky = kv + nkv;
+        p.kz = kv + 2ull * nkv;
         if (n_sets) {
             RW_HIP(c, c->set_delta.reserve(8ull * n_sets));
             RW_HIP(c, hipMemcpyAsync(c->set_delta.p, set_delta, 8ull * n_sets, hipMemcpyHostToDevice, c->stream));

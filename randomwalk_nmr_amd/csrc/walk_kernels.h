@@ -46,8 +46,9 @@ struct PfgseArgs {
     const int32_t *x0, *y0, *z0;     // initial positions (displacement origin)
     const uint32_t *steps_inc;       // [n_delta] steps between consecutive diffusion times
     uint32_t n_delta;
-    const double *kx, *ky, *kz;      // [n_k] wave vectors
+    const double *kx, *ky, *kz;      // wave vectors: [n_k], or [n_delta][n_k] per diffusion time
     uint32_t n_k;
+    uint32_t k_delta_stride;         // 0: one k set for every diffusion time; n_k: a set per time
     double voxel;                    // voxel size (phase = k . dr * voxel)
     uint32_t n_sets;                 // 0: per-walker delta; else uniform deltas below
     const double *set_delta;         // [n_sets]

@@ -415,7 +415,8 @@ __global__ __launch_bounds__(1024) void pfgse_kernel(const WalkArgs a, const Pfg
             for (int q = 0; q < nsets; ++q) {
                 for (uint32_t kk = 0; kk < kc; ++kk) {
                     const uint32_t k = k0 + kk;
-                    const double ph = (p.kx[k] * dX + p.ky[k] * dY + p.kz[k] * dZ) * p.voxel;
+                    const uint64_t kk_i = (uint64_t)i * p.k_delta_stride + k;
+                    const double ph = (p.kx[kk_i] * dX + p.ky[kk_i] * dY + p.kz[kk_i] * dZ) * p.voxel;
                     double v = w.active ? cos(ph) * e[q] : 0.0;
                     v = wave_sum(v);
                     if (lane == 0)
