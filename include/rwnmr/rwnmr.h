@@ -156,12 +156,15 @@ rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr
  * k_per_delta = 0: kx/ky/kz hold one set of n_k wave vectors (the
  * reference's fixed gradient list, NMR_pfgse.cpp:272-313); 1: they hold
  * n_delta x n_k values, a k set per diffusion time.
+ * msd (nullable, [n_delta][3]): sums over walkers of the squared x, y, z
+ * displacements in voxel^2 at each diffusion time, unweighted (the inputs of
+ * recoverDmsdWithoutSampling, NMR_pfgse.cpp:1098-1160; exact integers).
  * Host/device placement of every array follows w->on_device. */
 rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,
                              const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc,
                              uint32_t n_delta, const double *kx, const double *ky, const double *kz,
                              uint32_t n_k, int32_t k_per_delta, double voxel, int32_t bc, uint32_t shift,
-                             uint32_t n_sets, const double *set_delta, double *M);
+                             uint32_t n_sets, const double *set_delta, double *M, double *msd);
 
 /* Laplace inversion, same names and semantics as the reference's C API
  * (B/Laplace/include/nmrinv.h:13-39, B/Laplace/src/nmrinv.cpp): one global

@@ -74,7 +74,7 @@ def lib():
                                               _I64, _I64, _P, _P, _P, _P, _I]
         L.orc_pfgse.restype = _I
         L.orc_pfgse.argtypes = [_P, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                _I, _P, _I, _P, _I, _I, _P, _P, _P, _D, _P, _I]
+                                _I, _P, _I, _P, _I, _I, _P, _P, _P, _D, _P, _P, _I]
         L.orc_mt_state_size.restype = _ct.c_size_t
         L.orc_mt_seed.restype = None
         L.orc_mt_seed.argtypes = [_P, _U32]
@@ -212,14 +212,15 @@ def pfgse(blocks, dims, x, y, z, seed, *, steps_inc, kx, ky, kz, voxel, bc="mirr
     if nsets == 0 and pen is None:
         pen = _np.ones(n)
     M = _np.zeros((max(nsets, 1), len(steps_inc), nk))
+    msd = _np.zeros((len(steps_inc), 3))
     blocks = _c(blocks, _np.uint64)
     rc = lib().orc_pfgse(_ptr(blocks), W, H, D, BC_CODES.get(bc, bc), shift, n, _ptr(x0), _ptr(y0),
                          _ptr(z0), _ptr(x), _ptr(y), _ptr(z), _ptr(seed), _ptr(energy), _ptr(hits),
                          _ptr(pen), nsets, _ptr(sd), len(steps_inc), _ptr(steps_inc), nk, int(per_delta),
-                         _ptr(kx), _ptr(ky), _ptr(kz), float(voxel), _ptr(M), threads)
+                         _ptr(kx), _ptr(ky), _ptr(kz), float(voxel), _ptr(M), _ptr(msd), threads)
     if rc:
         raise ValueError(f"orc_pfgse failed ({rc})")
-    return dict(M=M, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
+    return dict(M=M, msd=msd, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
 
 
 # ---------------------------------------------------------------- seeding

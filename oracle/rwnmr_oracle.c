@@ -464,8 +464,13 @@ int orc_pfgse(const uint64_t *blocks, int W, int H, int D, int bc, int shift, in
               int32_t *z, uint64_t *seed, double *energy, uint32_t *hits, const double *penalty,
               int n_sets, const double *set_delta, int n_delta, const int64_t *steps_inc, int n_k,
               int k_per_delta, const double *kx, const double *ky, const double *kz, double voxel,
-              double *M, int threads)
+              double *M, double *msd, int threads)
 {
+    /* msd (optional, n_delta x 3): per-axis sums of squared displacements
+     * (voxel^2) at each diffusion time, the inputs of
+     * recoverDmsdWithoutSampling (B/NMR_Simulation/NMR_pfgse.cpp:1098-1160). */
+    if (msd)
+        memset(msd, 0, sizeof(double) * 3 * (size_t)(n_delta > 0 ? n_delta : 0));
     if (n < 0 || n_delta < 0 || n_k < 0)
         return 1;
     int ns = n_sets > 0 ? n_sets : 1;
@@ -516,6 +521,20 @@ int orc_pfgse(const uint64_t *blocks, int W, int H, int D, int bc, int shift, in
                 double dX = (double)(px - x0[w]);
                 double dY = (double)(py - y0[w]);
                 double dZ = (double)(pz - z0[w]);
+                if (msd) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    msd[3 * i + 0] += dX * dX;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    msd[3 * i + 1] += dY * dY;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    msd[3 * i + 2] += dZ * dZ;
+                }
                 for (int k = 0; k < n_k; ++k) {
                     const size_t kk = (k_per_delta ? (size_t)i * n_k : 0) + (size_t)k;
                     double c = cos((kx[kk] * dX + ky[kk] * dY + kz[kk] * dZ) * voxel);

@@ -63,7 +63,7 @@ SIGNATURES = {
     "rwnmr_cpmg_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
     "rwnmr_cpmg_run_labels": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P, P]),
     "rwnmr_pfgse_run": (I32, [P, P, ct.POINTER(Walkers), P, P, P, P, U32, P, P, P, U32, I32, D, I32, U32,
-                              U32, P, P]),
+                              U32, P, P, P]),
     "rwnmr_cpmg_batch_run": (I32, [P, ct.POINTER(CpmgInstance), U32, U32]),
     "rwnmr_cpmg_forward_ckpt": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, P, U32, P, P]),
     "rwnmr_cpmg_backward": (I32, [P, P, P, U64, U32, U32, I32, P, U32, P, P, P]),

@@ -314,10 +314,11 @@ def map_run(ctx: Context, img: Image, x, y, z, seed, *, steps, bc="mirror", shif
 
 def pfgse_run(ctx: Context, img: Image, x, y, z, seed, *, steps_inc, kx, ky, kz, voxel, bc="mirror",
               x0=None, y0=None, z0=None, penalty=None, delta=1.0, set_delta=None, energy=None, hits=None,
-              M=None, sync=True):
+              M=None, msd=None, sync=True):
     """PFGSE walk + all-k measurement (rwnmr_pfgse_run).  kx/ky/kz are 1-D
     (one k set) or 2-D [n_delta, n_k] (a k set per diffusion time).  Returns
-    dict with M shaped [n_sets or 1, n_delta, n_k] plus the walker state."""
+    dict with M shaped [n_sets or 1, n_delta, n_k], msd [n_delta, 3] (sums of
+    squared displacements in voxel^2) plus the walker state."""
     dev = _is_dev(x)
     n = len(x)
     L = load()
@@ -330,11 +331,11 @@ def pfgse_run(ctx: Context, img: Image, x, y, z, seed, *, steps_inc, kx, ky, kz,
         w = _walker_struct(x, y, z, seed, energy, hits, penalty, delta, n, 1)
         code = L.rwnmr_pfgse_run(ctx.handle, img.handle, ct.byref(w), _ptr(x0), _ptr(y0), _ptr(z0),
                                  _ptr(steps_inc), nd, _ptr(kx), _ptr(ky), _ptr(kz), nk, int(per_delta),
-                                 float(voxel), bc_code(bc), 0, ns, _ptr(set_delta), _ptr(M))
+                                 float(voxel), bc_code(bc), 0, ns, _ptr(set_delta), _ptr(M), _ptr(msd))
         check(code, ctx.handle)
         if sync:
             ctx.synchronize()
-        return dict(M=M, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
+        return dict(M=M, msd=msd, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
     x0 = _host(x if x0 is None else x0, np.int32)
     y0 = _host(y if y0 is None else y0, np.int32)
     z0 = _host(z if z0 is None else z0, np.int32)
@@ -348,9 +349,10 @@ def pfgse_run(ctx: Context, img: Image, x, y, z, seed, *, steps_inc, kx, ky, kz,
                   for a in (kx, ky, kz))
     sd = None if set_delta is None else _host(set_delta, np.float64)
     M = np.zeros((max(ns, 1), nd, nk))
+    msd = np.zeros((nd, 3))
     w = _walker_struct(x, y, z, seed, energy, hits, pen, delta, n, 0)
     code = L.rwnmr_pfgse_run(ctx.handle, img.handle, ct.byref(w), _ptr(x0), _ptr(y0), _ptr(z0), _ptr(si), nd,
                              _ptr(kx), _ptr(ky), _ptr(kz), nk, int(per_delta), float(voxel), bc_code(bc), 0, ns,
-                             _ptr(sd), _ptr(M))
+                             _ptr(sd), _ptr(M), _ptr(msd))
     check(code, ctx.handle)
-    return dict(M=M, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
+    return dict(M=M, msd=msd, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)

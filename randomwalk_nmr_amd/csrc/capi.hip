@@ -785,7 +785,7 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
                              const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc, uint32_t n_delta,
                              const double *kx, const double *ky, const double *kz, uint32_t n_k, int32_t k_per_delta,
                              double voxel, int32_t bc, uint32_t shift, uint32_t n_sets, const double *set_delta,
-                             double *M)
+                             double *M, double *msd)
 {
     rwnmr_status s;
     if (!c)
@@ -801,9 +801,12 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
     if (!M || (n_delta && !steps_inc) || (n_k && (!kx || !ky || !kz)) || (n_sets && !set_delta) ||
         (w->n && (!x0 || !y0 || !z0)))
         return fail(c, RWNMR_ERR_INVALID, "pfgse_run: null array");
+    if (w->n == 0)
+        return fail(c, RWNMR_ERR_INVALID, "pfgse_run: no walkers");
     const uint32_t threads = c->shape.threads;
     const uint32_t blocks = (uint32_t)((w->n + threads - 1) / threads);
-    const uint64_t rows = (uint64_t)(n_sets ? n_sets : 1) * n_delta * n_k;
+    const uint64_t mrows = (uint64_t)(n_sets ? n_sets : 1) * n_delta * n_k;
+    const uint64_t rows = mrows + (msd ? 3ull * n_delta : 0ull);
     if (rows * blocks * 8ull > (8ull << 30))
         return fail(c, RWNMR_ERR_INVALID, "pfgse_run: partial buffer above 8 GiB; split the k or delta set");
     WalkerView v;
@@ -859,21 +862,24 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
     RW_HIP(c, c->partials.reserve(8ull * rows * blocks + 8));
     p.partials = c->partials.as<double>();
     p.partial_stride = blocks;
+    p.msd_rows = msd ? mrows : 0;
     WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
-    double *Mdev = M;
+    double *Mdev = M, *msd_dev = msd;
     if (!w->on_device) {
         RW_HIP(c, c->sums.reserve(8ull * rows + 8));
         Mdev = c->sums.as<double>();
+        msd_dev = Mdev + mrows;
     }
     hipEvent_t t0 = timing_begin(c);
     RW_HIP(c, rwnmr::launch_pfgse(a, p, bc, threads, c->stream));
     timing_end(c, t0);
-    if (w->n)
-        RW_HIP(c, rwnmr::launch_reduce_rows(p.partials, rows, blocks, Mdev, c->stream));
-    else if (rows)
-        return fail(c, RWNMR_ERR_INVALID, "pfgse_run: no walkers");
+    RW_HIP(c, rwnmr::launch_reduce_rows(p.partials, mrows, blocks, Mdev, c->stream));
+    if (msd)
+        RW_HIP(c, rwnmr::launch_reduce_rows(p.partials + mrows * blocks, 3ull * n_delta, blocks, msd_dev, c->stream));
     if (!w->on_device) {
-        RW_HIP(c, hipMemcpyAsync(M, Mdev, 8ull * rows, hipMemcpyDeviceToHost, c->stream));
+        RW_HIP(c, hipMemcpyAsync(M, Mdev, 8ull * mrows, hipMemcpyDeviceToHost, c->stream));
+        if (msd)
+            RW_HIP(c, hipMemcpyAsync(msd, msd_dev, 24ull * n_delta, hipMemcpyDeviceToHost, c->stream));
         if ((s = stage_out(c, w, v)) != RWNMR_OK)
             return s;
         return sync_and_check(c);

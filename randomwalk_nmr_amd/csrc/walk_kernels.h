@@ -52,8 +52,10 @@ struct PfgseArgs {
     double voxel;                    // voxel size (phase = k . dr * voxel)
     uint32_t n_sets;                 // 0: per-walker delta; else uniform deltas below
     const double *set_delta;         // [n_sets]
-    double *partials;                // [(n_sets|1) * n_delta * n_k][partial_stride]
+    double *partials;                // [(n_sets|1) * n_delta * n_k (+ 3 n_delta)][partial_stride]
     uint32_t partial_stride;         // = grid blocks
+    uint64_t msd_rows;               // 0: no displacement sums; else first row of the
+                                     // [n_delta][3] squared-displacement sums
 };
 
 uint32_t walk_grid_blocks(uint64_t n, const LaunchShape &ls, int shift);

@@ -433,6 +433,26 @@ __global__ __launch_bounds__(1024) void pfgse_kernel(const WalkArgs a, const Pfg
                 __syncthreads();
             }
         }
+        if (p.msd_rows) {
+            // squared displacements per axis (voxel^2; integer-valued, so the
+            // FP64 sums are exact for any order)
+            const double d2[3] = {w.active ? dX * dX : 0.0, w.active ? dY * dY : 0.0, w.active ? dZ * dZ : 0.0};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const double v = wave_sum(d2[ax]);
+                if (lane == 0)
+                    lds[ax][wave] = v;
+            }
+            __syncthreads();
+            if (threadIdx.x < 3) {
+                double s = 0.0;
+                for (int qq = 0; qq < nw; ++qq)
+                    s += lds[threadIdx.x][qq];
+                const uint64_t row = p.msd_rows + (uint64_t)i * 3 + threadIdx.x;
+                p.partials[row * p.partial_stride + blockIdx.x] = s;
+            }
+            __syncthreads();
+        }
     }
     if (w.active) {
         a.x[id] = w.x;

@@ -118,6 +118,25 @@ def test_pfgse_matches_oracle(ctx, bc):
     assert np.allclose(g["energy"], o["energy"], rtol=1e-13, atol=0)
     M0 = o["M"][..., :1]
     assert np.all(np.abs(g["M"] - o["M"]) <= 1e-11 * np.abs(M0))
+    assert np.array_equal(g["msd"], o["msd"])  # integer-valued sums: exact
+
+
+def test_pfgse_k_set_per_diffusion_time(ctx):
+    """k_max scaled per diffusion time (k sqrt(D0 Delta) = const)."""
+    print("[test] pfgse k per delta", flush=True)
+    solid, seeds, x, y, z, _ = _case(32, 32, 32, 5, 0.35, 1200, 31)
+    img = R.Image.from_solid(ctx, solid)
+    steps_inc = np.array([4, 8, 16, 32], np.int64)
+    cum = np.cumsum(steps_inc)
+    kmax = 3.0 / np.sqrt(cum.astype(float))
+    kx = np.stack([np.linspace(0, km, 12) for km in kmax])
+    zeros = np.zeros_like(kx)
+    g = R.pfgse_run(ctx, img, x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=zeros, kz=zeros, voxel=1.0,
+                    set_delta=np.array([1.0, 0.99]))
+    o = O.pfgse(O.bitblock_pack(solid), (32, 32, 32), x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=zeros,
+                kz=zeros, voxel=1.0, set_delta=np.array([1.0, 0.99]))
+    assert np.all(np.abs(g["M"] - o["M"]) <= 1e-11 * np.abs(o["M"][..., :1]))
+    assert np.array_equal(g["msd"], o["msd"])
 
 
 def test_pfgse_energy_sets(ctx):
