@@ -1,0 +1,100 @@
+/*
+ * rwnmr_drivers.h -- C ABI of the CPMG and PFGSE sequence drivers (the
+ * host logic of NMR_cpmg::run_simulation and NMR_PFGSE::run around the GPU
+ * kernels).  Reference paths under src/backend/ of RandomWalk_NMR.
+ */
+#ifndef RWNMR_DRIVERS_H
+#define RWNMR_DRIVERS_H
+
+#include "rwnmr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CPMG sequence parameters (cpmg.config OBS_TIME / APPLY_BULK and
+ * rwnmr.config STEPS_PER_ECHO / D0 / BULK_TIME / RHO / BC; uct.config
+ * RESOLUTION / VOXEL_DIVISION give voxel_size and shift). */
+typedef struct {
+    double observation_time;   /* ms */
+    uint32_t steps_per_echo;
+    double voxel_size;         /* um (resolution / voxel division) */
+    double D0;                 /* um^2/ms */
+    double bulk_time;          /* ms */
+    int32_t apply_bulk;
+    double rho;                /* um/s, uniform relaxivity (RHO_TYPE uniform) */
+    const double *sigmoid;     /* nullable: 8 sigmoid parameters (RHO_TYPE sigmoid) */
+    int32_t bc;
+    uint32_t shift;
+} rwnmr_cpmg_params;
+
+/* NMR_cpmg::run_simulation (B/NMR_Simulation/NMR_cpmg.cpp:112-175, 346-385):
+ * time framework (echoes = ceil(round(T 6 D0 / eps^2) / steps_per_echo)),
+ * optional sigmoid relaxivity from a map pass, the GPU CPMG run from the
+ * walkers' current state, bulk factor exp(-t/T2b), normalisation by the
+ * t = 0 value.  times/signal receive echoes+1 points (capacity must be
+ * enough); uniform rho is used when w->penalty is NULL. */
+rwnmr_status rwnmr_cpmg_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w,
+                                 const rwnmr_cpmg_params *p, uint32_t capacity, double *times, double *signal,
+                                 uint32_t *n_points);
+
+/* PFGSE parameters (pfgse.config PULSE_WIDTH / APPLY_BULK / THRESHOLD_*,
+ * rwnmr.config D0 / BULK_TIME / RHO / STEPS_PER_ECHO / BC). */
+typedef struct {
+    double voxel_size;         /* um */
+    double D0;                 /* um^2/ms */
+    double bulk_time;          /* ms */
+    int32_t apply_bulk;
+    uint32_t steps_per_echo;
+    int32_t bc;
+    double pulse_width;        /* ms; enters RHS = -k^2 (Delta - pulse/3) */
+    double rho;                /* um/s (uniform), used when n_rho_sets == 0 */
+    const double *sigmoid;     /* nullable: 8 sigmoid parameters */
+    uint32_t n_rho_sets;       /* > 0: evaluate these relaxivities on one trajectory set */
+    const double *rho_sets;    /* [n_rho_sets] um/s */
+    int32_t k_per_delta;       /* k arrays are [n_times][n_k] instead of [n_k] */
+    uint32_t threshold_samples;/* THRESHOLD_TYPE samples: points in the fit (0 = all) */
+    double threshold_lhs;      /* THRESHOLD_TYPE lhs: value in (0,1), else unused */
+    uint32_t threshold_window;
+} rwnmr_pfgse_params;
+
+/* Caller-owned outputs, S = n_rho_sets (or 1), T = n_times, K = n_k. */
+typedef struct {
+    double *Mkt;        /* [S][T][K] M(k, Delta) after bulk */
+    double *rhs;        /* [S][T][K] -k^2 (Delta - pulse/3) */
+    double *lhs;        /* [S][T][K] log(max(M_k/M_0, 1e-6)) */
+    double *D_sat;      /* [S][T] Stejskal-Tanner D */
+    double *D_sat_err;  /* [S][T] its standard error */
+    uint32_t *points;   /* [S][T] points used in the fit */
+    double *msd;        /* [T][3] sums of squared displacements, voxel^2 */
+    double *D_msd;      /* [T] msd / (6 Delta) */
+    uint64_t *steps;    /* [T] total steps at each Delta */
+} rwnmr_pfgse_result;
+
+/* NMR_PFGSE::run (B/NMR_Simulation/NMR_pfgse.cpp:98-131, 272-361, 441-692,
+ * 1098-1160) on host walkers: steps per Delta from the time framework,
+ * one GPU pass over all Delta and k (rwnmr_pfgse_run), then bulk, RHS,
+ * LHS, threshold, Stejskal-Tanner D and msd D per Delta.  Exposure times
+ * must be ascending. */
+rwnmr_status rwnmr_pfgse_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w,
+                                  const rwnmr_pfgse_params *p, const double *exposure_times, uint32_t n_times,
+                                  const double *kx, const double *ky, const double *kz, uint32_t n_k,
+                                  rwnmr_pfgse_result *out);
+
+/* Gradient list 0..Gmax with accumulated steps (setGradientVector,
+ * NMR_pfgse.cpp:272-297). */
+void rwnmr_pfgse_gradients(double gmax_x, double gmax_y, double gmax_z, uint32_t points, double *gx, double *gy,
+                           double *gz);
+/* k = (pulse 1e-3)(gamma 1e6)(G 1e-8) (computeWaveVectorK, NMR_pfgse.cpp:483-486). */
+double rwnmr_pfgse_wave_vector(double gradient, double pulse_width, double gamma);
+/* No-intercept least squares slope and its standard error
+ * (LeastSquareAdjust.cpp:77-170; NMR_pfgse.cpp:678-682). */
+int32_t rwnmr_stejskal_tanner(const double *rhs, const double *lhs, uint32_t n, double *D, double *D_err);
+/* Points kept by the LHS-value threshold (setThresholdFromLHSValue, NMR_pfgse.cpp:399-419). */
+uint32_t rwnmr_pfgse_threshold_lhs(const double *lhs, uint32_t n, double value, uint32_t window);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RWNMR_DRIVERS_H */

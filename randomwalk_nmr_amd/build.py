@@ -17,7 +17,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["walk_kernels.hip", "batch_kernels.hip", "backward_kernels.hip", "capi.hip",
+HIP_SOURCES = ["walk_kernels.hip", "batch_kernels.hip", "backward_kernels.hip", "capi.hip", "drivers.hip",
                "capi_batch.hip"]
 CXX_SOURCES = ["laplace.cpp", "host_sim.cpp"]
 

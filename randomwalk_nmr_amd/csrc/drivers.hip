@@ -1,0 +1,288 @@
+// drivers.hip -- host drivers of the CPMG and PFGSE sequences over the C
+// ABI: time framework, relaxivity, the GPU run, bulk relaxation,
+// normalisation, wave vectors, Stejskal-Tanner and msd diffusion
+// coefficients.  Reference paths under src/backend/ of RandomWalk_NMR.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/rwnmr/rwnmr.h"
+#include "../../include/rwnmr/rwnmr_drivers.h"
+#include "../../include/rwnmr/rwnmr_host.h"
+
+namespace {
+
+// Two-sigmoid relaxivity of a walker's collision rate xi
+// (Walker::setSurfaceRelaxivity, B/Walker/walker.h:174-196).
+double sigmoid_rho(const double *p, double xi)
+{
+    const double s1 = p[1] + (p[0] - p[1]) / (1.0 + exp(-p[3] * (xi - p[2])));
+    const double s2 = p[5] + (p[4] - p[5]) / (1.0 + exp(-p[7] * (xi - p[6])));
+    return s1 + s2;
+}
+
+// Per-walker deltas from a sigmoid relaxivity: map the walkers over `steps`
+// steps (NMR_Simulation::mapSimulation), xi = hits / steps
+// (Walker::updateXIrate, walker.h:162-172), then delta
+// (updateWalkersRelaxativity, NMR_Simulation.cpp:1500-1509).  The walkers
+// themselves are not moved (the map runs on copies).
+rwnmr_status sigmoid_penalties(rwnmr_ctx *ctx, const rwnmr_image *img, const rwnmr_walkers *w, uint64_t steps,
+                               int32_t bc, uint32_t shift, const double *sig, double voxel, double D0,
+                               std::vector<double> &pen)
+{
+    const uint64_t n = w->n;
+    std::vector<int32_t> x(n), y(n), z(n);
+    std::vector<uint64_t> s(n);
+    std::vector<uint32_t> h(n, 0u);
+    const hipMemcpyKind k = w->on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+    if (hipMemcpy(x.data(), w->x, n * 4, k) != hipSuccess || hipMemcpy(y.data(), w->y, n * 4, k) != hipSuccess ||
+        hipMemcpy(z.data(), w->z, n * 4, k) != hipSuccess || hipMemcpy(s.data(), w->seed, n * 8, k) != hipSuccess)
+        return RWNMR_ERR_HIP;
+    rwnmr_walkers m{};
+    m.x = x.data();
+    m.y = y.data();
+    m.z = z.data();
+    m.seed = s.data();
+    m.hits = h.data();
+    m.n = n;
+    m.on_device = 0;
+    rwnmr_status st = rwnmr_map_run(ctx, img, &m, steps, bc, shift);
+    if (st != RWNMR_OK)
+        return st;
+    pen.resize(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const double xi = (double)h[i] / (double)steps;
+        pen[i] = rwnmr_penalty(sigmoid_rho(sig, xi), voxel, D0);
+    }
+    return RWNMR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rwnmr_status rwnmr_cpmg_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w,
+                                 const rwnmr_cpmg_params *p, uint32_t capacity, double *times, double *signal,
+                                 uint32_t *n_points)
+{
+    if (!ctx || !img || !w || !p || !times || !signal || !n_points)
+        return RWNMR_ERR_INVALID;
+    uint32_t echoes = 0;
+    uint64_t steps = 0;
+    rwnmr_status st = rwnmr_time_framework(p->observation_time, p->voxel_size, p->D0, p->steps_per_echo, &echoes,
+                                           &steps);
+    if (st != RWNMR_OK)
+        return st;
+    if (capacity < echoes + 1)
+        return RWNMR_ERR_INVALID;
+    rwnmr_walkers run = *w;
+    std::vector<double> pen;
+    if (p->sigmoid) {
+        st = sigmoid_penalties(ctx, img, w, steps, p->bc, p->shift, p->sigmoid, p->voxel_size, p->D0, pen);
+        if (st != RWNMR_OK)
+            return st;
+        if (w->on_device)
+            return RWNMR_ERR_INVALID;  // per-walker sigmoid deltas: host walkers only
+        run.penalty = pen.data();
+    } else if (!w->penalty) {
+        run.delta_uniform = rwnmr_penalty(p->rho, p->voxel_size, p->D0);
+    }
+    // signal_times: t accumulates += steps_per_echo * dt (NMR_cpmg.cpp:112-130)
+    const double dt = rwnmr_time_step(p->voxel_size, p->D0);
+    const double step_t = (double)p->steps_per_echo * dt;
+    double t = 0.0;
+    times[0] = 0.0;
+    for (uint32_t e = 1; e <= echoes; ++e) {
+        t += step_t;
+        times[e] = t;
+    }
+    if (w->on_device) {
+        double *dsums = nullptr;
+        if (hipMalloc((void **)&dsums, 8ull * (echoes + 1)) != hipSuccess)
+            return RWNMR_ERR_ALLOC;
+        st = rwnmr_cpmg_run(ctx, img, &run, echoes, p->steps_per_echo, p->bc, p->shift, dsums);
+        if (st == RWNMR_OK)
+            st = rwnmr_ctx_synchronize(ctx);
+        if (st == RWNMR_OK && hipMemcpy(signal, dsums, 8ull * (echoes + 1), hipMemcpyDeviceToHost) != hipSuccess)
+            st = RWNMR_ERR_HIP;
+        (void)hipFree(dsums);
+    } else {
+        st = rwnmr_cpmg_run(ctx, img, &run, echoes, p->steps_per_echo, p->bc, p->shift, signal);
+    }
+    if (st != RWNMR_OK)
+        return st;
+    // applyBulk (NMR_cpmg.cpp:346-359) and normalizeSignal (:369-385)
+    if (p->apply_bulk) {
+        const double bulk = -1.0 / p->bulk_time;
+        for (uint32_t e = 0; e <= echoes; ++e)
+            signal[e] = exp(bulk * times[e]) * signal[e];
+    }
+    const double norm = 1.0 / signal[0];
+    for (uint32_t e = 0; e <= echoes; ++e)
+        signal[e] = norm * signal[e];
+    *n_points = echoes + 1;
+    return RWNMR_OK;
+}
+
+void rwnmr_pfgse_gradients(double gmax_x, double gmax_y, double gmax_z, uint32_t points, double *gx, double *gy,
+                           double *gz)
+{
+    // setGradientVector (NMR_pfgse.cpp:272-297): 0 .. Gmax, value += gap
+    const double ax = gmax_x / (double)(points - 1), ay = gmax_y / (double)(points - 1),
+                 az = gmax_z / (double)(points - 1);
+    double vx = 0, vy = 0, vz = 0;
+    for (uint32_t i = 0; i < points; ++i) {
+        gx[i] = vx;
+        gy[i] = vy;
+        gz[i] = vz;
+        vx += ax;
+        vy += ay;
+        vz += az;
+    }
+}
+
+double rwnmr_pfgse_wave_vector(double gradient, double pulse_width, double gamma)
+{
+    // computeWaveVectorK (NMR_pfgse.cpp:483-486)
+    return (pulse_width * 1.0e-03) * (gamma * 1.0e+06) * (gradient * 1.0e-08);
+}
+
+int32_t rwnmr_stejskal_tanner(const double *rhs, const double *lhs, uint32_t n, double *D, double *D_err)
+{
+    // no-intercept least squares (LeastSquareAdjust.cpp:77-170) and
+    // err = sqrt(MSE n / (n - 1)) (NMR_pfgse.cpp:678-682)
+    if (n < 2 || !rhs || !lhs || !D || !D_err)
+        return RWNMR_ERR_INVALID;
+    double num = 0, den = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        num += rhs[i] * lhs[i];
+        den += rhs[i] * rhs[i];
+    }
+    const double B = num / den;
+    double res = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        res += pow(lhs[i] - B * rhs[i], 2.0);
+    res /= (double)n;
+    *D = B;
+    *D_err = sqrt(res * ((double)n / ((double)n - 1.0)));
+    return RWNMR_OK;
+}
+
+uint32_t rwnmr_pfgse_threshold_lhs(const double *lhs, uint32_t n, double value, uint32_t window)
+{
+    // setThresholdFromLHSValue (NMR_pfgse.cpp:399-419): the first point
+    // after `window` whose LHS falls below log(value)
+    if (n < window || window == 0)
+        return n;
+    int idx = (int)window - 1;
+    const double lv = log(value);
+    bool greater = true;
+    while (idx < (int)n && greater) {
+        if (lhs[idx] < lv)
+            greater = false;
+        else
+            ++idx;
+    }
+    if (greater)
+        --idx;
+    return (uint32_t)idx;
+}
+
+rwnmr_status rwnmr_pfgse_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w,
+                                  const rwnmr_pfgse_params *p, const double *exposure_times, uint32_t n_times,
+                                  const double *kx, const double *ky, const double *kz, uint32_t n_k,
+                                  rwnmr_pfgse_result *out)
+{
+    if (!ctx || !img || !w || !p || !exposure_times || !kx || !ky || !kz || !out || n_k < 2 || n_times == 0)
+        return RWNMR_ERR_INVALID;
+    if (w->on_device)
+        return RWNMR_ERR_INVALID;  // host walkers (the driver post-processes on the host)
+    const uint64_t n = w->n;
+    // steps per diffusion time: setTimeFramework(Delta) (NMR_Simulation.cpp:293-298),
+    // advanced incrementally (NMR_pfgse_cuda.cpp:563-565)
+    std::vector<uint32_t> inc(n_times);
+    uint64_t taken = 0;
+    for (uint32_t i = 0; i < n_times; ++i) {
+        uint32_t echoes;
+        uint64_t steps;
+        rwnmr_status st = rwnmr_time_framework(exposure_times[i], p->voxel_size, p->D0, p->steps_per_echo, &echoes,
+                                               &steps);
+        if (st != RWNMR_OK)
+            return st;
+        if (steps < taken)
+            return RWNMR_ERR_INVALID;  // exposure times must be ascending
+        inc[i] = (uint32_t)(steps - taken);
+        taken = steps;
+        out->steps[i] = steps;
+    }
+    std::vector<int32_t> x0(w->x, w->x + n), y0(w->y, w->y + n), z0(w->z, w->z + n);
+    rwnmr_walkers run = *w;
+    std::vector<double> pen;
+    if (p->sigmoid) {
+        // initial map over the longest time (runInitialMapSimulation, NMR_pfgse.cpp:322-348)
+        rwnmr_status st = sigmoid_penalties(ctx, img, w, taken, p->bc, 0, p->sigmoid, p->voxel_size, p->D0, pen);
+        if (st != RWNMR_OK)
+            return st;
+        run.penalty = pen.data();
+    } else if (!w->penalty) {
+        run.delta_uniform = rwnmr_penalty(p->rho, p->voxel_size, p->D0);
+    }
+    const uint32_t ns = p->n_rho_sets;
+    std::vector<double> sets(ns);
+    for (uint32_t q = 0; q < ns; ++q)
+        sets[q] = rwnmr_penalty(p->rho_sets[q], p->voxel_size, p->D0);
+    const uint32_t nsets = ns ? ns : 1;
+    rwnmr_status st = rwnmr_pfgse_run(ctx, img, &run, x0.data(), y0.data(), z0.data(), inc.data(), n_times, kx, ky,
+                                      kz, n_k, p->k_per_delta, p->voxel_size, p->bc, 0, ns, ns ? sets.data() : nullptr,
+                                      out->Mkt, out->msd);
+    if (st != RWNMR_OK)
+        return st;
+    for (uint32_t q = 0; q < nsets; ++q)
+        for (uint32_t i = 0; i < n_times; ++i) {
+            const double delta_t = exposure_times[i];
+            double *M = out->Mkt + ((size_t)q * n_times + i) * n_k;
+            // applyBulk (NMR_pfgse.cpp:519-529)
+            if (p->apply_bulk) {
+                const double bm = exp(-1.0 / p->bulk_time * delta_t);
+                for (uint32_t k = 0; k < n_k; ++k)
+                    M[k] *= bm;
+            }
+            double *rhs = out->rhs + ((size_t)q * n_times + i) * n_k;
+            double *lhs = out->lhs + ((size_t)q * n_times + i) * n_k;
+            const double *kxi = kx + (p->k_per_delta ? (size_t)i * n_k : 0);
+            const double *kyi = ky + (p->k_per_delta ? (size_t)i * n_k : 0);
+            const double *kzi = kz + (p->k_per_delta ? (size_t)i * n_k : 0);
+            // RHS = -|k|^2 (Delta - pulse/3) (computeRHS :450-453), LHS =
+            // log(max(M_k/M_0, 1e-6)) of the normalised signal (computeLHS
+            // :474-481, recoverDsatWithoutSampling :641-692)
+            const double M0 = M[0];
+            for (uint32_t k = 0; k < n_k; ++k) {
+                const double kn = sqrt(kxi[k] * kxi[k] + kyi[k] * kyi[k] + kzi[k] * kzi[k]);
+                rhs[k] = -1.0 * kn * kn * (delta_t - (p->pulse_width / 3.0));
+                const double ratio = (M[k] / M0) / (M[0] / M0);
+                lhs[k] = ratio > 1.0e-06 ? log(ratio) : log(1.0e-06);
+            }
+            uint32_t used = n_k;
+            if (p->threshold_lhs > 0.0 && p->threshold_lhs < 1.0)
+                used = rwnmr_pfgse_threshold_lhs(lhs, n_k, p->threshold_lhs, p->threshold_window);
+            else if (p->threshold_samples > 1 && p->threshold_samples <= n_k)
+                used = p->threshold_samples;
+            out->points[(size_t)q * n_times + i] = used;
+            st = rwnmr_stejskal_tanner(rhs, lhs, used, &out->D_sat[(size_t)q * n_times + i],
+                                       &out->D_sat_err[(size_t)q * n_times + i]);
+            if (st != RWNMR_OK)
+                return st;
+        }
+    // D_msd = eps^2 sum |dr|^2 / N / (6 Delta) (recoverDmsdWithoutSampling :1098-1160)
+    for (uint32_t i = 0; i < n_times; ++i) {
+        const double v2 = p->voxel_size * p->voxel_size;
+        const double s2 = (out->msd[3 * i] + out->msd[3 * i + 1] + out->msd[3 * i + 2]) * v2 / (double)n;
+        out->D_msd[i] = s2 / (6.0 * exposure_times[i]);
+    }
+    return RWNMR_OK;
+}
+
+}  // extern "C"
