@@ -42,6 +42,26 @@ class CpmgInstance(ct.Structure):
                 ("bc", I32), ("echo_sums", P)]
 
 
+class CpmgParams(ct.Structure):
+    """rwnmr_cpmg_params (include/rwnmr/rwnmr_drivers.h)."""
+    _fields_ = [("observation_time", D), ("steps_per_echo", U32), ("voxel_size", D), ("D0", D),
+                ("bulk_time", D), ("apply_bulk", I32), ("rho", D), ("sigmoid", P), ("bc", I32), ("shift", U32)]
+
+
+class PfgseParams(ct.Structure):
+    """rwnmr_pfgse_params (include/rwnmr/rwnmr_drivers.h)."""
+    _fields_ = [("voxel_size", D), ("D0", D), ("bulk_time", D), ("apply_bulk", I32), ("steps_per_echo", U32),
+                ("bc", I32), ("pulse_width", D), ("rho", D), ("sigmoid", P), ("n_rho_sets", U32),
+                ("rho_sets", P), ("k_per_delta", I32), ("threshold_samples", U32), ("threshold_lhs", D),
+                ("threshold_window", U32)]
+
+
+class PfgseResult(ct.Structure):
+    """rwnmr_pfgse_result (include/rwnmr/rwnmr_drivers.h)."""
+    _fields_ = [("Mkt", P), ("rhs", P), ("lhs", P), ("D_sat", P), ("D_sat_err", P), ("points", P), ("msd", P),
+                ("D_msd", P), ("steps", P)]
+
+
 # (name, restype, argtypes) of every exported symbol the package binds.
 SIGNATURES = {
     "rwnmr_version": (ct.c_char_p, []),
@@ -68,6 +88,15 @@ SIGNATURES = {
     "rwnmr_cpmg_forward_ckpt": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, P, U32, P, P]),
     "rwnmr_cpmg_backward": (I32, [P, P, P, U64, U32, U32, I32, P, U32, P, P, P]),
     "rwnmr_ckpt_bytes": (U64, [U64, U32, U32]),
+    # sequence drivers (include/rwnmr/rwnmr_drivers.h)
+    "rwnmr_cpmg_simulate": (I32, [P, P, ct.POINTER(Walkers), ct.POINTER(CpmgParams), U32, P, P,
+                                  ct.POINTER(U32)]),
+    "rwnmr_pfgse_simulate": (I32, [P, P, ct.POINTER(Walkers), ct.POINTER(PfgseParams), P, U32, P, P, P, U32,
+                                   ct.POINTER(PfgseResult)]),
+    "rwnmr_pfgse_gradients": (None, [D, D, D, U32, P, P, P]),
+    "rwnmr_pfgse_wave_vector": (D, [D, D, D]),
+    "rwnmr_stejskal_tanner": (I32, [P, P, U32, ct.POINTER(D), ct.POINTER(D)]),
+    "rwnmr_pfgse_threshold_lhs": (U32, [P, U32, D, U32]),
     # host set-up (include/rwnmr/rwnmr_host.h)
     "rwnmr_synth_sphere_pack": (I32, [I32, I32, I32, D, D, D, U64, P, ct.POINTER(D)]),
     "rwnmr_synth_voronoi_labels": (I32, [I32, I32, I32, I32, I32, U64, P, P]),

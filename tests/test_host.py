@@ -1,0 +1,208 @@
+"""Host-side logic of the product library on the CPU (no GPU): C ABI
+exports, bitBlock packing, reference seeding/placement, time framework,
+Laplace inversion vs the scipy oracle, Stejskal-Tanner and PFGSE helpers.
+"""
+import ctypes as ct
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import randomwalk_nmr_amd as R
+from oracle import laplace as OL
+from randomwalk_nmr_amd import laplace as RL
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include", "rwnmr")
+    for f in os.listdir(inc):
+        text = open(os.path.join(inc, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            prev = text[:m.start()].rstrip()
+            if name in ("if", "sizeof", "defined") or prev.endswith("#define"):
+                continue
+            # a declaration: return type token right before the name
+            if re.search(r"(void|int|int32_t|int64_t|uint32_t|uint64_t|double|bool|rwnmr_status|char)\s*\**$",
+                         prev):
+                names.add(name)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = R.load()
+    declared = _declared_functions()
+    assert len(declared) > 40
+    missing = [n for n in sorted(declared) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_ctx_create_without_gpu_fails_cleanly():
+    from conftest import HAS_GPU
+    if HAS_GPU:
+        pytest.skip("GPU present")
+    with pytest.raises(R.RwnmrError):
+        R.Context(0)
+
+
+def test_pack_matches_oracle_and_fixtures():
+    data = np.load(os.path.join(ROOT, "tests", "golden", "images.npz"))
+    for key in ("tiny_3D", "sphere_r2_5", "antisphere_r2_5"):
+        assert np.array_equal(R.pack_bitblocks(data[f"{key}_solid"]), data[f"{key}_blocks"])
+    rng = np.random.default_rng(1)
+    s = (rng.random((11, 6, 13)) < 0.5).astype(np.uint8)
+    assert np.array_equal(R.pack_bitblocks(s), O.bitblock_pack(s))
+
+
+@pytest.mark.parametrize("seed,n", [(1, 1), (12345, 5000), (2**40 + 7, 777)])
+def test_seed_and_place_matches_oracle(seed, n):
+    solid, _ = R.sphere_pack(40, 30, 20, 3, 5, 0.3, seed)
+    s1, x1, y1, z1 = R.seed_and_place(solid, seed, n)
+    s2, x2, y2, z2 = O.seed_and_place(seed, n, O.pore_list(solid), solid.size)
+    assert np.array_equal(s1, s2) and np.array_equal(x1, x2) and np.array_equal(y1, y2) and np.array_equal(z1, z2)
+    assert np.all(solid[z1, y1, x1] == 0)
+
+
+def test_sphere_pack_reaches_porosity():
+    for phi in (0.2, 0.3):
+        solid, got = R.sphere_pack(64, 64, 64, 4, 12, phi, 7)
+        assert got <= phi and phi - got < 0.02
+        assert abs(1.0 - solid.mean() - got) < 1e-12
+
+
+def test_time_framework_and_penalty():
+    # cfg1: eps 1, D0 2.5 -> dt 1/15 ms; 400 ms at 3 steps/echo -> 2000 echoes
+    assert R.time_framework(400.0, 1.0, 2.5, 3) == (2000, 6000)
+    # cfg2: eps 0.5 -> dt 1/60 ms; TE 0.2 ms = 12 steps
+    assert abs(R.time_step(0.5, 2.5) - 1.0 / 60.0) < 1e-15
+    assert R.time_framework(2000.0, 0.5, 2.5, 12) == (10000, 120000)
+    # rounding and the at-least-one-echo rule
+    assert R.time_framework(0.01, 1.0, 2.5, 3) == (1, 3)
+    assert R.time_framework(1.0, 1.0, 2.5, 4) == (4, 16)  # round(15) = 15 -> ceil(15/4) = 4
+    assert R.penalty(10.0, 1.0, 2.5) == 1.0 - (2.0 / 3.0) * ((10.0 * 1.0) / (2.5 * 1e03))
+
+
+def test_nnls_matches_scipy():
+    rng = np.random.default_rng(0)
+    for t in range(30):
+        m, n = int(rng.integers(5, 60)), int(rng.integers(2, 30))
+        A = rng.random((m, n))
+        b = rng.standard_normal(m) if t % 3 else rng.random(m)
+        lam = [0.0, 0.1, 1e-3][t % 3]
+        x = RL.nnls(A, b, lam, tol=1e-12)
+        xs = OL.nnls_tikhonov(A, b, lam)
+        assert np.max(np.abs(x - xs)) <= 1e-8 * max(1.0, np.max(np.abs(xs)))
+
+
+def test_nnls_kkt_brute_force():
+    """Tiny problems: the oracle's answer equals the best feasible
+    least-squares solution over every active set (brute force)."""
+    import itertools
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        A = rng.standard_normal((6, 4))
+        b = rng.standard_normal(6)
+        best, bestx = np.inf, None
+        for r in range(0, 5):
+            for S in itertools.combinations(range(4), r):
+                x = np.zeros(4)
+                if S:
+                    sol, *_ = np.linalg.lstsq(A[:, S], b, rcond=None)
+                    if np.any(sol < 0):
+                        continue
+                    x[list(S)] = sol
+                res = np.sum((A @ x - b) ** 2)
+                if res < best:
+                    best, bestx = res, x
+        assert np.allclose(OL.nnls_tikhonov(A, b, 0.0), bestx, atol=1e-9)
+        assert np.allclose(RL.nnls(A, b), bestx, atol=1e-9)
+
+
+def _decay(times):
+    return 0.35 * np.exp(-times / 40.0) + 0.65 * np.exp(-times / 700.0)
+
+
+def test_t2_bins_and_pruning_match_oracle():
+    times = np.arange(1, 2001) * (3.0 / 15.0)
+    inv = RL.Inverter(times)
+    assert np.allclose(inv.t2_bins, OL.t2_bins(0.1, 1e4, 128), rtol=1e-15, atol=0)
+    start, sl = OL.pruning_slices(times, 512)
+    assert np.allclose(inv.used_raw_bins, OL.prune(times, 512, start, sl, True), rtol=1e-14, atol=0)
+    d = _decay(times)
+    assert np.allclose(inv.processed(d), OL.prune(d, 512, start, sl, False), rtol=1e-15, atol=0)
+
+
+def test_inversion_fixed_lambda_matches_oracle():
+    times = np.arange(1, 2001) * 0.2
+    d = _decay(times)
+    ref = OL.invert_decay(times, d, lam=1e-2)
+    bins, amps, lam = RL.apply_laplace(np.concatenate([[0.0], times]), np.concatenate([[1.0], d]), lam=1e-2)
+    assert lam == 1e-2
+    assert np.max(np.abs(amps - ref["t2_amps"])) <= 1e-9 * np.max(ref["t2_amps"])
+    # the recovered distribution puts its mass near the two input T2s
+    w = amps / amps.sum()
+    mean_log = np.sum(w * np.log10(bins))
+    assert abs(mean_log - (0.35 * np.log10(40) + 0.65 * np.log10(700))) < 0.15
+
+
+def test_lcurve_matches_oracle_and_finite_difference_curvature():
+    times = np.arange(1, 1001) * 0.5
+    d = _decay(times) + 1e-4 * np.sin(times)
+    inv = RL.Inverter(times, num_lambdas=256)
+    lam = inv.find_best_lambda(d)
+    K = OL.kernel_matrix(inv.used_raw_bins, inv.t2_bins)
+    lc = OL.lcurve(K, inv.processed(d), num_lambdas=256)
+    assert lam == lc["best_lambda"]
+    got = inv.lcurve()
+    # singular vectors of the ~1e-14 singular-value cluster are not unique, so
+    # the smallest-lambda residuals depend on the SVD implementation; the
+    # curvature agrees to a small fraction of its peak everywhere
+    scale = np.max(np.abs(lc["curvature"]))
+    assert np.max(np.abs(got["curvature"] - lc["curvature"])) <= 1e-5 * scale
+    assert np.allclose(got["solution_norms"], lc["solution_norms"], rtol=1e-9)
+    # the curvature's inputs are pinned by explicit Tikhonov solves:
+    # |x_lam| and |K x_lam - b| (less the part of b outside range(K)'s
+    # left singular space) from the augmented least-squares problem
+    bb = inv.processed(d)
+    U, s, _ = np.linalg.svd(K, full_matrices=False)
+    b_perp2 = np.sum(bb ** 2) - np.sum((U.T @ bb) ** 2)
+    for il in (60, 128, 200):
+        lamv = lc["lambdas"][il]
+        A = np.vstack([K, lamv * np.eye(K.shape[1])])
+        xl, *_ = np.linalg.lstsq(A, np.concatenate([bb, np.zeros(K.shape[1])]), rcond=None)
+        assert np.isclose(np.linalg.norm(xl), lc["solution_norms"][il], rtol=1e-6)
+        r2 = np.sum((K @ xl - bb) ** 2) - b_perp2
+        assert np.isclose(np.sqrt(max(r2, 0.0)), lc["residual_norms"][il], rtol=1e-4)
+
+
+def test_stejskal_tanner_hand_vectors():
+    L = R.load()
+    x = np.array([0.0, -1.0, -2.0, -3.0, -4.5])
+    y = 2.5 * x
+    D, E = ct.c_double(), ct.c_double()
+    assert L.rwnmr_stejskal_tanner(x.ctypes.data_as(ct.c_void_p), y.ctypes.data_as(ct.c_void_p), 5, ct.byref(D),
+                                   ct.byref(E)) == 0
+    assert D.value == 2.5 and E.value == 0.0
+    y2 = y + np.array([0.1, -0.2, 0.05, 0.0, 0.3])
+    L.rwnmr_stejskal_tanner(x.ctypes.data_as(ct.c_void_p), y2.ctypes.data_as(ct.c_void_p), 5, ct.byref(D), ct.byref(E))
+    b, e = OL.stejskal_tanner(x, y2)
+    assert math.isclose(D.value, b, rel_tol=1e-15) and math.isclose(E.value, e, rel_tol=1e-14)
+
+
+def test_pfgse_gradient_list_and_k():
+    L = R.load()
+    g = [np.zeros(10) for _ in range(3)]
+    L.rwnmr_pfgse_gradients(3000.0, 0.0, 0.0, 10, *(a.ctypes.data_as(ct.c_void_p) for a in g))
+    acc, ref = 0.0, []
+    for _ in range(10):
+        ref.append(acc)
+        acc += 3000.0 / 9.0
+    assert np.array_equal(g[0], np.array(ref)) and not g[1].any()
+    assert L.rwnmr_pfgse_wave_vector(100.0, 0.1, 42.576) == (0.1 * 1e-3) * (42.576 * 1e6) * (100.0 * 1e-8)
