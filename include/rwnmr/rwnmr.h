@@ -142,6 +142,27 @@ rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr
                                    uint32_t echoes, uint32_t steps_per_echo, int32_t bc, uint32_t shift,
                                    const double *label_delta, uint32_t *label_hits, double *echo_sums);
 
+/* One CPMG instance of a batch: its image, device-resident walkers
+ * (on_device must be 1; per-walker penalty or delta_uniform), echo count,
+ * steps per echo, boundary, and a device echo_sums array of echoes+1. */
+typedef struct {
+    const rwnmr_image *image;
+    rwnmr_walkers walkers;
+    uint32_t echoes;
+    uint32_t steps_per_echo;
+    int32_t bc;
+    double *echo_sums;
+} rwnmr_cpmg_instance;
+
+/* Many CPMG instances (uneven images, walker counts, relaxivities, echo
+ * counts) in one persistent launch fed by an atomic tile queue, longest
+ * tiles first.  flags = 1 runs the instances one launch each instead (the
+ * comparison path).  Results equal n_inst separate rwnmr_cpmg_run calls
+ * bit for bit in the walkers; echo sums agree to rounding.  Asynchronous on
+ * the context stream.  No reference counterpart: the reference simulates
+ * one sample per process (B/NMR_Simulation/NMR_cpmg_cuda.cpp:929-1774). */
+rwnmr_status rwnmr_cpmg_batch_run(rwnmr_ctx *ctx, const rwnmr_cpmg_instance *inst, uint32_t n_inst, uint32_t flags);
+
 /* PFGSE, reference GPU semantics (NMR_PFGSE::simulation_cuda,
  * B/NMR_Simulation/NMR_pfgse_cuda.cpp:536-1348): for each diffusion time i
  * in order, walk steps_inc[i] more steps (PFG_map_*, :34-339), multiply the

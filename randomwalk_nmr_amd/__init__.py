@@ -4,7 +4,7 @@ relaxivity fit).  Compute runs in librwnmr_amd.so (HIP kernels for gfx950
 behind the C ABI in include/rwnmr/); this package is the Python host mirror.
 """
 from ._lib import RwnmrError, load  # noqa: F401
-from .core import (BC, Context, Image, bc_code, bitblock_dims, cpmg_run, map_run,  # noqa: F401
+from .core import (BC, Context, Image, bc_code, bitblock_dims, cpmg_batch_run, cpmg_run, map_run,  # noqa: F401
                    pack_bitblocks, penalty, pfgse_run, seed_and_place, sphere_pack, time_framework,
                    time_step, voronoi_labels)
 

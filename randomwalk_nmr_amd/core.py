@@ -356,3 +356,25 @@ def pfgse_run(ctx: Context, img: Image, x, y, z, seed, *, steps_inc, kx, ky, kz,
                              _ptr(sd), _ptr(M), _ptr(msd))
     check(code, ctx.handle)
     return dict(M=M, msd=msd, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
+
+
+def cpmg_batch_run(ctx: Context, instances, *, sequential=False, sync=True):
+    """Many CPMG instances in one persistent launch (rwnmr_cpmg_batch_run).
+
+    instances: list of dicts with keys img, x, y, z, seed (torch CUDA
+    tensors, updated in place), echo_sums (CUDA float64, echoes+1), echoes,
+    steps_per_echo, bc, and optionally energy, hits, penalty, delta.
+    sequential=True runs one launch per instance (the comparison path)."""
+    _torch_sync()
+    arr = (_lib.CpmgInstance * len(instances))()
+    for i, q in enumerate(instances):
+        arr[i].image = q["img"].handle
+        arr[i].walkers = _walker_struct(q["x"], q["y"], q["z"], q["seed"], q.get("energy"), q.get("hits"),
+                                        q.get("penalty"), q.get("delta", 1.0), len(q["x"]), 1)
+        arr[i].echoes = int(q["echoes"])
+        arr[i].steps_per_echo = int(q["steps_per_echo"])
+        arr[i].bc = bc_code(q.get("bc", "mirror"))
+        arr[i].echo_sums = _ptr(q["echo_sums"])
+    check(load().rwnmr_cpmg_batch_run(ctx.handle, arr, len(instances), 1 if sequential else 0), ctx.handle)
+    if sync:
+        ctx.synchronize()

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/rwnmr/rwnmr.h"
+#include "batch_kernels.h"
 #include "walk_kernels.h"
 
 using rwnmr::LaunchShape;
@@ -74,6 +75,8 @@ struct rwnmr_ctx {
     DevBuf x, y, z, seed, energy, hits, penalty, x0, y0, z0;
     // per-call small arrays and reductions
     DevBuf partials, sums, label_delta, label_hits, steps_inc, kvec, set_delta;
+    // batched CPMG: descriptors, tile list, reduction rows, queue head
+    DevBuf batch_inst, batch_tiles, batch_rows, batch_counter;
     DevBuf flag;  // uint32 error word
     uint32_t *host_flag = nullptr;  // pinned readback of flag
     // optional HIP-event timing of the walk / PFGSE kernels on this stream
@@ -461,7 +464,8 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf *b : {&c->x, &c->y, &c->z, &c->seed, &c->energy, &c->hits, &c->penalty, &c->x0, &c->y0, &c->z0,
                       &c->partials, &c->sums, &c->label_delta, &c->label_hits, &c->steps_inc, &c->kvec,
-                      &c->set_delta, &c->flag})
+                      &c->set_delta, &c->flag, &c->batch_inst, &c->batch_tiles, &c->batch_rows,
+                      &c->batch_counter})
         b->release();
     if (c->host_flag)
         (void)hipHostFree(c->host_flag);
@@ -888,3 +892,126 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
 }
 
 }  // extern "C"
+
+extern "C" rwnmr_status rwnmr_cpmg_batch_run(rwnmr_ctx *c, const rwnmr_cpmg_instance *inst, uint32_t n_inst,
+                                             uint32_t flags)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if (!inst || n_inst == 0)
+        return fail(c, RWNMR_ERR_INVALID, "batch_run: no instances");
+    if (flags > 1)
+        return fail(c, RWNMR_ERR_INVALID, "batch_run: unknown flags");
+    for (uint32_t i = 0; i < n_inst; ++i) {
+        const rwnmr_cpmg_instance &q = inst[i];
+        rwnmr_status s = check_image(c, q.image);
+        if (s != RWNMR_OK)
+            return s;
+        if ((s = check_walkers(c, q.image, &q.walkers, q.bc, 0)) != RWNMR_OK)
+            return s;
+        if (!q.walkers.on_device || !q.echo_sums)
+            return fail(c, RWNMR_ERR_INVALID, "batch_run: walkers and echo_sums must be device arrays");
+        if (q.walkers.n == 0)
+            return fail(c, RWNMR_ERR_INVALID, "batch_run: instance without walkers");
+    }
+    if (flags & 1u) {
+        // one launch per instance (the comparison path)
+        for (uint32_t i = 0; i < n_inst; ++i) {
+            rwnmr_walkers w = inst[i].walkers;
+            rwnmr_status s = cpmg_common(c, inst[i].image, &w, inst[i].echoes, inst[i].steps_per_echo, inst[i].bc, 0,
+                                         1, nullptr, nullptr, inst[i].echo_sums);
+            if (s != RWNMR_OK)
+                return s;
+        }
+        return RWNMR_OK;
+    }
+    const uint32_t K = c->shape.k;
+    const uint64_t tile_walkers = (uint64_t)rwnmr::kBatchThreads * K;
+    std::vector<rwnmr::BatchInstance> desc(n_inst);
+    std::vector<rwnmr::BatchTile> tiles;
+    std::vector<std::pair<double, uint32_t>> order;
+    uint64_t part_total = 0, rows_total = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) {
+        const rwnmr_cpmg_instance &q = inst[i];
+        const uint64_t nt = (q.walkers.n + tile_walkers - 1) / tile_walkers;
+        if (nt > 0xFFFFFFFFull)
+            return fail(c, RWNMR_ERR_INVALID, "batch_run: too many tiles");
+        part_total += (uint64_t)(q.echoes + 1) * nt;
+        rows_total += q.echoes + 1ull;
+    }
+    RW_HIP(c, hipStreamSynchronize(c->stream));  // descriptor buffers may still be in use
+    if (clear_flag(c) != RWNMR_OK)
+        return RWNMR_ERR_HIP;
+    RW_HIP(c, c->partials.reserve(8ull * part_total + 8));
+    double *part = c->partials.as<double>();
+    std::vector<rwnmr::ReduceRow> rows;
+    rows.reserve(rows_total);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) {
+        const rwnmr_cpmg_instance &q = inst[i];
+        const rwnmr_image *img = q.image;
+        const uint32_t nt = (uint32_t)((q.walkers.n + tile_walkers - 1) / tile_walkers);
+        rwnmr::BatchInstance &d = desc[i];
+        d.blocks = img->blocks;
+        d.W = img->W;
+        d.H = img->H;
+        d.D = img->D;
+        d.bC = img->bC;
+        d.bCR = img->bC * img->bR;
+        d.bc = q.bc;
+        d.x = q.walkers.x;
+        d.y = q.walkers.y;
+        d.z = q.walkers.z;
+        d.seed = q.walkers.seed;
+        d.energy = q.walkers.energy;
+        d.hits = q.walkers.hits;
+        d.penalty = q.walkers.penalty;
+        d.delta_uniform = q.walkers.delta_uniform;
+        d.n = q.walkers.n;
+        d.echoes = q.echoes;
+        d.steps_per_echo = q.steps_per_echo;
+        d.partials = part + off;
+        d.n_tiles = nt;
+        for (uint32_t t = 0; t < nt; ++t) {
+            const uint64_t walkers = std::min<uint64_t>(tile_walkers, q.walkers.n - (uint64_t)t * tile_walkers);
+            const double cost = (double)q.echoes * q.steps_per_echo * (double)walkers;
+            order.emplace_back(-cost, (uint32_t)tiles.size());
+            tiles.push_back(rwnmr::BatchTile{i, t});
+        }
+        for (uint32_t e = 0; e <= q.echoes; ++e)
+            rows.push_back(rwnmr::ReduceRow{part + off + (uint64_t)e * nt, nt, q.echo_sums + e});
+        off += (uint64_t)(q.echoes + 1) * nt;
+    }
+    // longest tiles first (LPT order) so the queue's tail is short work
+    std::stable_sort(order.begin(), order.end());
+    std::vector<rwnmr::BatchTile> sorted(tiles.size());
+    for (size_t i = 0; i < order.size(); ++i)
+        sorted[i] = tiles[order[i].second];
+    RW_HIP(c, c->batch_inst.reserve(sizeof(rwnmr::BatchInstance) * desc.size()));
+    RW_HIP(c, c->batch_tiles.reserve(sizeof(rwnmr::BatchTile) * sorted.size()));
+    RW_HIP(c, c->batch_rows.reserve(sizeof(rwnmr::ReduceRow) * rows.size()));
+    RW_HIP(c, c->batch_counter.reserve(sizeof(uint32_t)));
+    RW_HIP(c, hipMemcpy(c->batch_inst.p, desc.data(), sizeof(rwnmr::BatchInstance) * desc.size(),
+                        hipMemcpyHostToDevice));
+    RW_HIP(c, hipMemcpy(c->batch_tiles.p, sorted.data(), sizeof(rwnmr::BatchTile) * sorted.size(),
+                        hipMemcpyHostToDevice));
+    RW_HIP(c, hipMemcpy(c->batch_rows.p, rows.data(), sizeof(rwnmr::ReduceRow) * rows.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c->stream, c->batch_counter.as<uint32_t>(), 0u, 1ull);
+    RW_HIP(c, hipGetLastError());
+    int per_cu = 0, cus = 0;
+    RW_HIP(c, rwnmr::batch_resident_blocks(K, &per_cu));
+    RW_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const uint64_t resident = (uint64_t)std::max(per_cu, 1) * (uint64_t)std::max(cus, 1);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(resident, sorted.size());
+    rwnmr::BatchArgs b;
+    b.inst = c->batch_inst.as<rwnmr::BatchInstance>();
+    b.tiles = c->batch_tiles.as<rwnmr::BatchTile>();
+    b.n_tiles = (uint32_t)sorted.size();
+    b.counter = c->batch_counter.as<uint32_t>();
+    b.error_flag = c->flag.as<uint32_t>();
+    hipEvent_t t0 = timing_begin(c);
+    RW_HIP(c, rwnmr::launch_batch(b, K, grid, c->stream));
+    timing_end(c, t0);
+    RW_HIP(c, rwnmr::launch_reduce_segments(c->batch_rows.as<rwnmr::ReduceRow>(), rows.size(), c->stream));
+    return RWNMR_OK;
+}
