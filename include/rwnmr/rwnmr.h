@@ -163,6 +163,31 @@ typedef struct {
  * one sample per process (B/NMR_Simulation/NMR_cpmg_cuda.cpp:929-1774). */
 rwnmr_status rwnmr_cpmg_batch_run(rwnmr_ctx *ctx, const rwnmr_cpmg_instance *inst, uint32_t n_inst, uint32_t flags);
 
+/* Differentiable CPMG (no reference counterpart: the reference fits
+ * relaxivity with a genetic algorithm, B/GA/ga_core.cpp:463-490, over one
+ * relaxivity per walker).  Relaxation never changes a trajectory, so
+ * dM(n)/d delta_l = sum_w E_w(n) h_wl(n) / delta_l with h_wl(n) the
+ * walker's hits on solid label l up to echo n.
+ *
+ * rwnmr_ckpt_bytes: size of the checkpoint buffer for n walkers, n_segments
+ * = ceil(echoes / every) segments and n_labels labels.
+ * rwnmr_cpmg_forward_ckpt: the labelled CPMG run (as rwnmr_cpmg_run_labels,
+ * device walkers with an energy array, device label_delta and echo_sums of
+ * echoes+1) that also stores, every `every` echoes, each walker's position,
+ * RNG state, energy and per-label hits into ckpt (device memory of
+ * rwnmr_ckpt_bytes).  Asynchronous.
+ * rwnmr_cpmg_backward: given dL/dM(n) (device, echoes+1) returns
+ * dL/d delta_l (host, n_labels) by replaying every segment from its
+ * checkpoint in parallel (one thread per walker and segment) instead of
+ * storing hit histories.  Synchronous. */
+uint64_t rwnmr_ckpt_bytes(uint64_t n, uint32_t n_segments, uint32_t n_labels);
+rwnmr_status rwnmr_cpmg_forward_ckpt(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                     uint32_t steps_per_echo, int32_t bc, const double *label_delta, uint32_t every,
+                                     void *ckpt, double *echo_sums);
+rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *ctx, const rwnmr_image *img, const void *ckpt, uint64_t n,
+                                 uint32_t echoes, uint32_t steps_per_echo, int32_t bc, const double *label_delta,
+                                 uint32_t every, const double *grad_echo, double *grad_delta);
+
 /* PFGSE, reference GPU semantics (NMR_PFGSE::simulation_cuda,
  * B/NMR_Simulation/NMR_pfgse_cuda.cpp:536-1348): for each diffusion time i
  * in order, walk steps_inc[i] more steps (PFG_map_*, :34-339), multiply the

@@ -1,0 +1,56 @@
+// backward_kernels.h -- checkpoint layout and the replay backward kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "walk_kernels.h"
+
+namespace rwnmr {
+
+// One checkpoint segment, structure of arrays over n walkers.
+struct CkptView {
+    int32_t *x, *y, *z;
+    uint64_t *seed;
+    double *energy;
+    uint32_t *hits;   // [n_labels][n]
+};
+
+// Bytes per segment: 3 int32 + u64 seed + f64 energy + n_labels u32, 8-aligned.
+__host__ __device__ inline uint64_t ckpt_segment_bytes(uint64_t n, uint32_t n_labels)
+{
+    const uint64_t words = (3ull * n * 4 + 7) / 8 * 8;
+    return words + 16ull * n + ((4ull * n_labels * n + 7) / 8) * 8;
+}
+
+__host__ __device__ inline CkptView ckpt_view(void *base, uint64_t n, uint32_t n_segments, uint32_t n_labels,
+                                              uint32_t seg)
+{
+    (void)n_segments;
+    uint8_t *p = (uint8_t *)base + (uint64_t)seg * ckpt_segment_bytes(n, n_labels);
+    CkptView v;
+    v.x = (int32_t *)p;
+    v.y = v.x + n;
+    v.z = v.y + n;
+    p += (3ull * n * 4 + 7) / 8 * 8;
+    v.seed = (uint64_t *)p;
+    v.energy = (double *)(p + 8ull * n);
+    v.hits = (uint32_t *)(p + 16ull * n);
+    return v;
+}
+
+struct BackwardArgs {
+    void *ckpt;
+    uint64_t n;
+    uint32_t n_segments;
+    int32_t n_labels;
+    uint32_t every;            // echoes per segment
+    uint32_t echoes;
+    uint32_t steps_per_echo;
+    const double *grad_echo;   // [echoes + 1] dL/dM(n)
+    double *partials;          // [n_labels][partial_stride]
+    uint64_t partial_stride;   // = segments * blocks
+};
+
+hipError_t launch_backward(const WalkArgs &a, const BackwardArgs &b, int bc, uint32_t threads, hipStream_t st);
+
+}  // namespace rwnmr

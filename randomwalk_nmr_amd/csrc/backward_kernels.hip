@@ -1,0 +1,103 @@
+// backward_kernels.hip -- gradient of the CPMG echo sums with respect to the
+// per-label wall penalties, by trajectory replay from checkpoints.
+//
+// Relaxation never changes a trajectory (a wall hit leaves the walker in
+// place, B/NMR_Simulation/NMR_cpmg_cuda.cpp:113-124), so each walker's energy
+// after echo n is E_w(n) = prod over its hits of delta_label(hit), and
+//   dM(n)/d delta_l = sum_w E_w(n) h_wl(n) / delta_l,
+// with h_wl(n) the walker's hits on label l up to echo n.  For an upstream
+// gradient g(n) = dL/dM(n) the kernel accumulates, per walker and label,
+//   G_wl = sum_n g(n) E_w(n) h_wl(n)
+// while replaying one checkpoint segment (state saved every `every`
+// echoes: position, RNG state, energy, per-label hits), then reduces G over
+// walkers.  One thread per (walker, segment): segments replay in parallel
+// and no hit history is stored.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "backward_kernels.h"
+#include "walk_device.h"
+
+namespace rwnmr {
+
+template <int BC>
+__global__ __launch_bounds__(256) void backward_kernel(const WalkArgs a, const BackwardArgs b)
+{
+    __shared__ double lds[kMaxLabels][kMaxWaves];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const uint32_t seg = blockIdx.y;
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = b.n;
+    const bool active = id < n;
+    const uint64_t j = active ? id : 0;
+    const CkptView cv = ckpt_view(b.ckpt, n, b.n_segments, b.n_labels, seg);
+    Walker w;
+    w.active = active;
+    w.x = active ? cv.x[j] : 0;
+    w.y = active ? cv.y[j] : 0;
+    w.z = active ? cv.z[j] : 0;
+    w.s = active ? cv.seed[j] : 1;
+    w.e = active ? cv.energy[j] : 0.0;
+    w.h = 0;
+    w.delta = 1.0;
+    init_fold<BC>(a, w);
+    uint32_t h[kMaxLabels];
+    double G[kMaxLabels];
+#pragma unroll
+    for (int l = 0; l < kMaxLabels; ++l) {
+        h[l] = (active && l < b.n_labels) ? cv.hits[(uint64_t)l * n + j] : 0u;
+        G[l] = 0.0;
+    }
+    const uint32_t e0 = seg * b.every;
+    const uint32_t e1 = (e0 + b.every < b.echoes) ? e0 + b.every : b.echoes;
+    for (uint32_t echo = e0 + 1; echo <= e1; ++echo) {
+        for (uint32_t st = 0; st < b.steps_per_echo; ++st) {
+            Move mv = propose<BC>(a, w);
+            const uint64_t word = active ? a.blocks[mv.blk] : 0ull;
+            if (!((word >> mv.bit) & 1ull)) {
+                commit<BC>(w, mv);
+            } else {
+                const uint32_t lab = a.labels_blocked[(uint64_t)mv.blk * 64u + mv.bit];
+                w.e *= a.label_delta[lab];
+#pragma unroll
+                for (int l = 0; l < kMaxLabels; ++l)
+                    h[l] += (lab == (uint32_t)l) ? 1u : 0u;
+            }
+        }
+        const double ge = b.grad_echo[echo] * w.e;
+#pragma unroll
+        for (int l = 0; l < kMaxLabels; ++l)
+            G[l] += ge * (double)h[l];
+    }
+#pragma unroll
+    for (int l = 0; l < kMaxLabels; ++l) {
+        const double v = wave_sum(active ? G[l] : 0.0);
+        if (lane == 0)
+            lds[l][wave] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < b.n_labels) {
+        double s = 0.0;
+        for (int q = 0; q < nw; ++q)
+            s += lds[threadIdx.x][q];
+        const uint64_t col = (uint64_t)seg * gridDim.x + blockIdx.x;
+        b.partials[(uint64_t)threadIdx.x * b.partial_stride + col] = s;
+    }
+}
+
+hipError_t launch_backward(const WalkArgs &a, const BackwardArgs &b, int bc, uint32_t threads, hipStream_t st)
+{
+    const uint32_t bx = (uint32_t)((b.n + threads - 1) / threads);
+    const dim3 grid(bx, b.n_segments);
+    switch (bc) {
+    case kNoflux: hipLaunchKernelGGL(backward_kernel<kNoflux>, grid, dim3(threads), 0, st, a, b); break;
+    case kPeriodic: hipLaunchKernelGGL(backward_kernel<kPeriodic>, grid, dim3(threads), 0, st, a, b); break;
+    case kMirror: hipLaunchKernelGGL(backward_kernel<kMirror>, grid, dim3(threads), 0, st, a, b); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rwnmr

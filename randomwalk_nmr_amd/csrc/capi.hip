@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/rwnmr/rwnmr.h"
+#include "backward_kernels.h"
 #include "batch_kernels.h"
 #include "walk_kernels.h"
 
@@ -892,6 +893,113 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
 }
 
 }  // extern "C"
+
+extern "C" uint64_t rwnmr_ckpt_bytes(uint64_t n, uint32_t n_segments, uint32_t n_labels)
+{
+    return rwnmr::ckpt_segment_bytes(n, n_labels) * (uint64_t)n_segments;
+}
+
+extern "C" rwnmr_status rwnmr_cpmg_forward_ckpt(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
+                                                uint32_t echoes, uint32_t steps_per_echo, int32_t bc,
+                                                const double *label_delta, uint32_t every, void *ckpt,
+                                                double *echo_sums)
+{
+    rwnmr_status s;
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if ((s = check_walkers(c, img, w, bc, 0)) != RWNMR_OK)
+        return s;
+    if (!w->on_device || !w->energy || !label_delta || !ckpt || !echo_sums || every == 0 || echoes == 0)
+        return fail(c, RWNMR_ERR_INVALID,
+                    "forward_ckpt: device walkers with energies, label deltas, checkpoint buffer, echo sums");
+    if (!img->labels_blocked)
+        return fail(c, RWNMR_ERR_INVALID, "forward_ckpt: image has no solid labels");
+    const uint64_t n = w->n;
+    const uint32_t nl = (uint32_t)img->n_labels;
+    const uint32_t nseg = (echoes + every - 1) / every;
+    RW_HIP(c, c->label_hits.reserve(4ull * nl * n + 4));
+    uint32_t *lh = c->label_hits.as<uint32_t>();
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)((nl * n + 255) / 256)), dim3(256), 0, c->stream, lh, 0u,
+                       (uint64_t)nl * n);
+    RW_HIP(c, hipGetLastError());
+    for (uint32_t seg = 0; seg < nseg; ++seg) {
+        const rwnmr::CkptView v = rwnmr::ckpt_view(ckpt, n, nseg, nl, seg);
+        RW_HIP(c, hipMemcpyAsync(v.x, w->x, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(v.y, w->y, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(v.z, w->z, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(v.seed, w->seed, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(v.energy, w->energy, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(v.hits, lh, 4ull * nl * n, hipMemcpyDeviceToDevice, c->stream));
+        const uint32_t e0 = seg * every;
+        const uint32_t ne = std::min(every, echoes - e0);
+        s = cpmg_common(c, img, w, ne, steps_per_echo, bc, 0, 2, label_delta, lh, echo_sums + e0);
+        if (s != RWNMR_OK)
+            return s;
+    }
+    return RWNMR_OK;
+}
+
+extern "C" rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *c, const rwnmr_image *img, const void *ckpt, uint64_t n,
+                                            uint32_t echoes, uint32_t steps_per_echo, int32_t bc,
+                                            const double *label_delta, uint32_t every, const double *grad_echo,
+                                            double *grad_delta)
+{
+    rwnmr_status s;
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if (!img->labels_blocked || !ckpt || !label_delta || !grad_echo || !grad_delta || every == 0 || echoes == 0 ||
+        n == 0)
+        return fail(c, RWNMR_ERR_INVALID, "backward: null argument or empty run");
+    if (bc < RWNMR_BC_NOFLUX || bc > RWNMR_BC_MIRROR)
+        return fail(c, RWNMR_ERR_INVALID, "backward: unknown boundary condition");
+    const uint32_t nl = (uint32_t)img->n_labels;
+    const uint32_t nseg = (echoes + every - 1) / every;
+    const uint32_t threads = 256;
+    const uint64_t bx = (n + threads - 1) / threads;
+    if (bx > 0x7FFFFFFFull || nseg > 65535)
+        return fail(c, RWNMR_ERR_INVALID, "backward: grid too large");
+    rwnmr::BackwardArgs b;
+    b.ckpt = const_cast<void *>(ckpt);
+    b.n = n;
+    b.n_segments = nseg;
+    b.n_labels = (int32_t)nl;
+    b.every = every;
+    b.echoes = echoes;
+    b.steps_per_echo = steps_per_echo;
+    b.grad_echo = grad_echo;
+    b.partial_stride = (uint64_t)nseg * bx;
+    RW_HIP(c, c->partials.reserve(8ull * nl * b.partial_stride + 8));
+    b.partials = c->partials.as<double>();
+    WalkArgs a;
+    memset(&a, 0, sizeof(a));
+    a.blocks = img->blocks;
+    a.labels_blocked = img->labels_blocked;
+    a.W = img->W;
+    a.H = img->H;
+    a.D = img->D;
+    a.bC = img->bC;
+    a.bCR = img->bC * img->bR;
+    a.label_delta = label_delta;
+    a.n_labels = (int32_t)nl;
+    a.n = n;
+    hipEvent_t t0 = timing_begin(c);
+    RW_HIP(c, rwnmr::launch_backward(a, b, bc, threads, c->stream));
+    timing_end(c, t0);
+    RW_HIP(c, c->sums.reserve(8ull * nl + 8));
+    RW_HIP(c, rwnmr::launch_reduce_rows(b.partials, nl, (uint32_t)b.partial_stride, c->sums.as<double>(), c->stream));
+    double raw[rwnmr::kMaxLabels], dl[rwnmr::kMaxLabels];
+    RW_HIP(c, hipMemcpyAsync(raw, c->sums.p, 8ull * nl, hipMemcpyDeviceToHost, c->stream));
+    RW_HIP(c, hipMemcpyAsync(dl, label_delta, 8ull * nl, hipMemcpyDeviceToHost, c->stream));
+    if ((s = sync_and_check(c)) != RWNMR_OK)
+        return s;
+    for (uint32_t l = 0; l < nl; ++l)
+        grad_delta[l] = raw[l] / dl[l];
+    return RWNMR_OK;
+}
 
 extern "C" rwnmr_status rwnmr_cpmg_batch_run(rwnmr_ctx *c, const rwnmr_cpmg_instance *inst, uint32_t n_inst,
                                              uint32_t flags)
