@@ -33,6 +33,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const WalkArgs a, const B
     const bool active = id < n;
     const uint64_t j = active ? id : 0;
     const CkptView cv = ckpt_view(b.ckpt, n, b.n_segments, b.n_labels, seg);
+    const Geom g = make_geom(a);
     Walker w;
     w.active = active;
     w.x = active ? cv.x[j] : 0;
@@ -42,41 +43,44 @@ __global__ __launch_bounds__(256) void backward_kernel(const WalkArgs a, const B
     w.e = active ? cv.energy[j] : 0.0;
     w.h = 0;
     w.delta = 1.0;
-    init_fold<BC>(a, w);
+    init_fold<BC>(g, w);
     uint32_t h[kMaxLabels];
     double G[kMaxLabels];
-#pragma unroll
-    for (int l = 0; l < kMaxLabels; ++l) {
-        h[l] = (active && l < b.n_labels) ? cv.hits[(uint64_t)l * n + j] : 0u;
+    const int n_labels = b.n_labels;
+    static_for<kMaxLabels>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        h[l] = (active && l < n_labels) ? cv.hits[(uint64_t)l * n + j] : 0u;
         G[l] = 0.0;
-    }
+    });
     const uint32_t e0 = seg * b.every;
     const uint32_t e1 = (e0 + b.every < b.echoes) ? e0 + b.every : b.echoes;
     for (uint32_t echo = e0 + 1; echo <= e1; ++echo) {
         for (uint32_t st = 0; st < b.steps_per_echo; ++st) {
-            Move mv = propose<BC>(a, w);
-            const uint64_t word = active ? a.blocks[mv.blk] : 0ull;
+            Move mv = propose<BC>(g, w);
+            const uint64_t word = active ? g.blocks[mv.blk] : 0ull;
             if (!((word >> mv.bit) & 1ull)) {
                 commit<BC>(w, mv);
             } else {
                 const uint32_t lab = a.labels_blocked[(uint64_t)mv.blk * 64u + mv.bit];
                 w.e *= a.label_delta[lab];
-#pragma unroll
-                for (int l = 0; l < kMaxLabels; ++l)
+                static_for<kMaxLabels>([&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
                     h[l] += (lab == (uint32_t)l) ? 1u : 0u;
+                });
             }
         }
         const double ge = b.grad_echo[echo] * w.e;
-#pragma unroll
-        for (int l = 0; l < kMaxLabels; ++l)
+        static_for<kMaxLabels>([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
             G[l] += ge * (double)h[l];
+        });
     }
-#pragma unroll
-    for (int l = 0; l < kMaxLabels; ++l) {
+    static_for<kMaxLabels>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
         const double v = wave_sum(active ? G[l] : 0.0);
         if (lane == 0)
             lds[l][wave] = v;
-    }
+    });
     __syncthreads();
     if ((int)threadIdx.x < b.n_labels) {
         double s = 0.0;

@@ -519,8 +519,9 @@ rwnmr_status rwnmr_ctx_set_launch(rwnmr_ctx *c, uint32_t tpb, uint32_t k)
         tpb = 256;
     if (k == 0)
         k = 1;
-    if (tpb % 64 != 0 || tpb > 1024 || !(k == 1 || k == 2 || k == 4))
-        return fail(c, RWNMR_ERR_INVALID, "threads_per_block must be a multiple of 64 <= 1024, k in {1,2,4}");
+    if (tpb % 64 != 0 || !(k == 1 || k == 2 || k == 4) || tpb > rwnmr::max_threads_for_k((int)k))
+        return fail(c, RWNMR_ERR_INVALID,
+                    "threads_per_block must be a multiple of 64 and at most 1024 (k=1), 512 (k=2) or 256 (k=4)");
     c->shape.threads = tpb;
     c->shape.k = k;
     return RWNMR_OK;
@@ -714,6 +715,8 @@ rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *c, const rwnmr_image *img, const rwnm
     timing_collect(c);
     for (uint32_t t : threads)
         for (uint32_t k : ks) {
+            if (t > rwnmr::max_threads_for_k((int)k))
+                continue;
             rwnmr_walkers tw = *w;
             tw.x = tx.as<int32_t>();
             tw.y = ty.as<int32_t>();
@@ -808,7 +811,7 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
         return fail(c, RWNMR_ERR_INVALID, "pfgse_run: null array");
     if (w->n == 0)
         return fail(c, RWNMR_ERR_INVALID, "pfgse_run: no walkers");
-    const uint32_t threads = c->shape.threads;
+    const uint32_t threads = rwnmr::kPfgseThreads;
     const uint32_t blocks = (uint32_t)((w->n + threads - 1) / threads);
     const uint64_t mrows = (uint64_t)(n_sets ? n_sets : 1) * n_delta * n_k;
     const uint64_t rows = mrows + (msd ? 3ull * n_delta : 0ull);

@@ -1,11 +1,22 @@
 // walk_device.h -- device-side pieces shared by the gfx950 walk kernels
-// (plain CPMG/map/PFGSE launches and the batched persistent kernel): the
-// walker registers, one step split into propose (RNG, direction, fold,
-// block address) and commit, start validation and the FP64 wave / block
-// echo reductions.  Step semantics are documented in walk_common.h.
+// (plain CPMG/map/PFGSE launches, the batched persistent kernel and the
+// backward replay): the walker registers, one step split into propose (RNG,
+// direction, fold, block address) and commit, start validation and the FP64
+// wave / block echo reductions.  Step semantics are documented in
+// walk_common.h.
+//
+// Register discipline: a lane's K walkers live in arrays indexed only by
+// compile-time constants (static_for), and the image geometry is copied into
+// plain values (Geom) once, so every walker field stays in a VGPR and the
+// dimensions in SGPRs.  (Conditional reads of struct fields selected by a
+// runtime value are written on values, never on addresses; otherwise the
+// compiler folds them into address selects and demotes the walker to
+// LDS/scratch.)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "walk_common.h"
 #include "walk_kernels.h"
@@ -16,12 +27,39 @@ static constexpr int kWave = 64;
 static constexpr int kEchoChunk = 32;   // echoes staged in LDS between block syncs
 static constexpr int kMaxWaves = 16;    // up to 1024 threads per block
 
+// f(integral_constant<0>), ..., f(integral_constant<N-1>), fully inlined.
+template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (N > 0) {
+        static_for<N - 1>(f);
+        f(std::integral_constant<int, N - 1>{});
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1)
         v += __shfl_xor(v, off, kWave);
     return v;
+}
+
+// Image geometry as plain values (uniform across the grid).
+struct Geom {
+    const uint64_t *blocks;
+    int W, H, D, bC, bCR;
+};
+
+__device__ __forceinline__ Geom make_geom(const WalkArgs &a)
+{
+    Geom g;
+    g.blocks = a.blocks;
+    g.W = a.W;
+    g.H = a.H;
+    g.D = a.D;
+    g.bC = a.bC;
+    g.bCR = a.bCR;
+    return g;
 }
 
 // Registers of one walker.  ix/iy/iz are the image-voxel (folded)
@@ -37,20 +75,26 @@ struct Walker {
     int active;
 };
 
+__device__ __forceinline__ int pick3(int ax, int a0, int a1, int a2)
+{
+    const int t = ax == 1 ? a1 : a2;
+    return ax == 0 ? a0 : t;
+}
+
 template <int BC>
-__device__ __forceinline__ void init_fold(const WalkArgs &a, Walker &w)
+__device__ __forceinline__ void init_fold(const Geom &g, Walker &w)
 {
     if (BC == kMirror) {
-        w.ix = fold_mirror(w.x, a.W);
-        w.iy = fold_mirror(w.y, a.H);
-        w.iz = fold_mirror(w.z, a.D);
-        w.mx = mirror_flag(w.x, a.W);
-        w.my = mirror_flag(w.y, a.H);
-        w.mz = mirror_flag(w.z, a.D);
+        w.ix = fold_mirror(w.x, g.W);
+        w.iy = fold_mirror(w.y, g.H);
+        w.iz = fold_mirror(w.z, g.D);
+        w.mx = mirror_flag(w.x, g.W);
+        w.my = mirror_flag(w.y, g.H);
+        w.mz = mirror_flag(w.z, g.D);
     } else if (BC == kPeriodic) {
-        w.ix = fold_periodic(w.x, a.W);
-        w.iy = fold_periodic(w.y, a.H);
-        w.iz = fold_periodic(w.z, a.D);
+        w.ix = fold_periodic(w.x, g.W);
+        w.iy = fold_periodic(w.y, g.H);
+        w.iz = fold_periodic(w.z, g.D);
         w.mx = w.my = w.mz = 0;
     } else {
         w.ix = w.x;
@@ -64,24 +108,25 @@ __device__ __forceinline__ void init_fold(const WalkArgs &a, Walker &w)
 struct Move {
     int ax;     // axis 0/1/2
     int sg;     // +-1
-    int nix, niy, niz;
+    int ni;     // new folded coordinate of the moving axis
     int nm;     // new reflection flag of the moving axis
     uint32_t blk, bit;
 };
 
 template <int BC>
-__device__ __forceinline__ Move propose(const WalkArgs &a, Walker &w)
+__device__ __forceinline__ Move propose(const Geom &g, Walker &w)
 {
     Move mv;
     w.s = xorshift64(w.s);
     const uint32_t r = mod6(w.s);
     mv.ax = move_axis(r);
     int sg = move_sign(r);
-    const int i = mv.ax == 0 ? w.ix : (mv.ax == 1 ? w.iy : w.iz);
-    const int d = mv.ax == 0 ? a.W : (mv.ax == 1 ? a.H : a.D);
+    const int ix = w.ix, iy = w.iy, iz = w.iz;
+    const int i = pick3(mv.ax, ix, iy, iz);
+    const int d = pick3(mv.ax, g.W, g.H, g.D);
     int ni, nm = 0;
     if (BC == kMirror) {
-        const int m = mv.ax == 0 ? w.mx : (mv.ax == 1 ? w.my : w.mz);
+        const int m = pick3(mv.ax, w.mx, w.my, w.mz);
         mirror_step(i, m, sg, d, ni, nm);
     } else if (BC == kPeriodic) {
         ni = periodic_step(i, sg, d);
@@ -92,24 +137,26 @@ __device__ __forceinline__ Move propose(const WalkArgs &a, Walker &w)
         ni = i + sg;
     }
     mv.sg = sg;
+    mv.ni = ni;
     mv.nm = nm;
-    mv.nix = mv.ax == 0 ? ni : w.ix;
-    mv.niy = mv.ax == 1 ? ni : w.iy;
-    mv.niz = mv.ax == 2 ? ni : w.iz;
-    mv.blk = block_index(mv.nix, mv.niy, mv.niz, a.bC, a.bCR);
-    mv.bit = bit_index(mv.nix, mv.niy, mv.niz);
+    const int nix = mv.ax == 0 ? ni : ix;
+    const int niy = mv.ax == 1 ? ni : iy;
+    const int niz = mv.ax == 2 ? ni : iz;
+    mv.blk = block_index(nix, niy, niz, g.bC, g.bCR);
+    mv.bit = bit_index(nix, niy, niz);
     return mv;
 }
 
 template <int BC>
 __device__ __forceinline__ void commit(Walker &w, const Move &mv)
 {
-    w.x += mv.ax == 0 ? mv.sg : 0;
-    w.y += mv.ax == 1 ? mv.sg : 0;
-    w.z += mv.ax == 2 ? mv.sg : 0;
-    w.ix = mv.nix;
-    w.iy = mv.niy;
-    w.iz = mv.niz;
+    const int dx = mv.ax == 0 ? mv.sg : 0, dy = mv.ax == 1 ? mv.sg : 0, dz = mv.ax == 2 ? mv.sg : 0;
+    w.x += dx;
+    w.y += dy;
+    w.z += dz;
+    w.ix = mv.ax == 0 ? mv.ni : w.ix;
+    w.iy = mv.ax == 1 ? mv.ni : w.iy;
+    w.iz = mv.ax == 2 ? mv.ni : w.iz;
     if (BC == kMirror) {
         w.mx = mv.ax == 0 ? mv.nm : w.mx;
         w.my = mv.ax == 1 ? mv.nm : w.my;
@@ -120,11 +167,11 @@ __device__ __forceinline__ void commit(Walker &w, const Move &mv)
 // Validity of a walker's start position: noflux walkers must start inside
 // the image (the fold keeps periodic/mirror lookups in range for any start).
 template <int BC>
-__device__ __forceinline__ int start_ok(const WalkArgs &a, const Walker &w)
+__device__ __forceinline__ int start_ok(const Geom &g, const Walker &w)
 {
     if (BC != kNoflux)
         return 1;
-    return (unsigned)w.x < (unsigned)a.W && (unsigned)w.y < (unsigned)a.H && (unsigned)w.z < (unsigned)a.D;
+    return (unsigned)w.x < (unsigned)g.W && (unsigned)w.y < (unsigned)g.H && (unsigned)w.z < (unsigned)g.D;
 }
 
 // Stage one echo's block sum; every kEchoChunk echoes (and at the last echo)
@@ -162,13 +209,18 @@ template <int BC, int EMODE, int K>
 __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                           double (*lds)[kMaxWaves])
 {
+    constexpr int NL = EMODE == 2 ? kMaxLabels : 1;
+    const Geom g = make_geom(a);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
+    const uint64_t n = a.n;
+    const uint32_t echoes = a.echoes, spe = a.steps_per_echo;
+    const int n_labels = a.n_labels;
     Walker w[K];
-    uint32_t lhits[K][kMaxLabels];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
+    uint32_t lhits[K][NL];
+    static_for<K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
         const uint64_t id = base + (uint64_t)k * blockDim.x;
-        w[k].active = id < a.n;
+        w[k].active = id < n;
         const uint64_t j = w[k].active ? id : 0;
         w[k].x = w[k].active ? a.x[j] : 0;
         w[k].y = w[k].active ? a.y[j] : 0;
@@ -177,7 +229,7 @@ __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, doub
         w[k].e = (w[k].active && a.energy) ? a.energy[j] : 1.0;
         w[k].h = (w[k].active && a.hits) ? a.hits[j] : 0u;
         w[k].delta = EMODE == 1 ? ((w[k].active && a.penalty) ? a.penalty[j] : a.delta_uniform) : 1.0;
-        if (w[k].active && !start_ok<BC>(a, w[k])) {
+        if (w[k].active && !start_ok<BC>(g, w[k])) {
             w[k].active = 0;
             atomicOr(a.error_flag, 1u);
         }
@@ -185,34 +237,40 @@ __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, doub
             w[k].x = w[k].y = w[k].z = 0;
             w[k].e = 0.0;
         }
-        init_fold<BC>(a, w[k]);
-#pragma unroll
-        for (int l = 0; l < kMaxLabels; ++l)
-            lhits[k][l] = (EMODE == 2 && a.label_hits && w[k].active && l < a.n_labels)
-                              ? a.label_hits[(uint64_t)l * a.n + j]
+        init_fold<BC>(g, w[k]);
+        static_for<NL>([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            lhits[k][l] = (EMODE == 2 && a.label_hits && w[k].active && l < n_labels)
+                              ? a.label_hits[(uint64_t)l * n + j]
                               : 0u;
-    }
+        });
+    });
 
-    if (part) {
+    auto echo_value = [&]() {
         double v = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
+        static_for<K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
             v += w[k].active ? w[k].e : 0.0;
-        stage_echo(lds, 0, a.echoes, v, part, part_stride);
-    }
+        });
+        return v;
+    };
+    if (part)
+        stage_echo(lds, 0, echoes, echo_value(), part, part_stride);
 
-    for (uint32_t echo = 1; echo <= a.echoes; ++echo) {
-        for (uint32_t st = 0; st < a.steps_per_echo; ++st) {
+    for (uint32_t echo = 1; echo <= echoes; ++echo) {
+        for (uint32_t st = 0; st < spe; ++st) {
             Move mv[K];
             uint64_t word[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                mv[k] = propose<BC>(a, w[k]);
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                word[k] = w[k].active ? a.blocks[mv[k].blk] : 0ull;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                mv[k] = propose<BC>(g, w[k]);
+            });
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                word[k] = w[k].active ? g.blocks[mv[k].blk] : 0ull;
+            });
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
                 const int wall = (int)((word[k] >> mv[k].bit) & 1ull);
                 if (!wall) {
                     commit<BC>(w[k], mv[k]);
@@ -223,41 +281,38 @@ __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, doub
                     } else if (EMODE == 2) {
                         const uint32_t lab = a.labels_blocked[(uint64_t)mv[k].blk * 64u + mv[k].bit];
                         w[k].e *= a.label_delta[lab];
-#pragma unroll
-                        for (int l = 0; l < kMaxLabels; ++l)
+                        static_for<NL>([&](auto lc) {
+                            constexpr int l = decltype(lc)::value;
                             lhits[k][l] += (lab == (uint32_t)l) ? 1u : 0u;
+                        });
                     }
                 }
-            }
+            });
         }
-        if (part) {
-            double v = 0.0;
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                v += w[k].active ? w[k].e : 0.0;
-            stage_echo(lds, echo, a.echoes, v, part, part_stride);
-        }
+        if (part)
+            stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
     }
 
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (!w[k].active)
-            continue;
-        const uint64_t id = base + (uint64_t)k * blockDim.x;
-        a.x[id] = w[k].x;
-        a.y[id] = w[k].y;
-        a.z[id] = w[k].z;
-        a.seed[id] = w[k].s;
-        if (a.energy)
-            a.energy[id] = w[k].e;
-        if (a.hits)
-            a.hits[id] = w[k].h;
-        if (EMODE == 2 && a.label_hits)
-#pragma unroll
-            for (int l = 0; l < kMaxLabels; ++l)
-                if (l < a.n_labels)
-                    a.label_hits[(uint64_t)l * a.n + id] = lhits[k][l];
-    }
+    static_for<K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (w[k].active) {
+            const uint64_t id = base + (uint64_t)k * blockDim.x;
+            a.x[id] = w[k].x;
+            a.y[id] = w[k].y;
+            a.z[id] = w[k].z;
+            a.seed[id] = w[k].s;
+            if (a.energy)
+                a.energy[id] = w[k].e;
+            if (a.hits)
+                a.hits[id] = w[k].h;
+            if (EMODE == 2 && a.label_hits)
+                static_for<NL>([&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    if (l < n_labels)
+                        a.label_hits[(uint64_t)l * n + id] = lhits[k][l];
+                });
+        }
+    });
 }
 
 }  // namespace rwnmr

@@ -37,6 +37,12 @@ struct LaunchShape {
     uint32_t k;         // walkers per lane: 1, 2 or 4
 };
 
+// Largest block for K walkers per lane: the register budget per lane shrinks
+// as the block grows (512 VGPRs per SIMD lane / waves per SIMD), so wider K
+// runs in smaller blocks to keep every walker in registers.
+__host__ __device__ constexpr uint32_t max_threads_for_k(int k) { return k >= 4 ? 256u : (k == 2 ? 512u : 1024u); }
+
+static constexpr uint32_t kPfgseThreads = 256;
 static constexpr int kMaxSets = 8;       // energy sets evaluated on one PFGSE trajectory
 static constexpr int kPfgseKChunk = 64;  // k values staged in LDS per block sync
 

@@ -24,7 +24,7 @@ namespace rwnmr {
 
 // One block per tile of blockDim.x * K walkers.
 template <int BC, int EMODE, int K>
-__global__ __launch_bounds__(1024) void walk_kernel(const WalkArgs a)
+__global__ __launch_bounds__(max_threads_for_k(K)) void walk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kMaxWaves];
     walk_tile<BC, EMODE, K>(a, blockIdx.x, a.partials ? a.partials + blockIdx.x : nullptr, a.partial_stride, lds);
@@ -119,13 +119,14 @@ __global__ __launch_bounds__(1024) void walk_kernel_shift(const WalkArgs a)
 // pow(delta, hits of the increment) and the block sums of E cos(phase) for all
 // k are staged through LDS.  Partials: [(set * n_delta + i) * n_k + k][block].
 template <int BC>
-__global__ __launch_bounds__(1024) void pfgse_kernel(const WalkArgs a, const PfgseArgs p)
+__global__ __launch_bounds__(kPfgseThreads) void pfgse_kernel(const WalkArgs a, const PfgseArgs p)
 {
     __shared__ double lds[kPfgseKChunk][kMaxWaves];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
     const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const Geom g = make_geom(a);
     Walker w;
     w.active = id < a.n;
     const uint64_t j = w.active ? id : 0;
@@ -140,43 +141,44 @@ __global__ __launch_bounds__(1024) void pfgse_kernel(const WalkArgs a, const Pfg
     const double own_delta = (w.active && a.penalty) ? a.penalty[j] : a.delta_uniform;
     double e[kMaxSets];
     const int nsets = p.n_sets > 0 ? (int)p.n_sets : 1;
-#pragma unroll
-    for (int q = 0; q < kMaxSets; ++q)
-        e[q] = (q == 0 && w.active && a.energy) ? a.energy[j] : 1.0;
-    if (w.active && !start_ok<BC>(a, w)) {
-        w.active = 0;
+    const bool ok = w.active && start_ok<BC>(g, w);
+    if (w.active && !ok)
         atomicOr(a.error_flag, 1u);
-    }
-    if (!w.active) {
+    w.active = ok;
+    static_for<kMaxSets>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        e[q] = !w.active ? 0.0 : ((q == 0 && a.energy) ? a.energy[j] : 1.0);
+    });
+    if (!w.active)
         w.x = w.y = w.z = 0;
-#pragma unroll
-        for (int q = 0; q < kMaxSets; ++q)
-            e[q] = 0.0;
-    }
-    init_fold<BC>(a, w);
+    init_fold<BC>(g, w);
 
     for (uint32_t i = 0; i < p.n_delta; ++i) {
         uint32_t h = 0;
         const uint32_t steps = p.steps_inc[i];
         for (uint32_t st = 0; st < steps; ++st) {
-            Move mv = propose<BC>(a, w);
-            const uint64_t word = w.active ? a.blocks[mv.blk] : 0ull;
+            Move mv = propose<BC>(g, w);
+            const uint64_t word = w.active ? g.blocks[mv.blk] : 0ull;
             if (!((word >> mv.bit) & 1ull))
                 commit<BC>(w, mv);
             else
                 h += 1u;
         }
         w.h += h;
-#pragma unroll
-        for (int q = 0; q < kMaxSets; ++q)
+        static_for<kMaxSets>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
             if (q < nsets)
                 e[q] *= pow(p.n_sets > 0 ? p.set_delta[q] : own_delta, (double)h);
+        });
         const double dX = (double)(w.x - x0);
         const double dY = (double)(w.y - y0);
         const double dZ = (double)(w.z - z0);
         for (uint32_t k0 = 0; k0 < p.n_k; k0 += kPfgseKChunk) {
             const uint32_t kc = p.n_k - k0 < (uint32_t)kPfgseKChunk ? p.n_k - k0 : (uint32_t)kPfgseKChunk;
-            for (int q = 0; q < nsets; ++q) {
+            static_for<kMaxSets>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if (q >= nsets)
+                    return;
                 for (uint32_t kk = 0; kk < kc; ++kk) {
                     const uint32_t k = k0 + kk;
                     const uint64_t kk_i = (uint64_t)i * p.k_delta_stride + k;
@@ -195,17 +197,18 @@ __global__ __launch_bounds__(1024) void pfgse_kernel(const WalkArgs a, const Pfg
                     p.partials[row * p.partial_stride + blockIdx.x] = s;
                 }
                 __syncthreads();
-            }
+            });
         }
         if (p.msd_rows) {
             // squared displacements per axis (voxel^2; integer-valued, so the
             // FP64 sums are exact for any order)
-            const double d2[3] = {w.active ? dX * dX : 0.0, w.active ? dY * dY : 0.0, w.active ? dZ * dZ : 0.0};
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                const double v = wave_sum(d2[ax]);
-                if (lane == 0)
-                    lds[ax][wave] = v;
+            const double sx = wave_sum(w.active ? dX * dX : 0.0);
+            const double sy = wave_sum(w.active ? dY * dY : 0.0);
+            const double sz = wave_sum(w.active ? dZ * dZ : 0.0);
+            if (lane == 0) {
+                lds[0][wave] = sx;
+                lds[1][wave] = sy;
+                lds[2][wave] = sz;
             }
             __syncthreads();
             if (threadIdx.x < 3) {
@@ -298,7 +301,7 @@ hipError_t launch_walk(const WalkArgs &a, int bc, int emode, const LaunchShape &
 {
     if (a.n == 0)
         return hipSuccess;
-    if (walk_grid_blocks(a.n, ls, a.shift) > 0x7FFFFFFFu)
+    if (walk_grid_blocks(a.n, ls, a.shift) > 0x7FFFFFFFu || ls.threads > max_threads_for_k((int)ls.k))
         return hipErrorInvalidValue;
 #define RW_CASE(B, E)                                                                                   \
     if (bc == B && emode == E)                                                                            \
