@@ -86,6 +86,9 @@ struct rwnmr_ctx {
     std::vector<hipEvent_t> event_pool;
     double kernel_ms = 0.0;
     uint64_t kernel_launches = 0;
+    // live images: destroying the context releases their device memory and
+    // detaches them, so a later rwnmr_image_destroy only frees the handle
+    std::vector<rwnmr_image *> images;
 };
 
 struct rwnmr_image {
@@ -463,6 +466,16 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
         return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (rwnmr_image *img : c->images) {
+        if (img->blocks)
+            (void)hipFree(img->blocks);
+        if (img->labels_blocked)
+            (void)hipFree(img->labels_blocked);
+        img->blocks = nullptr;
+        img->labels_blocked = nullptr;
+        img->ctx = nullptr;
+    }
+    c->images.clear();
     for (DevBuf *b : {&c->x, &c->y, &c->z, &c->seed, &c->energy, &c->hits, &c->penalty, &c->x0, &c->y0, &c->z0,
                       &c->partials, &c->sums, &c->label_delta, &c->label_hits, &c->steps_inc, &c->kvec,
                       &c->set_delta, &c->flag, &c->batch_inst, &c->batch_tiles, &c->batch_rows,
@@ -617,6 +630,7 @@ rwnmr_status rwnmr_image_upload(rwnmr_ctx *c, const uint64_t *blocks, int32_t W,
         return fail(c, e == hipErrorOutOfMemory ? RWNMR_ERR_ALLOC : RWNMR_ERR_HIP,
                     std::string("image_upload: ") + hipGetErrorString(e));
     }
+    c->images.push_back(img);
     *out = img;
     return RWNMR_OK;
 }
@@ -625,12 +639,16 @@ void rwnmr_image_destroy(rwnmr_image *img)
 {
     if (!img)
         return;
-    if (img->ctx)
-        (void)hipStreamSynchronize(img->ctx->stream);
-    if (img->blocks)
-        (void)hipFree(img->blocks);
-    if (img->labels_blocked)
-        (void)hipFree(img->labels_blocked);
+    if (img->ctx) {
+        rwnmr_ctx *c = img->ctx;
+        (void)hipStreamSynchronize(c->stream);
+        c->images.erase(std::remove(c->images.begin(), c->images.end(), img), c->images.end());
+        if (img->blocks)
+            (void)hipFree(img->blocks);
+        if (img->labels_blocked)
+            (void)hipFree(img->labels_blocked);
+    }
+    // an image whose context is gone was already released by rwnmr_ctx_destroy
     delete img;
 }
 

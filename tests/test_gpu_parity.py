@@ -40,7 +40,7 @@ def _assert_echoes(ge, oe):
 
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
-@pytest.mark.parametrize("shape", [(256, 1), (64, 2), (1024, 4), (128, 4)])
+@pytest.mark.parametrize("shape", [(256, 1), (1024, 1), (64, 2), (512, 2), (128, 4), (256, 4)])
 def test_cpmg_matches_oracle(ctx, bc, shape):
     print(f"[test] cpmg bc={bc} launch={shape}", flush=True)
     solid, seeds, x, y, z, pen = _case(30, 21, 17, 3.5, 0.35, 1037, 11)
