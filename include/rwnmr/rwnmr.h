@@ -72,6 +72,18 @@ rwnmr_status rwnmr_ctx_get_launch(const rwnmr_ctx *ctx, uint32_t *threads_per_bl
  * returns the accumulated kernel milliseconds and launch count. */
 rwnmr_status rwnmr_ctx_set_timing(rwnmr_ctx *ctx, int32_t enable);
 rwnmr_status rwnmr_ctx_get_timing(rwnmr_ctx *ctx, double *kernel_ms, uint64_t *launches, int32_t reset);
+/* Device layout of images uploaded after this call: the bitBlock words are
+ * regrouped in bricks of 2^sx x 2^sy x 2^sz blocks so a cache line holds a
+ * compact 3-D neighbourhood (0,0,0 keeps the reference order,
+ * B/BitBlock/bitBlock.h:113-122).  Default 1,1,1.  Results do not depend on it. */
+rwnmr_status rwnmr_ctx_set_image_layout(rwnmr_ctx *ctx, int32_t sx, int32_t sy, int32_t sz);
+/* Locality ordering for CPMG and map runs of at least min_walkers walkers:
+ * walkers are sorted by the Morton code of their image block every
+ * resort_every echoes and tiles are striped so each XCD walks one compact
+ * region of the image (its L2 holds that region).  Walker results are
+ * identical with it on or off; echo sums differ only in summation order.
+ * Default on, every 1000 echoes, from 65,536 walkers. */
+rwnmr_status rwnmr_ctx_set_locality(rwnmr_ctx *ctx, int32_t enable, uint32_t resort_every, uint64_t min_walkers);
 
 /* Upload a packed image in the reference bitBlock layout: block index
  * (x>>2) + (y>>2)*bC + (z>>2)*bC*bR, bit (x&3) + 4*(y&3) + 16*(z&3),

@@ -69,6 +69,15 @@ class Context:
         check(load().rwnmr_ctx_get_launch(self.handle, ct.byref(t), ct.byref(k)), self.handle)
         return t.value, k.value
 
+    def set_image_layout(self, sx: int = 1, sy: int = 1, sz: int = 1):
+        """Brick layout (log2 blocks per axis) of images uploaded afterwards."""
+        check(load().rwnmr_ctx_set_image_layout(self.handle, sx, sy, sz), self.handle)
+
+    def set_locality(self, enable: bool = True, resort_every: int = 1000, min_walkers: int = 65536):
+        """Morton ordering of walkers + XCD-striped tiles for large runs."""
+        check(load().rwnmr_ctx_set_locality(self.handle, int(enable), int(resort_every), int(min_walkers)),
+              self.handle)
+
     def set_timing(self, enable: bool = True):
         check(load().rwnmr_ctx_set_timing(self.handle, int(enable)), self.handle)
 

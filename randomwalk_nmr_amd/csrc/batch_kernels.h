@@ -13,6 +13,7 @@ static constexpr uint32_t kBatchThreads = 256;
 struct BatchInstance {
     const uint64_t *blocks;
     int32_t W, H, D, bC, bCR;
+    BrickLayout layout;
     int32_t bc;
     int32_t *x, *y, *z;
     uint64_t *seed;

@@ -42,6 +42,9 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         a.D = I.D;
         a.bC = I.bC;
         a.bCR = I.bCR;
+        a.layout = I.layout;
+        a.tile_map = 0;
+        a.n_tiles = 0;
         a.shift = 0;
         a.x = I.x;
         a.y = I.y;

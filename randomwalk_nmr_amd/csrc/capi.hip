@@ -17,6 +17,7 @@
 #include "../../include/rwnmr/rwnmr.h"
 #include "backward_kernels.h"
 #include "batch_kernels.h"
+#include "locality_kernels.h"
 #include "walk_kernels.h"
 
 using rwnmr::LaunchShape;
@@ -89,13 +90,24 @@ struct rwnmr_ctx {
     // live images: destroying the context releases their device memory and
     // detaches them, so a later rwnmr_image_destroy only frees the handle
     std::vector<rwnmr_image *> images;
+    // device layout for images uploaded from now on (log2 brick extents)
+    int brick_sx = 1, brick_sy = 1, brick_sz = 1;
+    // locality ordering: Morton-sorted walkers, XCD-striped tiles, re-sorted
+    // every resort_every echoes (only for runs of at least min_walkers)
+    int locality = 1;
+    uint32_t resort_every = 1000;
+    uint64_t locality_min_walkers = 65536;
+    DevBuf lkeys, lidx, lsort_tmp;
+    DevBuf sortA[8], sortB[8];
 };
 
 struct rwnmr_image {
     rwnmr_ctx *ctx = nullptr;
     int32_t W = 0, H = 0, D = 0;
     int32_t bC = 0, bR = 0, bD = 0;
-    uint64_t nblocks = 0;
+    uint64_t nblocks = 0;        // blocks of the reference layout
+    rwnmr::BrickLayout layout{0, 0, 0, 0, 0};
+    uint64_t nwords = 0;         // words of the device layout (bricks padded)
     uint64_t *blocks = nullptr;        // device
     uint8_t *labels_blocked = nullptr; // device, 64 bytes per block
     int32_t n_labels = 0;
@@ -305,6 +317,9 @@ WalkArgs base_args(const rwnmr_image *img, const WalkerView &v, const rwnmr_walk
     a.D = img->D;
     a.bC = img->bC;
     a.bCR = img->bC * img->bR;
+    a.layout = img->layout;
+    a.tile_map = 0;
+    a.n_tiles = 0;
     a.shift = (int32_t)shift;
     a.x = v.x;
     a.y = v.y;
@@ -322,6 +337,95 @@ WalkArgs base_args(const rwnmr_image *img, const WalkerView &v, const rwnmr_walk
 // Upper bound on the partial-sum scratch of one walk launch; longer CPMG runs
 // are split into several launches of fewer echoes.
 constexpr uint64_t kMaxPartialBytes = 1ull << 30;
+
+// Walk in position order: every `resort_every` echoes the walkers are sorted
+// by the Morton code of their image block and gathered into context arrays,
+// the walk runs with XCD-striped tiles, and at the end the final state is
+// scattered back to the caller's order.  Per-walker results are identical to
+// the unsorted walk; echo sums differ only by summation order.
+// `a` holds the caller's (or staged) device arrays; sums is device or null.
+rwnmr_status walk_local(rwnmr_ctx *c, const rwnmr_image *img, const WalkArgs &a, int bc, int emode, uint32_t echoes,
+                        uint32_t steps_per_echo, double *sums, uint64_t max_chunk)
+{
+    const uint64_t n = a.n;
+    if (n > 0xFFFFFFFFull)
+        return fail(c, RWNMR_ERR_INVALID, "locality ordering supports up to 2^32 walkers");
+    const LaunchShape ls = c->shape;
+    const uint64_t per_tile = (uint64_t)ls.threads * ls.k;
+    const uint32_t tiles = (uint32_t)((n + per_tile - 1) / per_tile);
+    size_t tmp_bytes = 0;
+    RW_HIP(c, rwnmr::sort_pairs_temp_bytes(n, &tmp_bytes));
+    RW_HIP(c, c->lsort_tmp.reserve(tmp_bytes + 256));
+    RW_HIP(c, c->lkeys.reserve(8 * n));
+    RW_HIP(c, c->lidx.reserve(8 * n));
+    const size_t sizes[8] = {4, 4, 4, 8, 8, 4, 8, 4};
+    for (int i = 0; i < 8; ++i) {
+        RW_HIP(c, c->sortA[i].reserve(sizes[i] * n));
+        RW_HIP(c, c->sortB[i].reserve(sizes[i] * n));
+    }
+    auto state_of = [&](DevBuf *b) {
+        rwnmr::SortedState s;
+        s.x = b[0].as<int32_t>();
+        s.y = b[1].as<int32_t>();
+        s.z = b[2].as<int32_t>();
+        s.seed = b[3].as<uint64_t>();
+        s.energy = b[4].as<double>();
+        s.hits = a.hits ? b[5].as<uint32_t>() : nullptr;
+        s.penalty = a.penalty ? b[6].as<double>() : nullptr;
+        s.orig = b[7].as<uint32_t>();
+        return s;
+    };
+    rwnmr::SortedState caller;
+    caller.x = a.x;
+    caller.y = a.y;
+    caller.z = a.z;
+    caller.seed = a.seed;
+    caller.energy = a.energy;
+    caller.hits = a.hits;
+    caller.penalty = nullptr;
+    caller.orig = nullptr;
+    DevBuf *sets[2] = {c->sortA, c->sortB};
+    int which = 0;
+    rwnmr::SortedState cur = caller;
+    uint32_t *keys = c->lkeys.as<uint32_t>(), *keys_alt = keys + n;
+    uint32_t *idx = c->lidx.as<uint32_t>(), *idx_alt = idx + n;
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(c->resort_every, max_chunk));
+    uint32_t done = 0;
+    bool first = true;
+    do {
+        // sort the current state by position and gather it into the other set
+        const rwnmr::SortedState dst = state_of(sets[which]);
+        which ^= 1;
+        RW_HIP(c, rwnmr::launch_morton_keys(bc, cur.x, cur.y, cur.z, n, img->W, img->H, img->D, keys, idx, c->stream));
+        RW_HIP(c, rwnmr::sort_pairs(c->lsort_tmp.p, tmp_bytes, keys, keys_alt, idx, idx_alt, n, c->stream));
+        RW_HIP(c, rwnmr::launch_gather_walkers(dst, cur, idx_alt, first ? nullptr : cur.orig, a.penalty, n, c->stream));
+        cur = dst;
+        const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
+        WalkArgs b = a;
+        b.x = cur.x;
+        b.y = cur.y;
+        b.z = cur.z;
+        b.seed = cur.seed;
+        b.energy = cur.energy;
+        b.hits = cur.hits;
+        b.penalty = cur.penalty;
+        b.tile_map = 1;
+        b.n_tiles = tiles;
+        b.echoes = ne;
+        b.steps_per_echo = steps_per_echo;
+        b.partials = sums ? c->partials.as<double>() : nullptr;
+        b.partial_stride = tiles;
+        hipEvent_t t0 = timing_begin(c);
+        RW_HIP(c, rwnmr::launch_walk(b, bc, emode, ls, c->stream));
+        timing_end(c, t0);
+        if (sums)
+            RW_HIP(c, rwnmr::launch_reduce_rows(b.partials, (uint64_t)ne + 1, tiles, sums + done, c->stream));
+        done += ne;
+        first = false;
+    } while (done < echoes);
+    RW_HIP(c, rwnmr::launch_scatter_walkers(caller, cur, n, c->stream));
+    return RWNMR_OK;
+}
 
 // Shared CPMG driver: emode 1 (per-walker delta) or 2 (per-label delta).
 rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
@@ -386,6 +490,13 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
     }
     if (echo_sums)
         RW_HIP(c, c->partials.reserve(sizeof(double) * (size_t)blocks * (size_t)(chunk + 1) + 8));
+    const bool local = emode != 2 && shift == 0 && c->locality && w->n >= c->locality_min_walkers;
+    if (local) {
+        a.steps_per_echo = steps_per_echo;
+        if ((s = walk_local(c, img, a, bc, emode, echoes, steps_per_echo, echo_sums ? sums : nullptr, chunk)) !=
+            RWNMR_OK)
+            return s;
+    } else {
     if (a.energy == nullptr && chunk < echoes && w->n) {
         // energies must persist between launches: keep them in context memory
         RW_HIP(c, c->energy.reserve(w->n * 8 + 8));
@@ -410,6 +521,7 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         }
         done += ne;
     } while (done < echoes);
+    }
     if (!w->on_device) {
         if (echo_sums)
             RW_HIP(c, hipMemcpyAsync(echo_sums, sums, sizeof(double) * ((size_t)echoes + 1), hipMemcpyDeviceToHost,
@@ -479,8 +591,12 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
     for (DevBuf *b : {&c->x, &c->y, &c->z, &c->seed, &c->energy, &c->hits, &c->penalty, &c->x0, &c->y0, &c->z0,
                       &c->partials, &c->sums, &c->label_delta, &c->label_hits, &c->steps_inc, &c->kvec,
                       &c->set_delta, &c->flag, &c->batch_inst, &c->batch_tiles, &c->batch_rows,
-                      &c->batch_counter})
+                      &c->batch_counter, &c->lkeys, &c->lidx, &c->lsort_tmp})
         b->release();
+    for (int i = 0; i < 8; ++i) {
+        c->sortA[i].release();
+        c->sortB[i].release();
+    }
     if (c->host_flag)
         (void)hipHostFree(c->host_flag);
     timing_collect(c);
@@ -488,6 +604,28 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
         (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+rwnmr_status rwnmr_ctx_set_image_layout(rwnmr_ctx *c, int32_t sx, int32_t sy, int32_t sz)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if (sx < 0 || sy < 0 || sz < 0 || sx > 4 || sy > 4 || sz > 4)
+        return fail(c, RWNMR_ERR_INVALID, "brick shifts must be in [0, 4]");
+    c->brick_sx = sx;
+    c->brick_sy = sy;
+    c->brick_sz = sz;
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_ctx_set_locality(rwnmr_ctx *c, int32_t enable, uint32_t resort_every, uint64_t min_walkers)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    c->locality = enable != 0;
+    c->resort_every = resort_every ? resort_every : 1000;
+    c->locality_min_walkers = min_walkers;
+    return RWNMR_OK;
 }
 
 rwnmr_status rwnmr_ctx_set_timing(rwnmr_ctx *c, int32_t enable)
@@ -593,22 +731,48 @@ rwnmr_status rwnmr_image_upload(rwnmr_ctx *c, const uint64_t *blocks, int32_t W,
         delete img;
         return fail(c, RWNMR_ERR_INVALID, "image_upload: more than 2^32 blocks");
     }
-    hipError_t e = hipMalloc((void **)&img->blocks, img->nblocks * 8);
+    // device layout: bricks of 2^sx x 2^sy x 2^sz blocks, brick grid padded
+    rwnmr::BrickLayout &L = img->layout;
+    L.sx = c->brick_sx;
+    L.sy = c->brick_sy;
+    L.sz = c->brick_sz;
+    const int nbx = (img->bC + (1 << L.sx) - 1) >> L.sx;
+    const int nby = (img->bR + (1 << L.sy) - 1) >> L.sy;
+    const int nbz = (img->bD + (1 << L.sz) - 1) >> L.sz;
+    L.nbx = nbx;
+    L.nbxy = nbx * nby;
+    img->nwords = ((uint64_t)nbx * nby * nbz) << (L.sx + L.sy + L.sz);
+    if (img->nwords > 0xFFFFFFFFull) {
+        delete img;
+        return fail(c, RWNMR_ERR_INVALID, "image_upload: more than 2^32 blocks after brick padding");
+    }
+    // device word of every reference block
+    std::vector<uint32_t> where(img->nblocks);
+    for (int32_t bz = 0; bz < img->bD; ++bz)
+        for (int32_t by = 0; by < img->bR; ++by)
+            for (int32_t bx = 0; bx < img->bC; ++bx)
+                where[(size_t)bx + (size_t)by * img->bC + (size_t)bz * img->bC * img->bR] =
+                    rwnmr::brick_block_index(bx << 2, by << 2, bz << 2, L);
+    std::vector<uint64_t> dev_words(img->nwords, 0ull);
+    for (uint64_t b = 0; b < img->nblocks; ++b)
+        dev_words[where[b]] = blocks[b];
+    hipError_t e = hipMalloc((void **)&img->blocks, img->nwords * 8);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(img->blocks, blocks, img->nblocks * 8, hipMemcpyHostToDevice, c->stream);
+        e = hipMemcpyAsync(img->blocks, dev_words.data(), img->nwords * 8, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && labels) {
         for (size_t i = 0; i < (size_t)W * H * D; ++i)
             if (labels[i] >= (uint8_t)n_labels) {
+                (void)hipStreamSynchronize(c->stream);
                 (void)hipFree(img->blocks);
                 delete img;
                 return fail(c, RWNMR_ERR_INVALID, "image_upload: label out of range");
             }
-        std::vector<uint8_t> lb(img->nblocks * 64, 0);
+        std::vector<uint8_t> lb(img->nwords * 64, 0);
         for (int32_t z = 0; z < D; ++z)
             for (int32_t y = 0; y < H; ++y)
                 for (int32_t x = 0; x < W; ++x) {
-                    const size_t b = (size_t)(x >> 2) + (size_t)(y >> 2) * img->bC +
-                                     (size_t)(z >> 2) * img->bC * img->bR;
+                    const size_t b = where[(size_t)(x >> 2) + (size_t)(y >> 2) * img->bC +
+                                           (size_t)(z >> 2) * img->bC * img->bR];
                     const int bit = (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
                     lb[b * 64 + bit] = labels[((size_t)z * H + y) * W + x];
                 }
@@ -1004,6 +1168,7 @@ extern "C" rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *c, const rwnmr_image *img
     a.D = img->D;
     a.bC = img->bC;
     a.bCR = img->bC * img->bR;
+    a.layout = img->layout;
     a.label_delta = label_delta;
     a.n_labels = (int32_t)nl;
     a.n = n;
@@ -1087,6 +1252,7 @@ extern "C" rwnmr_status rwnmr_cpmg_batch_run(rwnmr_ctx *c, const rwnmr_cpmg_inst
         d.D = img->D;
         d.bC = img->bC;
         d.bCR = img->bC * img->bR;
+        d.layout = img->layout;
         d.bc = q.bc;
         d.x = q.walkers.x;
         d.y = q.walkers.y;

@@ -113,4 +113,40 @@ RW_HD uint32_t bit_index(int ix, int iy, int iz)
     return (uint32_t)((ix & 3) | ((iy & 3) << 2) | ((iz & 3) << 4));
 }
 
+// Device layout of the bitBlock words.  Blocks are grouped in bricks of
+// 2^sx x 2^sy x 2^sz blocks stored contiguously (bricks in x-fastest order
+// over a grid of nbx x nby x nbz bricks), so one cache line holds a compact
+// 3-D neighbourhood instead of a thin row along x.  sx = sy = sz = 0 is the
+// reference layout (block index bx + by*bC + bz*bC*bR).
+struct BrickLayout {
+    int sx, sy, sz;      // log2 of the brick extent in blocks per axis
+    int nbx, nbxy;       // bricks along x, bricks per z-layer
+};
+
+RW_HD uint32_t brick_block_index(int ix, int iy, int iz, const BrickLayout &L)
+{
+    const uint32_t bx = (uint32_t)ix >> 2, by = (uint32_t)iy >> 2, bz = (uint32_t)iz >> 2;
+    const uint32_t brick = (bx >> L.sx) + (by >> L.sy) * (uint32_t)L.nbx + (bz >> L.sz) * (uint32_t)L.nbxy;
+    const uint32_t inner = (bx & ((1u << L.sx) - 1u)) | ((by & ((1u << L.sy) - 1u)) << L.sx) |
+                           ((bz & ((1u << L.sz) - 1u)) << (L.sx + L.sy));
+    return (brick << (L.sx + L.sy + L.sz)) | inner;
+}
+
+// 30-bit Morton (Z-order) code of block coordinates (10 bits per axis),
+// used to order walkers by position.
+RW_HD uint32_t spread3(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+RW_HD uint32_t morton3(uint32_t bx, uint32_t by, uint32_t bz)
+{
+    return spread3(bx) | (spread3(by) << 1) | (spread3(bz) << 2);
+}
+
 }  // namespace rwnmr

@@ -47,7 +47,8 @@ __device__ __forceinline__ double wave_sum(double v)
 // Image geometry as plain values (uniform across the grid).
 struct Geom {
     const uint64_t *blocks;
-    int W, H, D, bC, bCR;
+    int W, H, D;
+    BrickLayout L;
 };
 
 __device__ __forceinline__ Geom make_geom(const WalkArgs &a)
@@ -57,9 +58,20 @@ __device__ __forceinline__ Geom make_geom(const WalkArgs &a)
     g.W = a.W;
     g.H = a.H;
     g.D = a.D;
-    g.bC = a.bC;
-    g.bCR = a.bCR;
+    g.L = a.layout;
     return g;
+}
+
+// Tile walked by this workgroup.  Workgroups are dealt round-robin to the
+// XCDs (workgroup b runs on XCD b % 8), so with tile_map == 1 XCD j walks the
+// j-th contiguous eighth of the (position-sorted) tiles and its L2 holds one
+// compact region of the image.  Returns n_tiles or more for a spare block.
+__device__ __forceinline__ uint64_t tile_of_block(const WalkArgs &a)
+{
+    if (!a.tile_map)
+        return blockIdx.x;
+    const uint32_t per_xcd = (a.n_tiles + kXcds - 1) / kXcds;
+    return (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 }
 
 // Registers of one walker.  ix/iy/iz are the image-voxel (folded)
@@ -142,7 +154,7 @@ __device__ __forceinline__ Move propose(const Geom &g, Walker &w)
     const int nix = mv.ax == 0 ? ni : ix;
     const int niy = mv.ax == 1 ? ni : iy;
     const int niz = mv.ax == 2 ? ni : iz;
-    mv.blk = block_index(nix, niy, niz, g.bC, g.bCR);
+    mv.blk = brick_block_index(nix, niy, niz, g.L);
     mv.bit = bit_index(nix, niy, niz);
     return mv;
 }

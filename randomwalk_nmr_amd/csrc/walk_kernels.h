@@ -3,17 +3,24 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "walk_common.h"
+
 namespace rwnmr {
+
+static constexpr uint32_t kXcds = 8;     // MI355X: 8 accelerator dies, each with its own L2
 
 static constexpr int kMaxLabels = 8;   // solid labels per image (per-label relaxivity)
 
 // Everything one walk launch reads or writes.  Walker arrays are device
 // pointers of length n (structure of arrays).
 struct WalkArgs {
-    const uint64_t *blocks;          // bitBlock words, reference layout
+    const uint64_t *blocks;          // bitBlock words in the image's device layout
     const uint8_t *labels_blocked;   // optional: 64 label bytes per block (bit order)
     int32_t W, H, D;                 // image columns, rows, depth
     int32_t bC, bCR;                 // block columns, block columns * block rows
+    BrickLayout layout;              // device word order of the blocks
+    uint32_t tile_map;               // 0: block b walks tile b; 1: tiles striped per XCD
+    uint32_t n_tiles;                // tiles of the launch (tile_map == 1)
     int32_t shift;                   // log2(voxel division)
     int32_t *x, *y, *z;
     uint64_t *seed;

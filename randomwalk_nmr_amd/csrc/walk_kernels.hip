@@ -27,7 +27,10 @@ template <int BC, int EMODE, int K>
 __global__ __launch_bounds__(max_threads_for_k(K)) void walk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kMaxWaves];
-    walk_tile<BC, EMODE, K>(a, blockIdx.x, a.partials ? a.partials + blockIdx.x : nullptr, a.partial_stride, lds);
+    const uint64_t tile = tile_of_block(a);
+    if (a.tile_map && tile >= a.n_tiles)
+        return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
+    walk_tile<BC, EMODE, K>(a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride, lds);
 }
 
 // Voxel-division path (shift > 0): walker coordinates are sub-voxel units
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(1024) void walk_kernel_shift(const WalkArgs a)
                 iy = ny >> sh;
                 iz = nz >> sh;
             }
-            const uint32_t blk = block_index(ix, iy, iz, a.bC, a.bCR);
+            const uint32_t blk = brick_block_index(ix, iy, iz, a.layout);
             const uint32_t bit = bit_index(ix, iy, iz);
             const uint64_t word = active ? a.blocks[blk] : 0ull;
             if (!((word >> bit) & 1ull)) {
@@ -281,7 +284,11 @@ static hipError_t launch_bc_emode(const WalkArgs &a, const LaunchShape &ls, hipS
         return hipGetLastError();
     }
     const uint64_t per_block = (uint64_t)ls.threads * ls.k;
-    const uint32_t blocks = (uint32_t)((a.n + per_block - 1) / per_block);
+    const uint32_t tiles = (uint32_t)((a.n + per_block - 1) / per_block);
+    if (a.tile_map && a.n_tiles != tiles)
+        return hipErrorInvalidValue;
+    // XCD-striped tiles: a grid of 8 * ceil(tiles / 8) workgroups (spares exit)
+    const uint32_t blocks = a.tile_map ? ((tiles + kXcds - 1) / kXcds) * kXcds : tiles;
     switch (ls.k) {
     case 1: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 1>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
     case 2: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 2>), dim3(blocks), dim3(ls.threads), 0, st, a); break;

@@ -57,6 +57,29 @@ def test_cpmg_matches_oracle(ctx, bc, shape):
 
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
+@pytest.mark.parametrize("layout", [(0, 0, 0), (1, 1, 1), (2, 1, 0)])
+@pytest.mark.parametrize("shape", [(256, 1), (128, 2)])
+def test_locality_order_and_brick_layout_match_oracle(ctx, bc, layout, shape):
+    """Morton-sorted walkers with XCD-striped tiles (re-sorted every 7
+    echoes) on a brick-layout image: walkers bit-exact, sums to 1e-12."""
+    print(f"[test] locality bc={bc} layout={layout} launch={shape}", flush=True)
+    solid, seeds, x, y, z, pen = _case(37, 22, 29, 3.5, 0.35, 5003, 17)
+    ctx.set_image_layout(*layout)
+    ctx.set_locality(True, resort_every=7, min_walkers=0)
+    ctx.set_launch(*shape)
+    try:
+        img = R.Image.from_solid(ctx, solid)
+        g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=30, steps_per_echo=3, bc=bc, penalty=pen)
+    finally:
+        ctx.set_image_layout(1, 1, 1)
+        ctx.set_locality(True, resort_every=1000, min_walkers=65536)
+        ctx.set_launch(256, 1)
+    o = O.walk(O.bitblock_pack(solid), (37, 22, 29), x, y, z, seeds, echoes=30, steps_per_echo=3, bc=bc, penalty=pen)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+
+
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 def test_map_matches_oracle(ctx, bc):
     print(f"[test] map bc={bc}", flush=True)
     solid, seeds, x, y, z, _ = _case(24, 24, 24, 4, 0.3, 777, 5)
