@@ -127,10 +127,14 @@ def main():
     avg_kernel_s = kern_ms * 1e-3 / max(launches, 1)
     steps_per_launch = walker_steps * args.steps / max(launches, 1)
     achieved = steps_per_launch * BYTES_PER_STEP / avg_kernel_s
+    # HBM bytes per launch come from the committed FETCH_SIZE / WRITE_SIZE profile of this
+    # same command; they are only reported when that profile was taken at this launch shape.
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        if prof.get("launch") == f"{tpb},{k}":
+            traffic = prof.get("hbm_bytes_per_launch")
 
     sums = SUMS.cpu().numpy()
     decay_ok = bool(np.all(np.isfinite(sums)) and sums[0] == N and np.all(np.diff(sums) <= 0))
