@@ -84,6 +84,15 @@ rwnmr_status rwnmr_ctx_set_image_layout(rwnmr_ctx *ctx, int32_t sx, int32_t sy, 
  * identical with it on or off; echo sums differ only in summation order.
  * Default on, every 1000 echoes, from 65,536 walkers. */
 rwnmr_status rwnmr_ctx_set_locality(rwnmr_ctx *ctx, int32_t enable, uint32_t resort_every, uint64_t min_walkers);
+/* Periodic and mirror CPMG / map walks (shift 0, uniform or per-walker
+ * delta) on an unfolded copy of the image: the mirror tiling of period 2d
+ * (or the periodic one of period d) stored once with Z-order voxel
+ * addresses, so a step needs no fold (replaces the per-step fold of
+ * NMR_cpmg_cuda.cpp:1991-2015).  Built per image on first use: 2W x 2H x 2D
+ * bits for mirror (128 MiB at 512^3), W x H x D for periodic; volumes over
+ * 2^32 voxels use the folded kernels.  Results are identical either way.
+ * Default on. */
+rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *ctx, int32_t enable);
 
 /* Upload a packed image in the reference bitBlock layout: block index
  * (x>>2) + (y>>2)*bC + (z>>2)*bC*bR, bit (x&3) + 4*(y&3) + 16*(z&3),

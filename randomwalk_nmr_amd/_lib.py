@@ -74,6 +74,7 @@ SIGNATURES = {
     "rwnmr_ctx_get_launch": (I32, [P, ct.POINTER(U32), ct.POINTER(U32)]),
     "rwnmr_ctx_set_image_layout": (I32, [P, I32, I32, I32]),
     "rwnmr_ctx_set_locality": (I32, [P, I32, U32, U64]),
+    "rwnmr_ctx_set_unfolded": (I32, [P, I32]),
     "rwnmr_ctx_set_timing": (I32, [P, I32]),
     "rwnmr_ctx_get_timing": (I32, [P, ct.POINTER(D), ct.POINTER(U64), I32]),
     "rwnmr_ctx_autotune": (I32, [P, P, ct.POINTER(Walkers), U32, I32, U32, ct.POINTER(U32), ct.POINTER(U32),

@@ -17,8 +17,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["walk_kernels.hip", "batch_kernels.hip", "backward_kernels.hip", "locality_kernels.hip", "capi.hip",
-               "drivers.hip"]
+HIP_SOURCES = ["walk_kernels.hip", "zwalk_kernels.hip", "batch_kernels.hip", "backward_kernels.hip",
+               "locality_kernels.hip", "capi.hip", "drivers.hip"]
 CXX_SOURCES = ["laplace.cpp", "host_sim.cpp"]
 
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

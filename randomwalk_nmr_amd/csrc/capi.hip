@@ -18,6 +18,7 @@
 #include "backward_kernels.h"
 #include "batch_kernels.h"
 #include "locality_kernels.h"
+#include "zwalk_kernels.h"
 #include "walk_kernels.h"
 
 using rwnmr::LaunchShape;
@@ -99,6 +100,15 @@ struct rwnmr_ctx {
     uint64_t locality_min_walkers = 65536;
     DevBuf lkeys, lidx, lsort_tmp;
     DevBuf sortA[8], sortB[8];
+    // periodic / mirror walks on the unfolded image (built per image on first use)
+    int unfolded = 1;
+};
+
+// Unfolded image of one boundary condition (dilated addresses).
+struct ZImage {
+    uint32_t *words = nullptr;
+    rwnmr::ZLayout zl{};
+    int state = 0;   // 0: not built, 1: built, -1: not available (over 2^32 voxels or no memory)
 };
 
 struct rwnmr_image {
@@ -111,6 +121,8 @@ struct rwnmr_image {
     uint64_t *blocks = nullptr;        // device
     uint8_t *labels_blocked = nullptr; // device, 64 bytes per block
     int32_t n_labels = 0;
+    // unfolded bit volumes: [0] periodic (W x H x D), [1] mirror (2W x 2H x 2D)
+    mutable ZImage z[2];
 };
 
 namespace {
@@ -334,6 +346,66 @@ WalkArgs base_args(const rwnmr_image *img, const WalkerView &v, const rwnmr_walk
     return a;
 }
 
+// Unfolded image for a periodic or mirror walk, built on the context stream
+// on first use; null when the walk runs on the folded kernels (option off,
+// noflux, more than 2^32 voxels in the unfolded volume, or no memory for it).
+const ZImage *zimage_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
+{
+    if (!c->unfolded || (bc != rwnmr::kPeriodic && bc != rwnmr::kMirror))
+        return nullptr;
+    ZImage &z = img->z[bc == rwnmr::kMirror ? 1 : 0];
+    if (z.state == 0) {
+        const uint32_t f = bc == rwnmr::kMirror ? 2u : 1u;
+        if (!rwnmr::zlayout_make(f * (uint32_t)img->W, f * (uint32_t)img->H, f * (uint32_t)img->D, z.zl)) {
+            z.state = -1;
+            return nullptr;
+        }
+        if (hipMalloc((void **)&z.words, rwnmr::zimage_words(z.zl) * 4) != hipSuccess) {
+            (void)hipGetLastError();
+            z.words = nullptr;
+            z.state = -1;
+            return nullptr;
+        }
+        if (rwnmr::launch_zimage_build(img->blocks, img->layout, img->W, img->H, img->D,
+                                       bc == rwnmr::kMirror ? 1 : 0, z.zl, z.words, c->stream) != hipSuccess) {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(z.words);
+            z.words = nullptr;
+            z.state = -1;
+            return nullptr;
+        }
+        z.state = 1;
+    }
+    return z.state == 1 ? &z : nullptr;
+}
+
+void free_zimages(rwnmr_image *img)
+{
+    for (ZImage &z : img->z) {
+        if (z.words)
+            (void)hipFree(z.words);
+        z.words = nullptr;
+        z.state = 0;
+    }
+}
+
+// Use the unfolded-image kernel for this walk when one applies.
+void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, WalkArgs &a)
+{
+    a.zwords = nullptr;
+    if ((emode != 0 && emode != 1) || a.shift != 0)
+        return;
+    if (const ZImage *z = zimage_for(c, img, bc)) {
+        a.zwords = z->words;
+        a.zl = z->zl;
+    }
+}
+
+hipError_t launch_any_walk(const WalkArgs &a, int bc, int emode, const LaunchShape &ls, hipStream_t st)
+{
+    return a.zwords ? rwnmr::launch_zwalk(a, emode, ls, st) : rwnmr::launch_walk(a, bc, emode, ls, st);
+}
+
 // Upper bound on the partial-sum scratch of one walk launch; longer CPMG runs
 // are split into several launches of fewer echoes.
 constexpr uint64_t kMaxPartialBytes = 1ull << 30;
@@ -353,8 +425,10 @@ rwnmr_status walk_local(rwnmr_ctx *c, const rwnmr_image *img, const WalkArgs &a,
     const LaunchShape ls = c->shape;
     const uint64_t per_tile = (uint64_t)ls.threads * ls.k;
     const uint32_t tiles = (uint32_t)((n + per_tile - 1) / per_tile);
+    // keys: the unfolded word address (zl.bits - 5 bits) or the 30-bit Morton code of the block
+    const int key_bits = a.zwords ? (int)std::max<uint32_t>(1u, a.zl.bits > 5 ? a.zl.bits - 5 : 1u) : 30;
     size_t tmp_bytes = 0;
-    RW_HIP(c, rwnmr::sort_pairs_temp_bytes(n, &tmp_bytes));
+    RW_HIP(c, rwnmr::sort_pairs_temp_bytes(n, key_bits, &tmp_bytes));
     RW_HIP(c, c->lsort_tmp.reserve(tmp_bytes + 256));
     RW_HIP(c, c->lkeys.reserve(8 * n));
     RW_HIP(c, c->lidx.reserve(8 * n));
@@ -396,8 +470,12 @@ rwnmr_status walk_local(rwnmr_ctx *c, const rwnmr_image *img, const WalkArgs &a,
         // sort the current state by position and gather it into the other set
         const rwnmr::SortedState dst = state_of(sets[which]);
         which ^= 1;
-        RW_HIP(c, rwnmr::launch_morton_keys(bc, cur.x, cur.y, cur.z, n, img->W, img->H, img->D, keys, idx, c->stream));
-        RW_HIP(c, rwnmr::sort_pairs(c->lsort_tmp.p, tmp_bytes, keys, keys_alt, idx, idx_alt, n, c->stream));
+        if (a.zwords)
+            RW_HIP(c, rwnmr::launch_zkeys(cur.x, cur.y, cur.z, n, a.zl, keys, idx, c->stream));
+        else
+            RW_HIP(c, rwnmr::launch_morton_keys(bc, cur.x, cur.y, cur.z, n, img->W, img->H, img->D, keys, idx,
+                                                c->stream));
+        RW_HIP(c, rwnmr::sort_pairs(c->lsort_tmp.p, tmp_bytes, keys, keys_alt, idx, idx_alt, n, key_bits, c->stream));
         RW_HIP(c, rwnmr::launch_gather_walkers(dst, cur, idx_alt, first ? nullptr : cur.orig, a.penalty, n, c->stream));
         cur = dst;
         const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
@@ -416,7 +494,7 @@ rwnmr_status walk_local(rwnmr_ctx *c, const rwnmr_image *img, const WalkArgs &a,
         b.partials = sums ? c->partials.as<double>() : nullptr;
         b.partial_stride = tiles;
         hipEvent_t t0 = timing_begin(c);
-        RW_HIP(c, rwnmr::launch_walk(b, bc, emode, ls, c->stream));
+        RW_HIP(c, launch_any_walk(b, bc, emode, ls, c->stream));
         timing_end(c, t0);
         if (sums)
             RW_HIP(c, rwnmr::launch_reduce_rows(b.partials, (uint64_t)ne + 1, tiles, sums + done, c->stream));
@@ -471,6 +549,7 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
             }
         }
     }
+    attach_zimage(c, img, bc, emode, a);
     const uint32_t blocks = rwnmr::walk_grid_blocks(w->n, c->shape, (int)shift);
     a.partial_stride = blocks;
     double *sums = nullptr;
@@ -512,7 +591,7 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         a.echoes = ne;
         a.partials = echo_sums ? c->partials.as<double>() : nullptr;
         hipEvent_t t0 = timing_begin(c);
-        RW_HIP(c, rwnmr::launch_walk(a, bc, emode, c->shape, c->stream));
+        RW_HIP(c, launch_any_walk(a, bc, emode, c->shape, c->stream));
         timing_end(c, t0);
         if (echo_sums) {
             // rows 0..ne of this launch are echoes done..done+ne (row 0 repeats
@@ -579,6 +658,7 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (rwnmr_image *img : c->images) {
+        free_zimages(img);
         if (img->blocks)
             (void)hipFree(img->blocks);
         if (img->labels_blocked)
@@ -625,6 +705,14 @@ rwnmr_status rwnmr_ctx_set_locality(rwnmr_ctx *c, int32_t enable, uint32_t resor
     c->locality = enable != 0;
     c->resort_every = resort_every ? resort_every : 1000;
     c->locality_min_walkers = min_walkers;
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *c, int32_t enable)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    c->unfolded = enable != 0;
     return RWNMR_OK;
 }
 
@@ -807,6 +895,7 @@ void rwnmr_image_destroy(rwnmr_image *img)
         rwnmr_ctx *c = img->ctx;
         (void)hipStreamSynchronize(c->stream);
         c->images.erase(std::remove(c->images.begin(), c->images.end(), img), c->images.end());
+        free_zimages(img);
         if (img->blocks)
             (void)hipFree(img->blocks);
         if (img->labels_blocked)
@@ -836,12 +925,13 @@ rwnmr_status rwnmr_map_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *
     WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
     a.energy = nullptr;  // the map walk never touches energies
     a.echoes = 1;
+    attach_zimage(c, img, bc, 0, a);
     uint64_t left = steps;
     while (left > 0) {
         const uint32_t now = (uint32_t)std::min<uint64_t>(left, 1ull << 30);
         a.steps_per_echo = now;
         hipEvent_t t0 = timing_begin(c);
-        RW_HIP(c, rwnmr::launch_walk(a, bc, 0, c->shape, c->stream));
+        RW_HIP(c, launch_any_walk(a, bc, 0, c->shape, c->stream));
         timing_end(c, t0);
         left -= now;
     }

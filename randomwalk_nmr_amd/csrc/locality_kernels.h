@@ -17,9 +17,9 @@ struct SortedState {
 
 hipError_t launch_morton_keys(int bc, const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n, int W, int H,
                               int D, uint32_t *keys, uint32_t *iota, hipStream_t st);
-hipError_t sort_pairs_temp_bytes(uint64_t n, size_t *bytes);
+hipError_t sort_pairs_temp_bytes(uint64_t n, int end_bit, size_t *bytes);
 hipError_t sort_pairs(void *temp, size_t bytes, const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in,
-                      uint32_t *vals_out, uint64_t n, hipStream_t st);
+                      uint32_t *vals_out, uint64_t n, int end_bit, hipStream_t st);
 hipError_t launch_gather_walkers(const SortedState &dst, const SortedState &src, const uint32_t *perm,
                                  const uint32_t *orig_src, const double *penalty, uint64_t n, hipStream_t st);
 hipError_t launch_scatter_walkers(const SortedState &out, const SortedState &s, uint64_t n, hipStream_t st);

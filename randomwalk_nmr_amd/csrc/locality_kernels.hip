@@ -4,7 +4,7 @@
 // A walk step reads one bitBlock word near the walker, so the cache
 // footprint of a workgroup is the union of its walkers' neighbourhoods.
 // Ordering walkers by the Morton code of their image block (radix sort of
-// 30-bit keys) makes every tile a compact cluster, and striping the sorted
+// 30-bit keys; the unfolded-image walk sorts by its own Z-order word address) makes every tile a compact cluster, and striping the sorted
 // tiles per XCD (tile_of_block) gives each XCD's L2 one region of the image.
 // Per-walker results do not depend on the order (each walker owns its RNG
 // stream), so the permutation is undone exactly at the end.
@@ -98,16 +98,17 @@ hipError_t launch_morton_keys(int bc, const int32_t *x, const int32_t *y, const 
     return hipGetLastError();
 }
 
-hipError_t sort_pairs_temp_bytes(uint64_t n, size_t *bytes)
+hipError_t sort_pairs_temp_bytes(uint64_t n, int end_bit, size_t *bytes)
 {
     return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                              (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 30);
+                                              (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, end_bit);
 }
 
 hipError_t sort_pairs(void *temp, size_t bytes, const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in,
-                      uint32_t *vals_out, uint64_t n, hipStream_t st)
+                      uint32_t *vals_out, uint64_t n, int end_bit, hipStream_t st)
 {
-    return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys_in, keys_out, vals_in, vals_out, (int)n, 0, 30, st);
+    return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys_in, keys_out, vals_in, vals_out, (int)n, 0, end_bit,
+                                              st);
 }
 
 hipError_t launch_gather_walkers(const SortedState &dst, const SortedState &src, const uint32_t *perm,

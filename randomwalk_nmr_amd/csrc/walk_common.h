@@ -149,4 +149,176 @@ RW_HD uint32_t morton3(uint32_t bx, uint32_t by, uint32_t bz)
     return spread3(bx) | (spread3(by) << 1) | (spread3(bz) << 2);
 }
 
+// ------------------------------------------------------------------------
+// Unfolded image with dilated voxel addresses (periodic and mirror walks).
+//
+// Mirror boundaries reflect the image into a tiling of period 2d per axis,
+// so fold_mirror(g) == fold of (g mod 2d) inside one 2W x 2H x 2D "unfolded"
+// image (u < d -> u, else 2d - 1 - u); periodic boundaries have period d.
+// Storing that unfolded image once turns both boundaries into one periodic
+// walk with no per-step fold.  Voxel (ux, uy, uz) of the unfolded image has
+// the bit address A = dep(ux, Mx) | dep(uy, My) | dep(uz, Mz), where dep
+// deposits the coordinate's bits into the positions of a mask and the three
+// masks interleave round-robin from bit 0 (x0 y0 z0 x1 y1 z1 ...: one 32-bit
+// word is 4 x 4 x 2 voxels, one 128-byte line 16 x 8 x 8, a Z-order curve
+// above that).  Axes whose bits run out are skipped.  A unit move along one
+// axis is then three 32-bit operations on A (a dilated add):
+//     +1: A' = (A & ~M) | (((A | ~M) + lowbit(M)) & M)
+//     -1: A' = (A & ~M) | (((A | ~M) + M) & M)
+// both wrapping at 2^k.  A period that is not a power of two needs one fix
+// after the add (field == dep(P) -> 0 going up; field == all ones ->
+// dep(P - 1) going down).
+
+struct ZLayout {
+    uint32_t m[3];       // address bits of x, y, z
+    uint32_t period[3];  // W, H, D (periodic) or 2W, 2H, 2D (mirror)
+    uint32_t bits;       // address bits in use (the volume holds 2^bits voxels)
+    uint32_t wrapfix;    // 1 when some period is not a power of two
+};
+
+RW_HD uint32_t zceil_log2(uint32_t v)
+{
+    uint32_t k = 0;
+    while (k < 32 && (1ull << k) < (uint64_t)v)
+        ++k;
+    return k;
+}
+
+// Layout for field periods px, py, pz (each >= 1).  Returns 0 when the
+// address would need more than 32 bits.
+RW_HD int zlayout_make(uint32_t px, uint32_t py, uint32_t pz, ZLayout &L)
+{
+    const uint32_t k[3] = {zceil_log2(px), zceil_log2(py), zceil_log2(pz)};
+    const uint32_t total = k[0] + k[1] + k[2];
+    if (total > 32)
+        return 0;
+    uint32_t used[3] = {0, 0, 0};
+    L.m[0] = L.m[1] = L.m[2] = 0;
+    L.period[0] = px;
+    L.period[1] = py;
+    L.period[2] = pz;
+    uint32_t pos = 0;
+    while (pos < total)
+        for (int a = 0; a < 3 && pos < total; ++a)
+            if (used[a] < k[a]) {
+                L.m[a] |= 1u << pos;
+                ++pos;
+                ++used[a];
+            }
+    L.bits = total;
+    L.wrapfix = 0;
+    for (int a = 0; a < 3; ++a)
+        if ((1ull << k[a]) != (uint64_t)L.period[a])
+            L.wrapfix = 1;
+    return 1;
+}
+
+// Deposit the low bits of v into the set bits of m (lowest first).
+RW_HD uint32_t zdeposit(uint32_t v, uint32_t m)
+{
+    uint32_t r = 0;
+    for (uint32_t b = 1; m; m &= m - 1u, b <<= 1)
+        if (v & b)
+            r |= m & (~m + 1u);
+    return r;
+}
+
+// Inverse of zdeposit: gather the bits of a at the set bits of m.
+RW_HD uint32_t zextract(uint32_t a, uint32_t m)
+{
+    uint32_t r = 0;
+    for (uint32_t b = 1; m; m &= m - 1u, b <<= 1)
+        if (a & m & (~m + 1u))
+            r |= b;
+    return r;
+}
+
+RW_HD uint32_t floor_mod(int32_t g, uint32_t p)
+{
+    const int64_t m = (int64_t)g % (int64_t)p;
+    return (uint32_t)(m < 0 ? m + (int64_t)p : m);
+}
+
+// Image voxel of unfolded coordinate u on an axis of image extent d.
+RW_HD int zunfold(uint32_t u, int d, int mirror)
+{
+    return (mirror && u >= (uint32_t)d) ? 2 * d - 1 - (int)u : (int)u;
+}
+
+// One direction of the step table, indexed by idx = 2 * (s % 3) + (s & 1)
+// (a bijection onto the residue s % 6 by the Chinese remainder theorem).
+struct ZDir {
+    uint32_t nm;      // ~mask of the moving axis
+    uint32_t add;     // lowbit(mask) going up, mask going down
+    int32_t dp;       // packed unbounded displacement: +-1 (x), +-2^10 (y), +-2^20 (z)
+    uint32_t m;       // mask of the moving axis
+    uint32_t wrapv;   // field value that needs the wrap fix (never matches if none)
+    uint32_t fixv;    // replacement field value
+};
+
+// s % 6 residue for table index idx (inverse of the CRT pairing).
+RW_HD uint32_t zdir_residue(uint32_t idx)
+{
+    return (0x521430u >> (4u * idx)) & 0xFu;   // idx 0..5 -> 0, 3, 4, 1, 2, 5
+}
+
+RW_HD ZDir zdir_make(const ZLayout &L, uint32_t idx)
+{
+    const uint32_t r = zdir_residue(idx);
+    const int ax = move_axis(r), sg = move_sign(r);
+    const uint32_t m = L.m[ax];
+    ZDir d;
+    d.m = m;
+    d.nm = ~m;
+    d.add = sg > 0 ? (m & (~m + 1u)) : m;
+    d.dp = sg * (ax == 0 ? 1 : (ax == 1 ? 1 << 10 : 1 << 20));
+    const uint32_t k = (uint32_t)__builtin_popcount(m);
+    const uint32_t p = L.period[ax];
+    if ((1ull << k) == (uint64_t)p) {
+        d.wrapv = 0xFFFFFFFFu;   // no axis owns every bit, so this never matches
+        d.fixv = 0;
+    } else if (sg > 0) {
+        d.wrapv = zdeposit(p, m);
+        d.fixv = 0;
+    } else {
+        d.wrapv = m;
+        d.fixv = zdeposit(p - 1u, m);
+    }
+    return d;
+}
+
+// Candidate address after one move.
+RW_HD uint32_t zmove(uint32_t A, const ZDir &d, int wrapfix)
+{
+    uint32_t c = (A & d.nm) | (((A | d.nm) + d.add) & ~d.nm);
+    if (wrapfix && (c & d.m) == d.wrapv)
+        c = (c & d.nm) | d.fixv;
+    return c;
+}
+
+// Byte sum of the 64-bit state (== state mod 3, since 256 == 1 mod 3) and
+// the table index; sb <= 2040 and (sb * 2731) >> 13 == sb / 3 there.
+RW_HD uint32_t zbyte_sum(uint64_t s)
+{
+    uint32_t t = 0;
+    for (int i = 0; i < 8; ++i)
+        t += (uint32_t)(s >> (8 * i)) & 0xFFu;
+    return t;
+}
+
+RW_HD uint32_t zdir_index_from_sum(uint32_t sb, uint32_t lo)
+{
+    const uint32_t q = (sb * 2731u) >> 13;
+    return 2u * (sb - 3u * q) + (lo & 1u);
+}
+
+// Packed displacement -> three int32 (|dx|, |dy|, |dz| <= 511).
+RW_HD void zunpack(int32_t p, int32_t &dx, int32_t &dy, int32_t &dz)
+{
+    dx = (int32_t)((uint32_t)p << 22) >> 22;
+    const int32_t p1 = (p - dx) >> 10;
+    dy = (int32_t)((uint32_t)p1 << 22) >> 22;
+    dz = (p1 - dy) >> 10;
+}
+
 }  // namespace rwnmr

@@ -37,6 +37,8 @@ struct WalkArgs {
     double *partials;                // nullable: [(echoes + 1)][partial_stride]
     uint32_t partial_stride;         // = grid blocks of the launch
     uint32_t *error_flag;            // device word, bit 0: walker started outside the image
+    const uint32_t *zwords;          // unfolded image (periodic / mirror fast path), else null
+    ZLayout zl;                      // its dilated address layout
 };
 
 struct LaunchShape {

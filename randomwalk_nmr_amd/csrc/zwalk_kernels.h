@@ -1,0 +1,32 @@
+// zwalk_kernels.h -- host-side view of the unfolded-image walk kernels
+// (periodic and mirror CPMG / map walks; see zwalk_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "walk_common.h"
+#include "walk_kernels.h"
+
+namespace rwnmr {
+
+// Steps between flushes of the packed displacement into x, y, z (each packed
+// component must stay within +-511).
+static constexpr uint32_t kZFlush = 511;
+
+// Words of the unfolded bit volume for layout zl (at least one).
+inline uint64_t zimage_words(const ZLayout &zl) { return zl.bits > 5 ? (1ull << (zl.bits - 5)) : 1ull; }
+
+// Build the unfolded image: bit A of out is voxel (zunfold(ux), zunfold(uy),
+// zunfold(uz)) of the device bitBlock image `blocks` (brick layout L), 1 for
+// addresses outside the periods.
+hipError_t launch_zimage_build(const uint64_t *blocks, const BrickLayout &L, int W, int H, int D, int mirror,
+                               const ZLayout &zl, uint32_t *out, hipStream_t st);
+
+// keys[i] = unfolded address of walker i >> 5 (its 32-bit word), iota[i] = i.
+hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n, const ZLayout &zl,
+                        uint32_t *keys, uint32_t *iota, hipStream_t st);
+
+// CPMG (emode 1) or map (emode 0) walk on a.zwords / a.zl; a.shift must be 0.
+hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hipStream_t st);
+
+}  // namespace rwnmr

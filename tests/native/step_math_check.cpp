@@ -4,6 +4,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../randomwalk_nmr_amd/csrc/walk_common.h"
 
 using namespace rwnmr;
@@ -25,6 +27,138 @@ static int textbook_reflect(int g, int d)
     if (t < 0)
         t += 2 * d;
     return t < d ? t : 2 * d - 1 - t;
+}
+
+// Unfolded-image arithmetic (walk_common.h, "Unfolded image with dilated
+// voxel addresses") against direct coordinate arithmetic.
+static void check_unfolded()
+{
+    // s % 3 by byte sum and a 24-bit multiply; the table index pairs with s % 6
+    for (uint32_t sb = 0; sb <= 2040; ++sb)
+        EXPECT(((sb * 2731u) >> 13) == sb / 3u, "q sb=%u\n", sb);
+    uint64_t s = 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < 5000000; ++i) {
+        s = xorshift64(s);
+        EXPECT(zbyte_sum(s) % 3u == s % 3u, "byte sum %llu\n", (unsigned long long)s);
+        EXPECT(zdir_residue(zdir_index_from_sum(zbyte_sum(s), (uint32_t)s)) == s % 6u, "residue %llu\n",
+               (unsigned long long)s);
+    }
+    for (uint64_t v = 0; v < 100000; ++v)
+        EXPECT(zdir_residue(zdir_index_from_sum(zbyte_sum(v), (uint32_t)v)) == v % 6u, "residue small\n");
+    for (uint64_t v : {~0ull, 1ull << 63, 0xFFFFFFFFull, 0x8080808080808080ull})
+        EXPECT(zdir_residue(zdir_index_from_sum(zbyte_sum(v), (uint32_t)v)) == v % 6u, "residue edge\n");
+    // unfold == the reference folds, for any unbounded coordinate
+    for (int d = 1; d <= 13; ++d)
+        for (int g = -100; g <= 100; ++g) {
+            EXPECT(zunfold(floor_mod(g, 2u * d), d, 1) == fold_mirror(g, d), "unfold mirror g=%d d=%d\n", g, d);
+            EXPECT(zunfold(floor_mod(g, (uint32_t)d), d, 0) == fold_periodic(g, d), "unfold periodic\n");
+        }
+    // layouts: disjoint masks covering the address bits; every move of every
+    // voxel lands on the wrapped neighbour
+    const uint32_t periods[][3] = {{1, 1, 1}, {2, 2, 2}, {8, 8, 8}, {16, 4, 2}, {5, 7, 3}, {10, 14, 6},
+                                   {12, 2, 9}, {1, 6, 4}, {26, 18, 22}, {32, 32, 1}};
+    for (const auto &p : periods) {
+        ZLayout L{};
+        EXPECT(zlayout_make(p[0], p[1], p[2], L) == 1, "layout\n");
+        EXPECT((L.m[0] & L.m[1]) == 0 && (L.m[0] & L.m[2]) == 0 && (L.m[1] & L.m[2]) == 0, "disjoint\n");
+        const uint64_t all = (uint64_t)L.m[0] | L.m[1] | L.m[2];
+        EXPECT(all == (L.bits == 32 ? 0xFFFFFFFFull : ((1ull << L.bits) - 1)), "cover\n");
+        for (uint32_t z = 0; z < p[2]; ++z)
+            for (uint32_t y = 0; y < p[1]; ++y)
+                for (uint32_t x = 0; x < p[0]; ++x) {
+                    const uint32_t u[3] = {x, y, z};
+                    const uint32_t A = zdeposit(x, L.m[0]) | zdeposit(y, L.m[1]) | zdeposit(z, L.m[2]);
+                    for (int a = 0; a < 3; ++a)
+                        EXPECT(zextract(A, L.m[a]) == u[a], "extract\n");
+                    for (uint32_t idx = 0; idx < 6; ++idx) {
+                        const ZDir dir = zdir_make(L, idx);
+                        const uint32_t r = zdir_residue(idx);
+                        const int ax = move_axis(r), sg = move_sign(r);
+                        uint32_t v[3] = {x, y, z};
+                        v[ax] = (uint32_t)(((int)v[ax] + sg + (int)p[ax]) % (int)p[ax]);
+                        const uint32_t want = zdeposit(v[0], L.m[0]) | zdeposit(v[1], L.m[1]) | zdeposit(v[2], L.m[2]);
+                        EXPECT(zmove(A, dir, (int)L.wrapfix) == want, "zmove p=%u,%u,%u u=%u,%u,%u idx=%u\n", p[0],
+                               p[1], p[2], x, y, z, idx);
+                        EXPECT(dir.dp == sg * (ax == 0 ? 1 : ax == 1 ? 1024 : 1 << 20), "dp\n");
+                    }
+                }
+    }
+    // packed displacement round trip
+    uint64_t t = 777;
+    for (int i = 0; i < 200000; ++i) {
+        t = xorshift64(t);
+        const int n = (int)(t % 512u);
+        int dx = 0, dy = 0, dz = 0, pk = 0;
+        for (int j = 0; j < n; ++j) {
+            t = xorshift64(t);
+            const uint32_t r = (uint32_t)(t % 6u);
+            const int ax = move_axis(r), sg = move_sign(r);
+            (ax == 0 ? dx : ax == 1 ? dy : dz) += sg;
+            pk += sg * (ax == 0 ? 1 : ax == 1 ? 1024 : 1 << 20);
+        }
+        int ux, uy, uz;
+        zunpack(pk, ux, uy, uz);
+        EXPECT(ux == dx && uy == dy && uz == dz, "unpack %d %d %d\n", dx, dy, dz);
+    }
+    // host walk on the unfolded volume == walk with the reference fold per step
+    for (int mirror = 0; mirror <= 1; ++mirror) {
+        const int W = 7, H = 5, D = 6;
+        uint8_t solid[D][H][W];
+        uint64_t g = 99 + mirror;
+        for (int z = 0; z < D; ++z)
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    g = xorshift64(g);
+                    solid[z][y][x] = (g % 10u) < 3u;
+                }
+        solid[2][2][3] = 0;
+        const uint32_t f = mirror ? 2u : 1u;
+        ZLayout L{};
+        zlayout_make(f * W, f * H, f * D, L);
+        std::vector<uint32_t> vol((L.bits > 5 ? (1u << (L.bits - 5)) : 1u), 0u);
+        for (uint64_t A = 0; A < (1ull << L.bits); ++A) {
+            const uint32_t ux = zextract((uint32_t)A, L.m[0]), uy = zextract((uint32_t)A, L.m[1]),
+                           uz = zextract((uint32_t)A, L.m[2]);
+            uint32_t bit = 1;
+            if (ux < L.period[0] && uy < L.period[1] && uz < L.period[2])
+                bit = solid[zunfold(uz, D, mirror)][zunfold(uy, H, mirror)][zunfold(ux, W, mirror)];
+            vol[A >> 5] |= bit << (A & 31);
+        }
+        for (int w = 0; w < 50; ++w) {
+            uint64_t sa = 1000 + w, sb = sa;
+            int X = 3, Y = 2, Z = 2, hx = 0, hy = 0, hz = 0;   // folded walker: unbounded X, Y, Z
+            uint32_t A = zdeposit(floor_mod(3, L.period[0]), L.m[0]) | zdeposit(2, L.m[1]) | zdeposit(2, L.m[2]);
+            int pk = 0, hitsA = 0, hitsB = 0;
+            for (int st = 0; st < 3000; ++st) {
+                sa = xorshift64(sa);
+                const uint32_t r = mod6(sa);
+                const int ax = move_axis(r), sg = move_sign(r);
+                const int nx = X + (ax == 0 ? sg : 0), ny = Y + (ax == 1 ? sg : 0), nz = Z + (ax == 2 ? sg : 0);
+                const int fx = mirror ? fold_mirror(nx, W) : fold_periodic(nx, W);
+                const int fy = mirror ? fold_mirror(ny, H) : fold_periodic(ny, H);
+                const int fz = mirror ? fold_mirror(nz, D) : fold_periodic(nz, D);
+                if (solid[fz][fy][fx])
+                    ++hitsA;
+                else
+                    X = nx, Y = ny, Z = nz;
+                sb = xorshift64(sb);
+                const uint32_t idx = zdir_index_from_sum(zbyte_sum(sb), (uint32_t)sb);
+                const ZDir dir = zdir_make(L, idx);
+                const uint32_t c = zmove(A, dir, (int)L.wrapfix);
+                if ((vol[c >> 5] >> (c & 31)) & 1u)
+                    ++hitsB;
+                else
+                    A = c, pk += dir.dp;
+                if (st % 500 == 499) {
+                    int dx, dy, dz;
+                    zunpack(pk, dx, dy, dz);
+                    hx += dx, hy += dy, hz += dz;
+                    pk = 0;
+                }
+            }
+            EXPECT(hitsA == hitsB && X == 3 + hx && Y == 2 + hy && Z == 2 + hz, "walk mirror=%d w=%d\n", mirror, w);
+        }
+    }
 }
 
 int main()
@@ -86,6 +220,7 @@ int main()
                 EXPECT(block_index(x, y, z, 3, 9) == (uint32_t)(x / 4 + (y / 4) * 3 + (z / 4) * 9), "blk\n");
                 EXPECT(bit_index(x, y, z) == (uint32_t)(x % 4 + 4 * (y % 4) + 16 * (z % 4)), "bit\n");
             }
+    check_unfolded();
     if (fails) {
         printf("%d failures\n", fails);
         return 1;

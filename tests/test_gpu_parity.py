@@ -79,6 +79,57 @@ def test_locality_order_and_brick_layout_match_oracle(ctx, bc, layout, shape):
     _assert_echoes(g["echo_sums"], o["echo_sums"])
 
 
+@pytest.mark.parametrize("bc", ["periodic", "mirror"])
+@pytest.mark.parametrize("dims", [(32, 16, 8), (16, 16, 16), (30, 21, 17), (9, 1, 5), (2, 3, 64)])
+@pytest.mark.parametrize("unfolded", [True, False])
+@pytest.mark.parametrize("shape", [(256, 1), (512, 2), (256, 4)])
+def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
+    """Periodic / mirror CPMG on the unfolded image (power-of-two periods:
+    plain dilated wrap; others: wrap fix; a dimension of 1) and on the folded
+    kernel: walkers bit-exact, echo sums to 1e-12, for every launch shape."""
+    print(f"[test] unfolded={unfolded} bc={bc} dims={dims} launch={shape}", flush=True)
+    W, H, D = dims
+    rng = np.random.default_rng(W * 10000 + H * 100 + D)
+    solid = (rng.random((D, H, W)) < 0.3).astype(np.uint8)
+    n = 1500
+    seeds, x, y, z = R.seed_and_place(solid, 31 + W, n)
+    pen = 1.0 - rng.uniform(0.001, 0.02, n)
+    ctx.set_unfolded(unfolded)
+    ctx.set_launch(*shape)
+    try:
+        img = R.Image.from_solid(ctx, solid)
+        g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=37, steps_per_echo=5, bc=bc, penalty=pen)
+        m = R.map_run(ctx, img, x, y, z, seeds, steps=37 * 5, bc=bc)
+    finally:
+        ctx.set_unfolded(True)
+        ctx.set_launch(256, 1)
+    o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=37, steps_per_echo=5, bc=bc, penalty=pen)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(m[k], o[k]), k
+
+
+@pytest.mark.parametrize("bc", ["periodic", "mirror"])
+@pytest.mark.parametrize("spe", [1, 600])
+def test_unfolded_long_echoes_and_locality_match_oracle(ctx, bc, spe):
+    """Steps per echo above the packed-displacement flush interval (511) and
+    the Z-order re-sort of the unfolded walk on a power-of-two image."""
+    print(f"[test] unfolded locality bc={bc} spe={spe}", flush=True)
+    solid, seeds, x, y, z, pen = _case(32, 32, 32, 3.5, 0.35, 4001, 23)
+    echoes = 9 if spe > 1 else 400
+    ctx.set_locality(True, resort_every=4, min_walkers=0)
+    try:
+        img = R.Image.from_solid(ctx, solid)
+        g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=echoes, steps_per_echo=spe, bc=bc, penalty=pen)
+    finally:
+        ctx.set_locality(True, resort_every=1000, min_walkers=65536)
+    o = O.walk(O.bitblock_pack(solid), (32, 32, 32), x, y, z, seeds, echoes=echoes, steps_per_echo=spe, bc=bc,
+               penalty=pen)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+
+
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 def test_map_matches_oracle(ctx, bc):
     print(f"[test] map bc={bc}", flush=True)

@@ -206,3 +206,18 @@ def test_pfgse_gradient_list_and_k():
         acc += 3000.0 / 9.0
     assert np.array_equal(g[0], np.array(ref)) and not g[1].any()
     assert L.rwnmr_pfgse_wave_vector(100.0, 0.1, 42.576) == (0.1 * 1e-3) * (42.576 * 1e6) * (100.0 * 1e-8)
+
+
+def test_step_arithmetic_host_build(tmp_path):
+    """The kernels' step arithmetic (walk_common.h) compiled for the host and
+    checked exhaustively: mod 6 by popcounts, folds, incremental mirror steps,
+    bitBlock addressing, and the unfolded-image path (byte-sum mod 3, dilated
+    moves with wrap, unfold == reference fold, packed displacement, and a host
+    walk on the unfolded volume equal to the per-step-fold walk)."""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = str(tmp_path / "step_math_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(here, "native", "step_math_check.cpp")],
+                   check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip().endswith("OK"), out.stdout[-2000:]
