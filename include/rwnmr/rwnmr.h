@@ -90,9 +90,11 @@ rwnmr_status rwnmr_ctx_set_locality(rwnmr_ctx *ctx, int32_t enable, uint32_t res
  * addresses, so a step needs no fold (replaces the per-step fold of
  * NMR_cpmg_cuda.cpp:1991-2015).  Built per image on first use: 2W x 2H x 2D
  * bits for mirror (128 MiB at 512^3), W x H x D for periodic; volumes over
- * 2^32 voxels use the folded kernels.  Results are identical either way.
- * Default on. */
-rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *ctx, int32_t enable);
+ * 2^32 voxels use the folded kernels.  mode 0: folded kernels; 1: unfolded;
+ * 2 (default): unfolded, each walker also keeping the 16-byte chunk
+ * (8 x 4 x 4 voxels) of its last lookup in registers so most steps read no
+ * memory.  Results are identical in every mode. */
+rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *ctx, int32_t mode);
 
 /* Upload a packed image in the reference bitBlock layout: block index
  * (x>>2) + (y>>2)*bC + (z>>2)*bC*bR, bit (x&3) + 4*(y&3) + 16*(z&3),

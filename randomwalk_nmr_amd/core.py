@@ -78,9 +78,10 @@ class Context:
         check(load().rwnmr_ctx_set_locality(self.handle, int(enable), int(resort_every), int(min_walkers)),
               self.handle)
 
-    def set_unfolded(self, enable: bool = True):
-        """Periodic / mirror walks on the unfolded image (no per-step fold)."""
-        check(load().rwnmr_ctx_set_unfolded(self.handle, int(enable)), self.handle)
+    def set_unfolded(self, mode: int = 2):
+        """Periodic / mirror walks: 0 folded kernels, 1 unfolded image (no
+        per-step fold), 2 unfolded + per-walker 16-byte chunk cache (default)."""
+        check(load().rwnmr_ctx_set_unfolded(self.handle, int(mode)), self.handle)
 
     def set_timing(self, enable: bool = True):
         check(load().rwnmr_ctx_set_timing(self.handle, int(enable)), self.handle)

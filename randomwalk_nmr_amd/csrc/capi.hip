@@ -100,8 +100,9 @@ struct rwnmr_ctx {
     uint64_t locality_min_walkers = 65536;
     DevBuf lkeys, lidx, lsort_tmp;
     DevBuf sortA[8], sortB[8];
-    // periodic / mirror walks on the unfolded image (built per image on first use)
-    int unfolded = 1;
+    // periodic / mirror walks on the unfolded image (built per image on first use):
+    // 0 folded kernels, 1 unfolded, 2 unfolded with the per-walker chunk cache
+    int unfolded = 2;
 };
 
 // Unfolded image of one boundary condition (dilated addresses).
@@ -398,6 +399,7 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
     if (const ZImage *z = zimage_for(c, img, bc)) {
         a.zwords = z->words;
         a.zl = z->zl;
+        a.zcache = c->unfolded >= 2 ? 1u : 0u;
     }
 }
 
@@ -712,7 +714,9 @@ rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *c, int32_t enable)
 {
     if (!c)
         return RWNMR_ERR_INVALID;
-    c->unfolded = enable != 0;
+    if (enable < 0 || enable > 2)
+        return fail(c, RWNMR_ERR_INVALID, "set_unfolded: mode must be 0, 1 or 2");
+    c->unfolded = enable;
     return RWNMR_OK;
 }
 

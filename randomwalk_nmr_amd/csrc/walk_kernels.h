@@ -39,6 +39,7 @@ struct WalkArgs {
     uint32_t *error_flag;            // device word, bit 0: walker started outside the image
     const uint32_t *zwords;          // unfolded image (periodic / mirror fast path), else null
     ZLayout zl;                      // its dilated address layout
+    uint32_t zcache;                 // 1: per-walker 16-byte chunk cache in registers
 };
 
 struct LaunchShape {

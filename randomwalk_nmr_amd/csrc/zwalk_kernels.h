@@ -13,8 +13,8 @@ namespace rwnmr {
 // component must stay within +-511).
 static constexpr uint32_t kZFlush = 511;
 
-// Words of the unfolded bit volume for layout zl (at least one).
-inline uint64_t zimage_words(const ZLayout &zl) { return zl.bits > 5 ? (1ull << (zl.bits - 5)) : 1ull; }
+// Words of the unfolded bit volume for layout zl (at least one 16-byte chunk).
+inline uint64_t zimage_words(const ZLayout &zl) { return zl.bits > 7 ? (1ull << (zl.bits - 5)) : 4ull; }
 
 // Build the unfolded image: bit A of out is voxel (zunfold(ux), zunfold(uy),
 // zunfold(uz)) of the device bitBlock image `blocks` (brick layout L), 1 for

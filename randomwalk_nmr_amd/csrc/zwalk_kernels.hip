@@ -7,7 +7,9 @@
 // the pair indexes a 6-entry direction table in LDS (axis mask, dilated
 // addend, packed displacement); the dilated add gives the candidate voxel
 // address with the periodic wrap included; one 4-byte gather of the word
-// holding the voxel's bit; then either the move (address and packed
+// holding the voxel's bit (or, with the register chunk cache, a 16-byte
+// gather only when the candidate leaves the walker's cached 8 x 4 x 4-voxel
+// chunk, about one step in five); then either the move (address and packed
 // displacement) or a wall hit (hit count + 1, energy *= delta, in that
 // order as the reference).  The unbounded coordinates the reference reports
 // are rebuilt from the packed displacement every kZFlush steps.  Nothing in
@@ -35,7 +37,7 @@ __device__ __forceinline__ uint32_t zaddress(int32_t x, int32_t y, int32_t z, co
 
 // One tile of blockDim.x * K walkers; EMODE 0 = hit counting (map), 1 = CPMG
 // with per-walker or uniform delta.
-template <int EMODE, int K, bool WRAPFIX>
+template <int EMODE, int K, bool WRAPFIX, bool RCACHE>
 __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kMaxWaves];
@@ -58,6 +60,8 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
     const uint32_t echoes = a.echoes, spe = a.steps_per_echo;
 
     uint32_t A[K], C[K], Hc[K];
+    uint32_t QB[K];   // RCACHE: 128-voxel chunk (address >> 7) held in QV
+    uint4 QV[K];
     int32_t P[K], DP[K], X[K], Y[K], Z[K];
     uint64_t S[K];
     double E[K], DL[K];
@@ -76,6 +80,8 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
         DL[k] = EMODE == 1 ? ((act[k] && a.penalty) ? a.penalty[j] : a.delta_uniform) : 1.0;
         A[k] = zaddress(X[k], Y[k], Z[k], a.zl);
         P[k] = 0;
+        QB[k] = 0xFFFFFFFFu;   // no chunk: addresses >> 7 stay below 2^25
+        QV[k] = make_uint4(0u, 0u, 0u, 0u);
     });
 
     auto echo_value = [&]() {
@@ -120,11 +126,31 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
             asm volatile("" : "+v"(DP[k]));
         });
         uint32_t word[K];
-        static_for<K>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            // c < 2^bits, so the byte offset of its word is inside the volume
-            word[k] = *reinterpret_cast<const uint32_t *>(words + ((C[k] >> 3) & ~3u));
-        });
+        if (RCACHE) {
+            // each walker keeps the 16-byte chunk (8 x 4 x 4 voxels) of its last
+            // lookup; only candidates outside it read memory
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const uint32_t cid = C[k] >> 7;
+                if (cid != QB[k]) {
+                    // c < 2^bits and the volume holds at least 4 words: in bounds
+                    QV[k] = *reinterpret_cast<const uint4 *>(words + ((C[k] >> 3) & ~15u));
+                    QB[k] = cid;
+                }
+            });
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const bool up = (C[k] & 64u) != 0u;
+                const uint32_t lo = up ? QV[k].z : QV[k].x, hi = up ? QV[k].w : QV[k].y;
+                word[k] = (C[k] & 32u) ? hi : lo;
+            });
+        } else {
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                // c < 2^bits, so the byte offset of its word is inside the volume
+                word[k] = *reinterpret_cast<const uint32_t *>(words + ((C[k] >> 3) & ~3u));
+            });
+        }
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             const uint32_t wall = __builtin_amdgcn_ubfe(word[k], C[k], 1u);   // offset = C & 31
@@ -226,16 +252,24 @@ hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, ui
     return hipGetLastError();
 }
 
-template <int EMODE, bool WRAPFIX>
+template <int EMODE, bool WRAPFIX, bool RCACHE>
 static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
     switch (ls.k) {
-    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, RCACHE>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, RCACHE>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX, RCACHE>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int EMODE>
+static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
+{
+    if (a.zl.wrapfix)
+        return a.zcache ? launch_z<EMODE, true, true>(a, ls, blocks, st) : launch_z<EMODE, true, false>(a, ls, blocks, st);
+    return a.zcache ? launch_z<EMODE, false, true>(a, ls, blocks, st) : launch_z<EMODE, false, false>(a, ls, blocks, st);
 }
 
 hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hipStream_t st)
@@ -251,9 +285,9 @@ hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hip
         return hipErrorInvalidValue;
     const uint32_t blocks = a.tile_map ? (uint32_t)(((tiles + kXcds - 1) / kXcds) * kXcds) : (uint32_t)tiles;
     if (emode == 0)
-        return a.zl.wrapfix ? launch_z<0, true>(a, ls, blocks, st) : launch_z<0, false>(a, ls, blocks, st);
+        return launch_z_mode<0>(a, ls, blocks, st);
     if (emode == 1)
-        return a.zl.wrapfix ? launch_z<1, true>(a, ls, blocks, st) : launch_z<1, false>(a, ls, blocks, st);
+        return launch_z_mode<1>(a, ls, blocks, st);
     return hipErrorInvalidValue;
 }
 

@@ -81,12 +81,13 @@ def test_locality_order_and_brick_layout_match_oracle(ctx, bc, layout, shape):
 
 @pytest.mark.parametrize("bc", ["periodic", "mirror"])
 @pytest.mark.parametrize("dims", [(32, 16, 8), (16, 16, 16), (30, 21, 17), (9, 1, 5), (2, 3, 64)])
-@pytest.mark.parametrize("unfolded", [True, False])
+@pytest.mark.parametrize("unfolded", [2, 1, 0])
 @pytest.mark.parametrize("shape", [(256, 1), (512, 2), (256, 4)])
 def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
-    """Periodic / mirror CPMG on the unfolded image (power-of-two periods:
-    plain dilated wrap; others: wrap fix; a dimension of 1) and on the folded
-    kernel: walkers bit-exact, echo sums to 1e-12, for every launch shape."""
+    """Periodic / mirror CPMG and map walks on the unfolded image with and
+    without the per-walker chunk cache (power-of-two periods: plain dilated
+    wrap; others: wrap fix; a dimension of 1) and on the folded kernel:
+    walkers bit-exact, echo sums to 1e-12, for every launch shape."""
     print(f"[test] unfolded={unfolded} bc={bc} dims={dims} launch={shape}", flush=True)
     W, H, D = dims
     rng = np.random.default_rng(W * 10000 + H * 100 + D)
@@ -101,7 +102,7 @@ def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
         g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=37, steps_per_echo=5, bc=bc, penalty=pen)
         m = R.map_run(ctx, img, x, y, z, seeds, steps=37 * 5, bc=bc)
     finally:
-        ctx.set_unfolded(True)
+        ctx.set_unfolded(2)
         ctx.set_launch(256, 1)
     o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=37, steps_per_echo=5, bc=bc, penalty=pen)
     _assert_walkers(g, o)
