@@ -25,6 +25,8 @@ struct BatchInstance {
     uint32_t echoes, steps_per_echo;
     double *partials;    // [(echoes + 1)][n_tiles]
     uint32_t n_tiles;
+    const uint32_t *zwords;   // unfolded image (periodic / mirror), else null: folded walk
+    ZLayout zl;
 };
 
 struct BatchTile {

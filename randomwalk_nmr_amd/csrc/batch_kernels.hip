@@ -6,8 +6,9 @@
 // sized to the device's resident capacity then loops: one workgroup pops the
 // next tile index from a global atomic counter, loads that instance's
 // descriptor (image pointer, dimensions, boundary, relaxivity, echo count)
-// and walks the tile with the same walk_tile() as the single-instance
-// kernel, writing one FP64 partial per (echo, tile).  Every workgroup leaves
+// and walks the tile with the same tile walk as the single-instance kernels
+// (zwalk_tile on the instance's unfolded image for periodic and mirror
+// boundaries, walk_tile otherwise), writing one FP64 partial per (echo, tile).  Every workgroup leaves
 // the loop once the counter passes the tile count, so the grid always drains.
 // The reference runs one sample per process invocation
 // (B/NMR_Simulation/NMR_cpmg_cuda.cpp:929-1774); this has no counterpart.
@@ -16,6 +17,7 @@
 
 #include "batch_kernels.h"
 #include "walk_device.h"
+#include "zwalk_device.h"
 
 namespace rwnmr {
 
@@ -23,6 +25,8 @@ template <int K>
 __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
 {
     __shared__ double lds[kEchoChunk][kMaxWaves];
+    __shared__ uint4 tab[6];
+    __shared__ uint2 fix[6];
     __shared__ uint32_t next;
     for (;;) {
         if (threadIdx.x == 0)
@@ -63,7 +67,20 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         a.partials = nullptr;
         a.partial_stride = 0;
         a.error_flag = b.error_flag;
+        a.zwords = I.zwords;
+        a.zl = I.zl;
+        a.zcache = 1;
         double *part = I.partials + bt.tile;
+        if (I.zwords) {
+            // the previous tile's table reads ended at its last block barrier
+            zbuild_table(I.zl, tab, fix);
+            __syncthreads();
+            if (I.zl.wrapfix)
+                zwalk_tile<1, K, true, true>(a, bt.tile, part, I.n_tiles, lds, tab, fix);
+            else
+                zwalk_tile<1, K, false, true>(a, bt.tile, part, I.n_tiles, lds, tab, fix);
+            continue;
+        }
         switch (I.bc) {
         case kNoflux: walk_tile<kNoflux, 1, K>(a, bt.tile, part, I.n_tiles, lds); break;
         case kPeriodic: walk_tile<kPeriodic, 1, K>(a, bt.tile, part, I.n_tiles, lds); break;

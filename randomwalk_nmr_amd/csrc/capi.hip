@@ -1361,6 +1361,11 @@ extern "C" rwnmr_status rwnmr_cpmg_batch_run(rwnmr_ctx *c, const rwnmr_cpmg_inst
         d.steps_per_echo = q.steps_per_echo;
         d.partials = part + off;
         d.n_tiles = nt;
+        rwnmr::WalkArgs za;
+        memset(&za, 0, sizeof(za));
+        attach_zimage(c, img, q.bc, 1, za);
+        d.zwords = za.zwords;
+        d.zl = za.zl;
         for (uint32_t t = 0; t < nt; ++t) {
             const uint64_t walkers = std::min<uint64_t>(tile_walkers, q.walkers.n - (uint64_t)t * tile_walkers);
             const double cost = (double)q.echoes * q.steps_per_echo * (double)walkers;
