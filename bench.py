@@ -9,6 +9,17 @@ full CPMG run (walkers reset to their start state, then 1.2e11 walker-steps)
 with all inputs resident in HBM.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+
+--workload selects the other measured configurations (one JSON line each):
+  small   128^3 CPMG, 10k walkers, 2000 echoes, plus the 128-bin Laplace
+          inversion (lambda 1e-2), with the full CPU reference loop beside it;
+  pfgse   256^3 PFGSE, 1M walkers, 16 diffusion times x 64 gradients, rho 0
+          and 10 um/s evaluated on one trajectory, fitted D(Delta);
+  batch   64 uneven CPMG instances (64^3..256^3, 10k..500k walkers, 4000
+          echoes) in one persistent launch vs 64 sequential launches;
+  fit     8 per-label relaxivities + T2bulk on a 256^3 Voronoi-labelled pack,
+          1M walkers, 4000 echoes: gradient fit (backward kernel) vs the
+          genetic algorithm, GPU wall time to 1e-3 RMS decay error.
 """
 from __future__ import annotations
 
@@ -43,6 +54,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch", default="", help="threads,k (skips the autotuner)")
+    ap.add_argument("--workload", default="cpmg512", choices=["cpmg512", "small", "pfgse", "batch", "fit"])
+    ap.add_argument("--ga-time-limit", type=float, default=150.0, help="fit workload: GA wall-time cap (s)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter scale)")
     args = ap.parse_args()
@@ -58,6 +71,10 @@ def main():
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs on one GPU "
               "(walkers would shard across GPUs with one all-reduce of the echo sums)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "cpmg512":
+        {"small": wl_small, "pfgse": wl_pfgse, "batch": wl_batch, "fit": wl_fit}[args.workload](args)
+        return
 
     import numpy as np
     import torch
@@ -196,6 +213,275 @@ def main():
     out["roofline"]["achieved"] = achieved / 1e9
     out["roofline"]["peak"] = HBM_PEAK / 1e9
     print(json.dumps(out), flush=True)
+    img.close()
+    ctx.close()
+
+
+# ------------------------------------------------------------------ other workloads
+
+def _line(d):
+    print(json.dumps(d), flush=True)
+
+
+def _timed(fn, reps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def wl_small(args):
+    """128^3 sphere pack (r 8, porosity 0.25), voxel 1 um, D0 2.5, rho 10,
+    T2b 2800 ms, 10,000 walkers, mirror, 2000 echoes at TE 0.2 ms (3 steps),
+    then the Laplace inversion onto 128 bins over 0.1-1e4 ms at lambda 1e-2."""
+    import numpy as np
+    import torch
+    import randomwalk_nmr_amd as R
+    from randomwalk_nmr_amd.laplace import apply_laplace
+    voxel, D0, rho, t2b, te, N, E, S = 1.0, 2.5, 10.0, 2800.0, 0.2, 10_000, 2000, 128
+    dt = R.time_step(voxel, D0)
+    spe = int(round(te / dt))
+    solid, phi = R.sphere_pack(S, S, S, 8.0, 8.0, 0.25, args.seed)
+    seeds, x0, y0, z0 = R.seed_and_place(solid, args.seed, N)
+    delta = R.penalty(rho, voxel, D0)
+    ctx = R.Context(0)
+    img = R.Image.from_solid(ctx, solid)
+    dev = torch.device("cuda:0")
+    X0, Y0, Z0 = (torch.from_numpy(a).to(dev) for a in (x0, y0, z0))
+    S0 = torch.from_numpy(seeds.view(np.int64)).to(dev)
+    X, Y, Z, SD = X0.clone(), Y0.clone(), Z0.clone(), S0.clone()
+    EN = torch.ones(N, dtype=torch.float64, device=dev)
+    SUMS = torch.zeros(E + 1, dtype=torch.float64, device=dev)
+
+    def run():
+        X.copy_(X0), Y.copy_(Y0), Z.copy_(Z0), SD.copy_(S0), EN.fill_(1.0)
+        R.cpmg_run(ctx, img, X, Y, Z, SD, echoes=E, steps_per_echo=spe, bc="mirror", delta=delta, energy=EN,
+                   echo_sums=SUMS, sync=True)
+    run()
+    sec = _timed(run, 10)
+    steps = float(N) * spe * E
+    M = SUMS.cpu().numpy()
+    t = np.arange(E + 1) * spe * dt
+    decay = np.exp(-t / t2b) * M / M[0]
+    t0 = time.perf_counter()
+    bins, amps, lam = apply_laplace(t, decay, min_t2=0.1, max_t2=1e4, num_t2_bins=128, lam=1e-2)
+    lap_ms = (time.perf_counter() - t0) * 1e3
+    t2_peak = float(bins[int(np.argmax(amps))])
+    import oracle as O
+    blocks = R.pack_bitblocks(solid)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    secs1, es1, xo, yo, zo = O.cpmg_reference_loop(blocks, (S, S, S), x0, y0, z0, seeds, delta, echoes=E,
+                                                   steps_per_echo=spe, bc="mirror", use_digit_sum=True, threads=1)
+    secsN, *_ = O.cpmg_reference_loop(blocks, (S, S, S), x0, y0, z0, seeds, delta, echoes=E, steps_per_echo=spe,
+                                      bc="mirror", use_digit_sum=True, threads=threads)
+    exact = bool(np.array_equal(X.cpu().numpy(), xo) and np.array_equal(Y.cpu().numpy(), yo) and
+                 np.array_equal(Z.cpu().numpy(), zo))
+    echo_rel = float(np.max(np.abs(M - es1) / np.abs(es1)))
+    _line({"metric": "walker-steps/s, CPMG 128^3 10k walkers", "value": steps / sec, "unit": "walker-steps/s",
+           "ms_per_run": sec * 1e3, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"CPMG {S}^3 sphere pack (r 8, porosity {phi:.3f}), {N} walkers, {E} echoes x "
+                                  f"{spe} steps, mirror, voxel {voxel} um, rho {rho} um/s, T2b {t2b} ms"},
+           "laplace": {"bins": 128, "lambda": lam, "ms": lap_ms, "t2_peak_ms": t2_peak},
+           "cpu_baseline": {"value": steps / secsN, "unit": "walker-steps/s", "cores": threads, "kind": "port",
+                            "sample": "all 10,000 walkers x 2000 echoes (reference loop order, digit-sum mod6)",
+                            "single_thread_value": steps / secs1},
+           "checks": {"positions_bit_exact_vs_cpu": exact, "echo_sums_max_rel_vs_cpu": echo_rel}})
+    img.close()
+    ctx.close()
+
+
+def wl_pfgse(args):
+    """256^3 sphere pack (r 10, porosity 0.30), voxel 1 um, D0 2.5, 1M
+    walkers, mirror; 16 diffusion times logspace(0.5, 40 ms), 64 gradients
+    per time with k sqrt(D0 Delta) linear in [0, 3]; narrow pulse; rho 0 and
+    10 um/s as two energy sets of one trajectory."""
+    import ctypes as ct
+    import numpy as np
+    import torch
+    import randomwalk_nmr_amd as R
+    from randomwalk_nmr_amd._lib import load
+    voxel, D0, N, S, NK, ND = 1.0, 2.5, 1_000_000, 256, 64, 16
+    dt = R.time_step(voxel, D0)
+    deltas = np.logspace(np.log10(0.5), np.log10(40.0), ND)
+    steps = np.array([int(round(d / dt)) for d in deltas])
+    steps_inc = np.diff(np.concatenate([[0], steps])).astype(np.uint32)
+    kmax = 3.0 / np.sqrt(D0 * deltas)
+    K = (np.linspace(0.0, 1.0, NK)[None, :] * kmax[:, None])
+    solid, phi = R.sphere_pack(S, S, S, 10.0, 10.0, 0.30, args.seed)
+    seeds, x0, y0, z0 = R.seed_and_place(solid, args.seed, N)
+    set_delta = np.array([1.0, R.penalty(10.0, voxel, D0)])
+    ctx = R.Context(0)
+    img = R.Image.from_solid(ctx, solid)
+    dev = torch.device("cuda:0")
+    X0, Y0, Z0 = (torch.from_numpy(a).to(dev) for a in (x0, y0, z0))
+    S0 = torch.from_numpy(seeds.view(np.int64)).to(dev)
+    X, Y, Z, SD = X0.clone(), Y0.clone(), Z0.clone(), S0.clone()
+    kx = torch.from_numpy(K).to(dev)
+    kz0 = torch.zeros_like(kx)
+    st = torch.from_numpy(steps_inc.view(np.int32)).to(dev)
+    sd = torch.from_numpy(set_delta).to(dev)
+    Mt = torch.zeros((2, ND, NK), dtype=torch.float64, device=dev)
+    msd = torch.zeros((ND, 3), dtype=torch.float64, device=dev)
+
+    def run():
+        X.copy_(X0), Y.copy_(Y0), Z.copy_(Z0), SD.copy_(S0)
+        R.pfgse_run(ctx, img, X, Y, Z, SD, steps_inc=st, kx=kx, ky=kz0, kz=kz0, voxel=voxel, bc="mirror",
+                    x0=X0, y0=Y0, z0=Z0, set_delta=sd, M=Mt, msd=msd, sync=True)
+    run()
+    sec = _timed(run, 3)
+    M = Mt.cpu().numpy()
+    walker_steps = float(N) * steps[-1]
+    terms = 2.0 * N * ND * NK
+    D = []
+    for s_ in range(2):
+        row = []
+        for i in range(ND):
+            ratio = M[s_, i] / M[s_, i, 0]
+            keep = ratio >= np.exp(-1.0)
+            lhs = np.log(ratio[keep])
+            rhs = -(K[i, keep] ** 2) * deltas[i]
+            d, e = ct.c_double(), ct.c_double()
+            load().rwnmr_stejskal_tanner(rhs.ctypes.data, lhs.ctypes.data, int(keep.sum()), ct.byref(d), ct.byref(e))
+            row.append(d.value)
+        D.append(row)
+    msd_np = msd.cpu().numpy()
+    D_msd = (msd_np.sum(axis=1) * voxel ** 2 / N / (6.0 * deltas)).tolist()
+    import oracle as O
+    ns = 2000
+    blocks = R.pack_bitblocks(solid)
+    t0 = time.perf_counter()
+    o = O.pfgse(blocks, (S, S, S), x0[:ns], y0[:ns], z0[:ns], seeds[:ns], steps_inc=steps_inc, kx=K,
+                ky=np.zeros_like(K), kz=np.zeros_like(K), voxel=voxel, bc="mirror", set_delta=set_delta, threads=1)
+    cpu_sec = time.perf_counter() - t0
+    g = R.pfgse_run(ctx, img, x0[:ns], y0[:ns], z0[:ns], seeds[:ns], steps_inc=steps_inc, kx=K, ky=np.zeros_like(K),
+                    kz=np.zeros_like(K), voxel=voxel, bc="mirror", set_delta=set_delta)
+    err = float(np.max(np.abs(g["M"] - o["M"]) / np.abs(o["M"][..., :1])))
+    _line({"metric": "PFGSE walker-steps/s, 256^3 1M walkers, 16 Delta x 64 k x 2 rho",
+           "value": walker_steps / sec, "unit": "walker-steps/s", "ms_per_run": sec * 1e3,
+           "terms_per_s": terms / sec, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"PFGSE {S}^3 sphere pack (r 10, porosity {phi:.3f}), {N} walkers, mirror, "
+                                  f"Delta logspace(0.5, 40 ms, {ND}), {NK} k per Delta up to k sqrt(D0 Delta) = 3, "
+                                  f"rho 0 and 10 um/s on one trajectory"},
+           "D_of_Delta": {"Delta_ms": deltas.tolist(), "rho0": D[0], "rho10": D[1], "msd": D_msd},
+           "cpu_baseline": {"value": float(ns) * steps[-1] / cpu_sec, "unit": "walker-steps/s (with the 2x16x64 "
+                            "measurement per walker)", "cores": 1, "kind": "port",
+                            "sample": f"first {ns} walkers, same Delta/k sets"},
+           "checks": {"subsample_M_max_err_rel_M0": err}})
+    img.close()
+    ctx.close()
+
+
+def wl_batch(args):
+    """64 instances: side U{64,96,128,192,256}, porosity U(0.10, 0.40),
+    radius U(4, 12), walkers U(10k, 500k), rho U(5, 50) um/s, voxel 1 um,
+    4000 echoes x 3 steps, mirror."""
+    import numpy as np
+    import torch
+    import randomwalk_nmr_amd as R
+    rng = np.random.default_rng(args.seed)
+    ctx = R.Context(0)
+    dev = torch.device("cuda:0")
+    voxel, D0, E = 1.0, 2.5, 4000
+    spe = int(round(0.2 / R.time_step(voxel, D0)))
+    inst, init, total = [], [], 0.0
+    t0 = time.time()
+    for i in range(64):
+        side = int(rng.choice([64, 96, 128, 192, 256]))
+        phi_t = float(rng.uniform(0.10, 0.40))
+        r = float(rng.uniform(4.0, 12.0))
+        n = int(rng.integers(10_000, 500_001))
+        rho = float(rng.uniform(5.0, 50.0))
+        solid, _ = R.sphere_pack(side, side, side, r, r, phi_t, args.seed + i)
+        seeds, x, y, z = R.seed_and_place(solid, args.seed + 1000 + i, n)
+        img = R.Image.from_solid(ctx, solid)
+        tens = [torch.from_numpy(a).to(dev) for a in (x, y, z)] + [torch.from_numpy(seeds.view(np.int64)).to(dev)]
+        init.append(tens)
+        inst.append(dict(img=img, x=tens[0].clone(), y=tens[1].clone(), z=tens[2].clone(), seed=tens[3].clone(),
+                         energy=torch.ones(n, dtype=torch.float64, device=dev),
+                         echo_sums=torch.zeros(E + 1, dtype=torch.float64, device=dev), echoes=E,
+                         steps_per_echo=spe, bc="mirror", delta=R.penalty(rho, voxel, D0)))
+        total += float(n) * E * spe
+    log(f"batch: 64 instances built in {time.time() - t0:.1f}s, {total:.3e} walker-steps")
+
+    def reset():
+        for q, t in zip(inst, init):
+            q["x"].copy_(t[0]), q["y"].copy_(t[1]), q["z"].copy_(t[2]), q["seed"].copy_(t[3])
+            q["energy"].fill_(1.0)
+
+    def run(seq):
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        R.cpmg_batch_run(ctx, inst, sequential=seq)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+    run(False)
+    run(True)
+    tb = min(run(False) for _ in range(2))
+    sums_b = [q["echo_sums"].cpu().numpy().copy() for q in inst]
+    ts = min(run(True) for _ in range(2))
+    sums_s = [q["echo_sums"].cpu().numpy() for q in inst]
+    rel = max(float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(sums_b, sums_s))
+    _line({"metric": "batched CPMG walker-steps/s, 64 uneven instances, one persistent launch",
+           "value": total / tb, "unit": "walker-steps/s", "batched_s": tb, "sequential_s": ts,
+           "speedup_vs_sequential": ts / tb, "sequential_walker_steps_per_s": total / ts, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "64 sphere packs, side U{64..256}, porosity U(0.1,0.4), r U(4,12), "
+                                  f"walkers U(10k,500k), rho U(5,50), {E} echoes x {spe} steps, mirror",
+                      "walker_steps": total},
+           "checks": {"echo_sums_max_rel_batched_vs_sequential": rel}})
+    ctx.close()
+
+
+def wl_fit(args):
+    """256^3 sphere pack (r 6-12, porosity 0.25), solid labelled 0-7 by a
+    64-site Voronoi partition, rho_l U(5, 40), T2b 2800 ms, 1M walkers, 4000
+    echoes x 3 steps (voxel 1 um); target = forward(truth) + 0.5% Gaussian
+    noise; gradient fit (checkpoint every 500 echoes) vs GA (population 64),
+    GPU wall time until the RMS error against the noise-free decay <= 1e-3."""
+    import numpy as np
+    import torch
+    import randomwalk_nmr_amd as R
+    from randomwalk_nmr_amd.fit import CpmgModel, genetic_fit, gradient_fit, rms
+    rng = np.random.default_rng(args.seed)
+    voxel, D0, N, E, S = 1.0, 2.5, 1_000_000, 4000, 256
+    spe = int(round(0.2 / R.time_step(voxel, D0)))
+    solid, phi = R.sphere_pack(S, S, S, 6.0, 12.0, 0.25, args.seed)
+    labels = R.voronoi_labels(solid, 64, 8, args.seed + 1)
+    seeds, x, y, z = R.seed_and_place(solid, args.seed + 2, N)
+    ctx = R.Context(0)
+    img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
+    model = CpmgModel(ctx, img, x, y, z, seeds, echoes=E, steps_per_echo=spe, voxel=voxel, D0=D0, ckpt_every=500)
+    rho_true = rng.uniform(5.0, 40.0, 8)
+    t2b_true = 2800.0
+    with torch.no_grad():
+        truth = model.decay(torch.tensor(rho_true), torch.tensor(t2b_true)).cpu().numpy()
+    target = truth + 0.005 * rng.standard_normal(truth.shape)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        model.decay(torch.tensor(rho_true), torch.tensor(t2b_true))
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+    g = gradient_fit(model, target, rho0=np.full(8, 20.0), t2b0=2000.0, truth=truth, tol=1e-3, max_evals=60,
+                     log=log)
+    ga = genetic_fit(model, target, bounds_lo=np.concatenate([np.full(8, 1.0), [500.0]]),
+                     bounds_hi=np.concatenate([np.full(8, 60.0), [5000.0]]), truth=truth, tol=1e-3, population=64,
+                     time_limit=args.ga_time_limit, log=log)
+    _line({"metric": "rho-fit GPU wall time to 1e-3 RMS decay error (gradient vs GA)",
+           "value": g["seconds"], "unit": "s", "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{S}^3 sphere pack (r 6-12, porosity {phi:.3f}), 64-site Voronoi labels 0-7, "
+                                  f"{N} walkers, {E} echoes x {spe} steps, mirror, noise 0.5%, checkpoint every 500",
+                      "forward_s": fwd_s},
+           "gradient": {"converged": g["converged"], "seconds": g["seconds"], "evals": g["evals"],
+                        "rms_truth": g["rms_truth"], "rho": np.asarray(g["rho"]).tolist(), "t2b": g["t2b"]},
+           "ga": {"converged": ga["converged"], "seconds": ga["seconds"], "generations": ga["generations"],
+                  "evals": ga["evals"], "rms_truth": ga["rms_truth"], "time_limit_s": args.ga_time_limit},
+           "truth": {"rho": rho_true.tolist(), "t2b": t2b_true},
+           "ga_over_gradient": (ga["seconds"] / g["seconds"]) if g["converged"] else None})
     img.close()
     ctx.close()
 
