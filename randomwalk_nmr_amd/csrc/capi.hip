@@ -110,6 +110,8 @@ struct ZImage {
     uint32_t *words = nullptr;
     rwnmr::ZLayout zl{};
     int state = 0;   // 0: not built, 1: built, -1: not available (over 2^32 voxels or no memory)
+    uint32_t *labels = nullptr;   // 4-bit solid labels in the same order (per-label walks)
+    int lstate = 0;  // as state
 };
 
 struct rwnmr_image {
@@ -385,21 +387,60 @@ void free_zimages(rwnmr_image *img)
     for (ZImage &z : img->z) {
         if (z.words)
             (void)hipFree(z.words);
+        if (z.labels)
+            (void)hipFree(z.labels);
         z.words = nullptr;
+        z.labels = nullptr;
         z.state = 0;
+        z.lstate = 0;
     }
+}
+
+// Unfolded label volume of an image with solid labels (built on first use);
+// null when unavailable.
+const uint32_t *zlabels_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
+{
+    if (!img->labels_blocked)
+        return nullptr;
+    const ZImage *zi = zimage_for(c, img, bc);
+    if (!zi)
+        return nullptr;
+    ZImage &z = img->z[bc == rwnmr::kMirror ? 1 : 0];
+    if (z.lstate == 0) {
+        if (hipMalloc((void **)&z.labels, rwnmr::zlabel_words(z.zl) * 4) != hipSuccess) {
+            (void)hipGetLastError();
+            z.labels = nullptr;
+            z.lstate = -1;
+            return nullptr;
+        }
+        if (rwnmr::launch_zlabels_build(img->labels_blocked, img->layout, img->W, img->H, img->D,
+                                        bc == rwnmr::kMirror ? 1 : 0, z.zl, z.labels, c->stream) != hipSuccess) {
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(z.labels);
+            z.labels = nullptr;
+            z.lstate = -1;
+            return nullptr;
+        }
+        z.lstate = 1;
+    }
+    return z.lstate == 1 ? z.labels : nullptr;
 }
 
 // Use the unfolded-image kernel for this walk when one applies.
 void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, WalkArgs &a)
 {
     a.zwords = nullptr;
-    if ((emode != 0 && emode != 1) || a.shift != 0)
+    a.zlabels = nullptr;
+    if (emode < 0 || emode > 2 || a.shift != 0)
+        return;
+    const uint32_t *zlab = nullptr;
+    if (emode == 2 && !(zlab = zlabels_for(c, img, bc)))
         return;
     if (const ZImage *z = zimage_for(c, img, bc)) {
         a.zwords = z->words;
         a.zl = z->zl;
         a.zcache = c->unfolded >= 2 ? 1u : 0u;
+        a.zlabels = zlab;
     }
 }
 
@@ -1266,6 +1307,7 @@ extern "C" rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *c, const rwnmr_image *img
     a.label_delta = label_delta;
     a.n_labels = (int32_t)nl;
     a.n = n;
+    attach_zimage(c, img, bc, 2, a);
     hipEvent_t t0 = timing_begin(c);
     RW_HIP(c, rwnmr::launch_backward(a, b, bc, threads, c->stream));
     timing_end(c, t0);

@@ -40,6 +40,7 @@ struct WalkArgs {
     const uint32_t *zwords;          // unfolded image (periodic / mirror fast path), else null
     ZLayout zl;                      // its dilated address layout
     uint32_t zcache;                 // 1: per-walker 16-byte chunk cache in registers
+    const uint32_t *zlabels;         // unfolded solid labels, 4 bits per voxel (per-label walks)
 };
 
 struct LaunchShape {

@@ -48,19 +48,78 @@ __device__ __forceinline__ void zbuild_table(const ZLayout &zl, uint4 *tab, uint
     }
 }
 
+// Candidate address of one move from A: advances the RNG state s, reads the
+// direction entry (16 bytes per entry in tab_bytes, 8 in fix_bytes) and
+// returns the dilated-add result; dp receives the packed displacement.
+template <bool WRAPFIX>
+__device__ __forceinline__ uint32_t zpropose(uint64_t &s, uint32_t A, const char *tab_bytes, const char *fix_bytes,
+                                             int32_t &dp)
+{
+    s = xorshift64(s);
+    const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+    // byte sum sb <= 2040 == state (mod 3); q = sb / 3 by a 24-bit multiply;
+    // table entry 2 * (sb - 3q) + (state & 1) at 16 bytes per entry
+    const uint32_t sb = __builtin_amdgcn_sad_u8(hi, 0u, __builtin_amdgcn_sad_u8(lo, 0u, 0u));
+    const uint32_t q = __umul24(sb, 2731u) >> 13;
+    const uint32_t off = (uint32_t)__mul24((int)q, -96) + ((sb << 5) | ((lo & 1u) << 4));
+    const uint4 t = *reinterpret_cast<const uint4 *>(tab_bytes + off);
+    const uint32_t m = t.x;
+    uint32_t c = (A & ~m) | (((A | ~m) + t.y) & m);
+    if (WRAPFIX) {
+        const uint2 f = *reinterpret_cast<const uint2 *>(fix_bytes + (off >> 1));
+        c = (c & m) == f.x ? ((c & ~m) | f.y) : c;
+    }
+    dp = (int32_t)t.z;
+    // keep the displacement in this one table read (the compiler would
+    // otherwise re-read it inside the move branch)
+    asm volatile("" : "+v"(dp));
+    return c;
+}
+
+// Register chunk cache: make QV hold the 16-byte chunk of address c (read
+// from words only when c left the chunk QB).  c < 2^bits and the volume holds
+// at least 4 words, so the read is in bounds.
+__device__ __forceinline__ void zchunk_fill(const char *words, uint32_t c, uint32_t &QB, uint4 &QV)
+{
+    const uint32_t cid = c >> 7;
+    if (cid != QB) {
+        QV = *reinterpret_cast<const uint4 *>(words + ((c >> 3) & ~15u));
+        QB = cid;
+    }
+}
+
+// The 32-bit word of QV that holds address c's bit.
+__device__ __forceinline__ uint32_t zchunk_word(uint32_t c, const uint4 &QV)
+{
+    const bool up = (c & 64u) != 0u;
+    const uint32_t lo = up ? QV.z : QV.x, hi = up ? QV.w : QV.y;
+    return (c & 32u) ? hi : lo;
+}
+
+// Solid label (4 bits per voxel, same address order) of address c.
+__device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
+{
+    return (zlabels[c >> 3] >> ((c & 7u) * 4u)) & 0xFu;
+}
+
 // One tile (blockDim.x * K consecutive walkers from tile * blockDim.x * K)
 // walked for a.echoes x a.steps_per_echo steps on the unfolded image
 // a.zwords / a.zl, with the direction table already in LDS.  EMODE 0 = hit
-// counting (map), 1 = CPMG with per-walker or uniform delta.  Echo partials
-// go to part[row * part_stride] (nullable).  All threads of the block call it.
+// counting (map), 1 = CPMG with per-walker or uniform delta, 2 = per-label
+// delta (ldelta: the label deltas in LDS; labels from a.zlabels; per-label
+// hit counts in a.label_hits).  Echo partials go to part[row * part_stride]
+// (nullable).  All threads of the block call it.
 template <int EMODE, int K, bool WRAPFIX, bool RCACHE>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
-                                           double (*lds)[kMaxWaves], const uint4 *tab, const uint2 *fix)
+                                           double (*lds)[kMaxWaves], const uint4 *tab, const uint2 *fix,
+                                           const double *ldelta = nullptr)
 {
+    constexpr int NL = EMODE == 2 ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
     const uint64_t n = a.n;
     const uint32_t echoes = a.echoes, spe = a.steps_per_echo;
+    const int n_labels = a.n_labels;
 
     uint32_t A[K], C[K], Hc[K];
     uint32_t QB[K];   // RCACHE: 128-voxel chunk (address >> 7) held in QV
@@ -68,6 +127,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     int32_t P[K], DP[K], X[K], Y[K], Z[K];
     uint64_t S[K];
     double E[K], DL[K];
+    uint32_t LH[K][NL];
     bool act[K];
     static_for<K>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
@@ -85,6 +145,11 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         P[k] = 0;
         QB[k] = 0xFFFFFFFFu;   // no chunk: addresses >> 7 stay below 2^25
         QV[k] = make_uint4(0u, 0u, 0u, 0u);
+        static_for<NL>([&](auto lc) {
+            constexpr int l = decltype(lc)::value;
+            LH[k][l] = (EMODE == 2 && a.label_hits && act[k] && l < n_labels) ? a.label_hits[(uint64_t)l * n + j]
+                                                                               : 0u;
+        });
     });
 
     auto echo_value = [&]() {
@@ -108,25 +173,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     auto step = [&]() {
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            S[k] = xorshift64(S[k]);
-            const uint32_t lo = (uint32_t)S[k], hi = (uint32_t)(S[k] >> 32);
-            // byte sum sb <= 2040 == state (mod 3); q = sb / 3 by a 24-bit multiply;
-            // table entry 2 * (sb - 3q) + (state & 1) at 16 bytes per entry
-            const uint32_t sb = __builtin_amdgcn_sad_u8(hi, 0u, __builtin_amdgcn_sad_u8(lo, 0u, 0u));
-            const uint32_t q = __umul24(sb, 2731u) >> 13;
-            const uint32_t off = (uint32_t)__mul24((int)q, -96) + ((sb << 5) | ((lo & 1u) << 4));
-            const uint4 t = *reinterpret_cast<const uint4 *>(tab_bytes + off);
-            const uint32_t m = t.x;
-            uint32_t c = (A[k] & ~m) | (((A[k] | ~m) + t.y) & m);
-            if (WRAPFIX) {
-                const uint2 f = *reinterpret_cast<const uint2 *>(fix_bytes + (off >> 1));
-                c = (c & m) == f.x ? ((c & ~m) | f.y) : c;
-            }
-            C[k] = c;
-            DP[k] = (int32_t)t.z;
-            // keep the displacement in this one table read (the compiler would
-            // otherwise re-read it inside the move branch)
-            asm volatile("" : "+v"(DP[k]));
+            C[k] = zpropose<WRAPFIX>(S[k], A[k], tab_bytes, fix_bytes, DP[k]);
         });
         uint32_t word[K];
         if (RCACHE) {
@@ -134,18 +181,11 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             // lookup; only candidates outside it read memory
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                const uint32_t cid = C[k] >> 7;
-                if (cid != QB[k]) {
-                    // c < 2^bits and the volume holds at least 4 words: in bounds
-                    QV[k] = *reinterpret_cast<const uint4 *>(words + ((C[k] >> 3) & ~15u));
-                    QB[k] = cid;
-                }
+                zchunk_fill(words, C[k], QB[k], QV[k]);
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                const bool up = (C[k] & 64u) != 0u;
-                const uint32_t lo = up ? QV[k].z : QV[k].x, hi = up ? QV[k].w : QV[k].y;
-                word[k] = (C[k] & 32u) ? hi : lo;
+                word[k] = zchunk_word(C[k], QV[k]);
             });
         } else {
             static_for<K>([&](auto kc) {
@@ -162,6 +202,14 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             Hc[k] += wall;
             if (EMODE == 1 && wall)
                 E[k] *= DL[k];
+            if (EMODE == 2 && wall) {
+                const uint32_t lab = zlabel(a.zlabels, C[k]);
+                E[k] *= ldelta[lab];
+                static_for<NL>([&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    LH[k][l] += lab == (uint32_t)l ? 1u : 0u;
+                });
+            }
         });
     };
 
@@ -198,6 +246,12 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 a.energy[id] = E[k];
             if (a.hits)
                 a.hits[id] = Hc[k];
+            if (EMODE == 2 && a.label_hits)
+                static_for<NL>([&](auto lc) {
+                    constexpr int l = decltype(lc)::value;
+                    if (l < n_labels)
+                        a.label_hits[(uint64_t)l * n + id] = LH[k][l];
+                });
         }
     });
 }

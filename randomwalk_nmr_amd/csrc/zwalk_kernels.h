@@ -22,11 +22,20 @@ inline uint64_t zimage_words(const ZLayout &zl) { return zl.bits > 7 ? (1ull << 
 hipError_t launch_zimage_build(const uint64_t *blocks, const BrickLayout &L, int W, int H, int D, int mirror,
                                const ZLayout &zl, uint32_t *out, hipStream_t st);
 
+// Words of the unfolded 4-bit label volume (8 voxels per word, at least one).
+inline uint64_t zlabel_words(const ZLayout &zl) { return zl.bits > 3 ? (1ull << (zl.bits - 3)) : 1ull; }
+
+// Build the unfolded label volume from the device labels (64 bytes per block
+// of `blocks`' brick layout L); 0 outside the periods.
+hipError_t launch_zlabels_build(const uint8_t *labels_blocked, const BrickLayout &L, int W, int H, int D, int mirror,
+                                const ZLayout &zl, uint32_t *out, hipStream_t st);
+
 // keys[i] = unfolded address of walker i >> 5 (its 32-bit word), iota[i] = i.
 hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n, const ZLayout &zl,
                         uint32_t *keys, uint32_t *iota, hipStream_t st);
 
-// CPMG (emode 1) or map (emode 0) walk on a.zwords / a.zl; a.shift must be 0.
+// CPMG (emode 1), map (emode 0) or per-label CPMG (emode 2, needs a.zlabels
+// and a.label_delta) walk on a.zwords / a.zl; a.shift must be 0.
 hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hipStream_t st);
 
 }  // namespace rwnmr

@@ -100,3 +100,29 @@ def test_autograd_function_gradients(ctx):
             assert abs(fd - gr[l]) <= 1e-4 * abs(gr).max()
         fd_t = ((m.decay(rho, t2b + 0.5) * w).sum() - (m.decay(rho, t2b - 0.5) * w).sum()) / 1.0
         assert abs(fd_t - gt) <= 1e-5 * abs(gt)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "mirror"])
+def test_backward_unfolded_equals_folded_replay(ctx, bc):
+    """The replay on the unfolded image (non-power-of-two periods) gives the
+    same label gradient as the folded replay."""
+    print(f"[test] backward unfolded vs folded bc={bc}", flush=True)
+    import torch
+    solid, _ = R.sphere_pack(30, 21, 17, 3, 5, 0.3, 8)
+    labels = R.voronoi_labels(solid, 10, 8, 9)
+    seeds, x, y, z = R.seed_and_place(solid, 10, 2500)
+    grads, sums = [], []
+    for mode in (2, 0):
+        ctx.set_unfolded(mode)
+        try:
+            img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
+            m = CpmgModel(ctx, img, x, y, z, seeds, echoes=50, steps_per_echo=3, voxel=VOXEL, D0=D0, bc=bc,
+                          ckpt_every=12)
+            delta = m.delta_of_rho(np.linspace(5, 40, 8))
+            sums.append(m.forward_sums(delta).cpu().numpy())
+            c = torch.tensor(np.random.default_rng(3).standard_normal(m.echoes + 1))
+            grads.append(np.asarray(m.backward_delta(c)))
+        finally:
+            ctx.set_unfolded(2)
+    assert np.array_equal(sums[0], sums[1])
+    assert np.all(np.abs(grads[0] - grads[1]) <= 1e-12 * np.abs(grads[1]).max())

@@ -174,6 +174,31 @@ def test_cpmg_labels_match_oracle(ctx):
     assert np.array_equal(g["label_hits"], o["label_hits"].T)
 
 
+@pytest.mark.parametrize("bc", ["periodic", "mirror"])
+@pytest.mark.parametrize("dims", [(28, 28, 28), (32, 16, 8)])
+@pytest.mark.parametrize("unfolded", [2, 1, 0])
+def test_cpmg_labels_unfolded_and_folded_match_oracle(ctx, bc, dims, unfolded):
+    """Per-label CPMG on the unfolded image (4-bit label volume) and on the
+    folded kernel: walkers, energies and per-label hit counts bit-exact."""
+    print(f"[test] labels unfolded={unfolded} bc={bc} dims={dims}", flush=True)
+    W, H, D = dims
+    solid, seeds, x, y, z, _ = _case(W, H, D, 3.5, 0.3, 1100, 13)
+    labels = R.voronoi_labels(solid, 16, 8, 7)
+    ld = np.linspace(0.97, 0.999, 8)
+    ctx.set_unfolded(unfolded)
+    try:
+        img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
+        g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=21, steps_per_echo=5, bc=bc, label_delta=ld,
+                       label_hits=True)
+    finally:
+        ctx.set_unfolded(2)
+    o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=21, steps_per_echo=5, bc=bc,
+               labels=labels, label_delta=ld, label_hits=True)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+    assert np.array_equal(g["label_hits"], o["label_hits"].T)
+
+
 @pytest.mark.parametrize("bc", ["noflux", "mirror"])
 def test_pfgse_matches_oracle(ctx, bc):
     print(f"[test] pfgse bc={bc}", flush=True)
