@@ -204,7 +204,7 @@ def main():
                                                                     "walkers_per_lane": k}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "walk_kernel<mirror, per-walker delta>",
+                     "kernel": "zwalk_kernel<CPMG, chunk cache> on the unfolded mirror image",
                      "avg_kernel_ms": avg_kernel_s * 1e3, "launches": launches,
                      "algorithmic_bytes_per_launch": steps_per_launch * BYTES_PER_STEP},
         "cpu_baseline": cpu,
