@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -447,6 +448,26 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
 hipError_t launch_any_walk(const WalkArgs &a, int bc, int emode, const LaunchShape &ls, hipStream_t st)
 {
     return a.zwords ? rwnmr::launch_zwalk(a, emode, ls, st) : rwnmr::launch_walk(a, bc, emode, ls, st);
+}
+
+// True when every k row (n_rows rows of n_k vectors; kx, ky, kz blocks of nkv
+// doubles) is k_0 + j (k_1 - k_0) to within a few ulps of its magnitude.
+bool k_rows_arithmetic(const double *k, uint64_t nkv, uint32_t n_k, uint32_t n_rows)
+{
+    if (n_k < 2)
+        return false;
+    for (int comp = 0; comp < 3; ++comp)
+        for (uint32_t r = 0; r < n_rows; ++r) {
+            const double *row = k + (uint64_t)comp * nkv + (uint64_t)r * n_k;
+            const double d = row[1] - row[0];
+            double mag = 0.0;
+            for (uint32_t j = 0; j < n_k; ++j)
+                mag = std::max(mag, std::fabs(row[j]));
+            for (uint32_t j = 0; j < n_k; ++j)
+                if (std::fabs(row[j] - (row[0] + (double)j * d)) > 4e-15 * mag)
+                    return false;
+        }
+    return true;
 }
 
 // Upper bound on the partial-sum scratch of one walk launch; longer CPMG runs
@@ -1188,7 +1209,20 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
     p.partials = c->partials.as<double>();
     p.partial_stride = blocks;
     p.msd_rows = msd ? mrows : 0;
+    {
+        // k rows that are arithmetic progressions (to rounding) take the
+        // rotation form of the phase sum
+        const uint64_t nkv = (uint64_t)n_k * (k_per_delta ? n_delta : 1u);
+        std::vector<double> hk(3 * nkv);
+        const hipMemcpyKind kind = w->on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+        RW_HIP(c, hipMemcpyAsync(hk.data(), kx, 8 * nkv, kind, c->stream));
+        RW_HIP(c, hipMemcpyAsync(hk.data() + nkv, ky, 8 * nkv, kind, c->stream));
+        RW_HIP(c, hipMemcpyAsync(hk.data() + 2 * nkv, kz, 8 * nkv, kind, c->stream));
+        RW_HIP(c, hipStreamSynchronize(c->stream));
+        p.k_rot = k_rows_arithmetic(hk.data(), nkv, n_k, k_per_delta ? n_delta : 1u) ? 1u : 0u;
+    }
     WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
+    attach_zimage(c, img, bc, 1, a);
     double *Mdev = M, *msd_dev = msd;
     if (!w->on_device) {
         RW_HIP(c, c->sums.reserve(8ull * rows + 8));

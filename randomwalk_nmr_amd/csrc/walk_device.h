@@ -213,6 +213,92 @@ __device__ __forceinline__ void stage_echo(double (*lds)[kMaxWaves], uint32_t ec
     }
 }
 
+// PFGSE measurement at diffusion time i for one walker with displacement
+// (dX, dY, dZ) voxels and energies e[0..nsets): block sums of
+// e[q] cos((k . dr) voxel) for every k of the time's k row and every set q,
+// staged in LDS (lds rows q * kPfgseKChunk + k) and written as partials
+// [(q * n_delta + i) * n_k + k][blockIdx.x].  cos is evaluated once per k and
+// shared by the sets; with ROT (k row k_0 + j d) the phases advance by a
+// complex rotation, e^{i(ph0 + j phd)} = e^{i ph0} (e^{i phd})^j, 4 FMAs per k
+// instead of a cos (rounding grows ~2 ulp per k, far inside the 1e-11 M(0)
+// tolerance).  All threads of the block call it.
+template <bool ROT>
+__device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, double dX, double dY, double dZ,
+                                              const double *e, int nsets, bool active,
+                                              double (*lds)[kPfgseWaves])
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const uint64_t krow = (uint64_t)i * p.k_delta_stride;
+    double c = 0.0, s = 0.0, c1 = 1.0, s1 = 0.0;
+    if (ROT) {
+        const double ph0 = (p.kx[krow] * dX + p.ky[krow] * dY + p.kz[krow] * dZ) * p.voxel;
+        const double phd = ((p.kx[krow + 1] - p.kx[krow]) * dX + (p.ky[krow + 1] - p.ky[krow]) * dY +
+                            (p.kz[krow + 1] - p.kz[krow]) * dZ) * p.voxel;
+        sincos(ph0, &s, &c);
+        sincos(phd, &s1, &c1);
+    }
+    for (uint32_t k0 = 0; k0 < p.n_k; k0 += kPfgseKChunk) {
+        const uint32_t kc = p.n_k - k0 < (uint32_t)kPfgseKChunk ? p.n_k - k0 : (uint32_t)kPfgseKChunk;
+        for (uint32_t kk = 0; kk < kc; ++kk) {
+            double cs;
+            if (ROT) {
+                cs = c;
+                const double cn = c * c1 - s * s1;
+                s = s * c1 + c * s1;
+                c = cn;
+            } else {
+                const uint64_t kk_i = krow + k0 + kk;
+                cs = cos((p.kx[kk_i] * dX + p.ky[kk_i] * dY + p.kz[kk_i] * dZ) * p.voxel);
+            }
+            static_for<kMaxSets>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if (q < nsets) {
+                    const double v = wave_sum(active ? cs * e[q] : 0.0);
+                    if (lane == 0)
+                        lds[q * kPfgseKChunk + kk][wave] = v;
+                }
+            });
+        }
+        __syncthreads();
+        for (uint32_t r = threadIdx.x; r < (uint32_t)nsets * kc; r += blockDim.x) {
+            const uint32_t q = r / kc, kk = r % kc;
+            double sum = 0.0;
+            for (int w = 0; w < kPfgseWaves; ++w)
+                sum += lds[q * kPfgseKChunk + kk][w];
+            const uint64_t row = ((uint64_t)q * p.n_delta + i) * p.n_k + k0 + kk;
+            p.partials[row * p.partial_stride + blockIdx.x] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// Sums of squared displacements per axis at diffusion time i (voxel^2;
+// integer-valued, so the FP64 sums are exact in any order).
+__device__ __forceinline__ void pfgse_msd(const PfgseArgs &p, uint32_t i, double dX, double dY, double dZ,
+                                          bool active, double (*lds)[kPfgseWaves])
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const double sx = wave_sum(active ? dX * dX : 0.0);
+    const double sy = wave_sum(active ? dY * dY : 0.0);
+    const double sz = wave_sum(active ? dZ * dZ : 0.0);
+    if (lane == 0) {
+        lds[0][wave] = sx;
+        lds[1][wave] = sy;
+        lds[2][wave] = sz;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double sum = 0.0;
+        for (int w = 0; w < kPfgseWaves; ++w)
+            sum += lds[threadIdx.x][w];
+        const uint64_t row = p.msd_rows + (uint64_t)i * 3 + threadIdx.x;
+        p.partials[row * p.partial_stride + blockIdx.x] = sum;
+    }
+    __syncthreads();
+}
+
 // One tile of walkers (blockDim.x * K consecutive walkers starting at
 // tile * blockDim.x * K) walked for a.echoes x a.steps_per_echo steps.
 // EMODE: 0 = hit counting only (map), 1 = per-walker delta, 2 = per-label

@@ -54,6 +54,7 @@ struct LaunchShape {
 __host__ __device__ constexpr uint32_t max_threads_for_k(int k) { return k >= 4 ? 256u : (k == 2 ? 512u : 1024u); }
 
 static constexpr uint32_t kPfgseThreads = 256;
+static constexpr int kPfgseWaves = (int)kPfgseThreads / 64;
 static constexpr int kMaxSets = 8;       // energy sets evaluated on one PFGSE trajectory
 static constexpr int kPfgseKChunk = 64;  // k values staged in LDS per block sync
 
@@ -73,6 +74,8 @@ struct PfgseArgs {
     uint32_t partial_stride;         // = grid blocks
     uint64_t msd_rows;               // 0: no displacement sums; else first row of the
                                      // [n_delta][3] squared-displacement sums
+    uint32_t k_rot;                  // 1: every k row is an arithmetic progression (n_k >= 2):
+                                     // cos(k_j . dr) by complex rotation instead of cos()
 };
 
 uint32_t walk_grid_blocks(uint64_t n, const LaunchShape &ls, int shift);

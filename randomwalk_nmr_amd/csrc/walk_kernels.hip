@@ -19,6 +19,7 @@
 
 #include "walk_common.h"
 #include "walk_device.h"
+#include "zwalk_kernels.h"
 
 namespace rwnmr {
 
@@ -120,14 +121,13 @@ __global__ __launch_bounds__(1024) void walk_kernel_shift(const WalkArgs a)
 // PFGSE: the walk between diffusion times counts hits only (relaxation never
 // changes a trajectory), then every energy set is updated once with
 // pow(delta, hits of the increment) and the block sums of E cos(phase) for all
-// k are staged through LDS.  Partials: [(set * n_delta + i) * n_k + k][block].
-template <int BC>
+// k are staged through LDS (pfgse_measure).  Partials:
+// [(set * n_delta + i) * n_k + k][block].  Used for noflux; periodic and
+// mirror run zpfgse_kernel on the unfolded image.
+template <int BC, bool ROT>
 __global__ __launch_bounds__(kPfgseThreads) void pfgse_kernel(const WalkArgs a, const PfgseArgs p)
 {
-    __shared__ double lds[kPfgseKChunk][kMaxWaves];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int nw = blockDim.x / kWave;
+    __shared__ double lds[kMaxSets * kPfgseKChunk][kPfgseWaves];
     const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const Geom g = make_geom(a);
     Walker w;
@@ -176,53 +176,9 @@ __global__ __launch_bounds__(kPfgseThreads) void pfgse_kernel(const WalkArgs a, 
         const double dX = (double)(w.x - x0);
         const double dY = (double)(w.y - y0);
         const double dZ = (double)(w.z - z0);
-        for (uint32_t k0 = 0; k0 < p.n_k; k0 += kPfgseKChunk) {
-            const uint32_t kc = p.n_k - k0 < (uint32_t)kPfgseKChunk ? p.n_k - k0 : (uint32_t)kPfgseKChunk;
-            static_for<kMaxSets>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                if (q >= nsets)
-                    return;
-                for (uint32_t kk = 0; kk < kc; ++kk) {
-                    const uint32_t k = k0 + kk;
-                    const uint64_t kk_i = (uint64_t)i * p.k_delta_stride + k;
-                    const double ph = (p.kx[kk_i] * dX + p.ky[kk_i] * dY + p.kz[kk_i] * dZ) * p.voxel;
-                    double v = w.active ? cos(ph) * e[q] : 0.0;
-                    v = wave_sum(v);
-                    if (lane == 0)
-                        lds[kk][wave] = v;
-                }
-                __syncthreads();
-                if (threadIdx.x < kc) {
-                    double s = 0.0;
-                    for (int qq = 0; qq < nw; ++qq)
-                        s += lds[threadIdx.x][qq];
-                    const uint64_t row = ((uint64_t)q * p.n_delta + i) * p.n_k + k0 + threadIdx.x;
-                    p.partials[row * p.partial_stride + blockIdx.x] = s;
-                }
-                __syncthreads();
-            });
-        }
-        if (p.msd_rows) {
-            // squared displacements per axis (voxel^2; integer-valued, so the
-            // FP64 sums are exact for any order)
-            const double sx = wave_sum(w.active ? dX * dX : 0.0);
-            const double sy = wave_sum(w.active ? dY * dY : 0.0);
-            const double sz = wave_sum(w.active ? dZ * dZ : 0.0);
-            if (lane == 0) {
-                lds[0][wave] = sx;
-                lds[1][wave] = sy;
-                lds[2][wave] = sz;
-            }
-            __syncthreads();
-            if (threadIdx.x < 3) {
-                double s = 0.0;
-                for (int qq = 0; qq < nw; ++qq)
-                    s += lds[threadIdx.x][qq];
-                const uint64_t row = p.msd_rows + (uint64_t)i * 3 + threadIdx.x;
-                p.partials[row * p.partial_stride + blockIdx.x] = s;
-            }
-            __syncthreads();
-        }
+        pfgse_measure<ROT>(p, i, dX, dY, dZ, e, nsets, w.active, lds);
+        if (p.msd_rows)
+            pfgse_msd(p, i, dX, dY, dZ, w.active, lds);
     }
     if (w.active) {
         a.x[id] = w.x;
@@ -240,14 +196,22 @@ hipError_t launch_pfgse(const WalkArgs &a, const PfgseArgs &p, int bc, uint32_t 
 {
     if (a.n == 0)
         return hipSuccess;
+    if (threads != kPfgseThreads)
+        return hipErrorInvalidValue;
+    if (a.zwords)
+        return launch_zpfgse(a, p, st);
     const uint32_t blocks = (uint32_t)((a.n + threads - 1) / threads);
-    switch (bc) {
-    case kNoflux: hipLaunchKernelGGL(pfgse_kernel<kNoflux>, dim3(blocks), dim3(threads), 0, st, a, p); break;
-    case kPeriodic: hipLaunchKernelGGL(pfgse_kernel<kPeriodic>, dim3(blocks), dim3(threads), 0, st, a, p); break;
-    case kMirror: hipLaunchKernelGGL(pfgse_kernel<kMirror>, dim3(blocks), dim3(threads), 0, st, a, p); break;
-    default: return hipErrorInvalidValue;
+#define RW_PF(B)                                                                                             \
+    if (bc == B) {                                                                                           \
+        if (p.k_rot)                                                                                         \
+            hipLaunchKernelGGL((pfgse_kernel<B, true>), dim3(blocks), dim3(threads), 0, st, a, p);           \
+        else                                                                                                 \
+            hipLaunchKernelGGL((pfgse_kernel<B, false>), dim3(blocks), dim3(threads), 0, st, a, p);          \
+        return hipGetLastError();                                                                            \
     }
-    return hipGetLastError();
+    RW_PF(kNoflux) RW_PF(kPeriodic) RW_PF(kMirror)
+#undef RW_PF
+    return hipErrorInvalidValue;
 }
 
 // out[r] = sum_c partials[r * cols + c], fixed summation order per row.

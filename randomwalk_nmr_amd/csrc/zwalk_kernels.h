@@ -34,6 +34,9 @@ hipError_t launch_zlabels_build(const uint8_t *labels_blocked, const BrickLayout
 hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n, const ZLayout &zl,
                         uint32_t *keys, uint32_t *iota, hipStream_t st);
 
+// PFGSE walk + measurement on the unfolded image (a.zwords / a.zl).
+hipError_t launch_zpfgse(const WalkArgs &a, const PfgseArgs &p, hipStream_t st);
+
 // CPMG (emode 1), map (emode 0) or per-label CPMG (emode 2, needs a.zlabels
 // and a.label_delta) walk on a.zwords / a.zl; a.shift must be 0.
 hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hipStream_t st);

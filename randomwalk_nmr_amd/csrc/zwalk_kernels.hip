@@ -29,6 +29,114 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
                                           tab, fix, ldelta);
 }
 
+// PFGSE on the unfolded image: the walk between diffusion times counts hits
+// (register chunk cache, packed displacement flushed every kZFlush steps and
+// at every diffusion time), then pow(delta, hits) per energy set and the
+// shared measurement (pfgse_measure).  One walker per thread.
+template <bool WRAPFIX, bool ROT>
+__global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a, const PfgseArgs p)
+{
+    __shared__ double lds[kMaxSets * kPfgseKChunk][kPfgseWaves];
+    __shared__ uint4 tab[6];
+    __shared__ uint2 fix[6];
+    zbuild_table(a.zl, tab, fix);
+    __syncthreads();
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = id < a.n;
+    const uint64_t j = active ? id : 0;
+    int32_t X = active ? a.x[j] : 0, Y = active ? a.y[j] : 0, Z = active ? a.z[j] : 0;
+    uint64_t s = active ? a.seed[j] : 1ull;
+    uint32_t hits = (active && a.hits) ? a.hits[j] : 0u;
+    const int32_t x0 = active ? p.x0[j] : 0, y0 = active ? p.y0[j] : 0, z0 = active ? p.z0[j] : 0;
+    const double own_delta = (active && a.penalty) ? a.penalty[j] : a.delta_uniform;
+    const int nsets = p.n_sets > 0 ? (int)p.n_sets : 1;
+    double e[kMaxSets];
+    static_for<kMaxSets>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        e[q] = !active ? 0.0 : ((q == 0 && a.energy) ? a.energy[j] : 1.0);
+    });
+    uint32_t A = zaddress(X, Y, Z, a.zl);
+    uint32_t QB = 0xFFFFFFFFu;
+    uint4 QV = make_uint4(0u, 0u, 0u, 0u);
+    int32_t P = 0;
+    uint32_t since = 0;
+    const char *words = reinterpret_cast<const char *>(a.zwords);
+    const char *tab_bytes = reinterpret_cast<const char *>(tab);
+    const char *fix_bytes = reinterpret_cast<const char *>(fix);
+    auto flush = [&]() {
+        int32_t dx, dy, dz;
+        zunpack(P, dx, dy, dz);
+        X += dx;
+        Y += dy;
+        Z += dz;
+        P = 0;
+        since = 0;
+    };
+    for (uint32_t i = 0; i < p.n_delta; ++i) {
+        uint32_t h = 0;
+        uint32_t left = p.steps_inc[i];
+        while (left) {
+            const uint32_t ns = left < kZFlush - since ? left : kZFlush - since;
+            for (uint32_t st = 0; st < ns; ++st) {
+                int32_t dp;
+                const uint32_t c = zpropose<WRAPFIX>(s, A, tab_bytes, fix_bytes, dp);
+                zchunk_fill(words, c, QB, QV);
+                const uint32_t wall = __builtin_amdgcn_ubfe(zchunk_word(c, QV), c, 1u);
+                A = wall ? A : c;
+                P += wall ? 0 : dp;
+                h += wall;
+            }
+            since += ns;
+            left -= ns;
+            if (since == kZFlush)
+                flush();
+        }
+        flush();
+        hits += h;
+        static_for<kMaxSets>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (q < nsets)
+                e[q] *= pow(p.n_sets > 0 ? p.set_delta[q] : own_delta, (double)h);
+        });
+        const double dX = (double)(X - x0), dY = (double)(Y - y0), dZ = (double)(Z - z0);
+        pfgse_measure<ROT>(p, i, dX, dY, dZ, e, nsets, active, lds);
+        if (p.msd_rows)
+            pfgse_msd(p, i, dX, dY, dZ, active, lds);
+    }
+    if (active) {
+        a.x[id] = X;
+        a.y[id] = Y;
+        a.z[id] = Z;
+        a.seed[id] = s;
+        if (a.energy)
+            a.energy[id] = e[0];
+        if (a.hits)
+            a.hits[id] = hits;
+    }
+}
+
+hipError_t launch_zpfgse(const WalkArgs &a, const PfgseArgs &p, hipStream_t st)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    if (a.zwords == nullptr || a.shift != 0)
+        return hipErrorInvalidValue;
+    const uint32_t blocks = (uint32_t)((a.n + kPfgseThreads - 1) / kPfgseThreads);
+    const dim3 g(blocks), t(kPfgseThreads);
+    if (a.zl.wrapfix) {
+        if (p.k_rot)
+            hipLaunchKernelGGL((zpfgse_kernel<true, true>), g, t, 0, st, a, p);
+        else
+            hipLaunchKernelGGL((zpfgse_kernel<true, false>), g, t, 0, st, a, p);
+    } else {
+        if (p.k_rot)
+            hipLaunchKernelGGL((zpfgse_kernel<false, true>), g, t, 0, st, a, p);
+        else
+            hipLaunchKernelGGL((zpfgse_kernel<false, false>), g, t, 0, st, a, p);
+    }
+    return hipGetLastError();
+}
+
 // One thread per output word of the unfolded image.
 __global__ __launch_bounds__(256) void zimage_build_kernel(const uint64_t *blocks, BrickLayout L, int W, int H, int D,
                                                           int mirror, ZLayout zl, uint32_t *out, uint64_t nwords)

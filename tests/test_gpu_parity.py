@@ -239,6 +239,42 @@ def test_pfgse_k_set_per_diffusion_time(ctx):
     assert np.array_equal(g["msd"], o["msd"])
 
 
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
+@pytest.mark.parametrize("kind", ["arithmetic", "random"])
+@pytest.mark.parametrize("unfolded", [2, 0])
+def test_pfgse_phase_paths_and_kernels_match_oracle(ctx, bc, kind, unfolded):
+    """Arithmetic k rows (phases by complex rotation) and irregular k rows
+    (cos per k), on the unfolded-image and folded PFGSE kernels, with a
+    non-power-of-two image and two energy sets: M within 1e-11 M(0), walkers
+    and msd exact."""
+    print(f"[test] pfgse phase={kind} bc={bc} unfolded={unfolded}", flush=True)
+    solid, seeds, x, y, z, pen = _case(27, 19, 33, 4, 0.35, 1300, 41)
+    steps_inc = np.array([5, 7, 30, 300, 600], np.int64)
+    rng = np.random.default_rng(5)
+    if kind == "arithmetic":
+        kx = np.stack([np.linspace(0.0, 2.0 / np.sqrt(c), 20) for c in np.cumsum(steps_inc)])
+        ky = 0.3 * kx
+        kz = np.stack([np.linspace(0.1, 0.4, 20)] * len(steps_inc))
+    else:
+        kx = rng.uniform(-1, 1, (len(steps_inc), 20))
+        ky = rng.uniform(-1, 1, (len(steps_inc), 20))
+        kz = rng.uniform(-1, 1, (len(steps_inc), 20))
+    sets = np.array([1.0, 0.985])
+    ctx.set_unfolded(unfolded)
+    try:
+        img = R.Image.from_solid(ctx, solid)
+        g = R.pfgse_run(ctx, img, x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=ky, kz=kz, voxel=0.8, bc=bc,
+                        set_delta=sets)
+    finally:
+        ctx.set_unfolded(2)
+    o = O.pfgse(O.bitblock_pack(solid), (27, 19, 33), x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=ky, kz=kz,
+                voxel=0.8, bc=bc, set_delta=sets)
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.all(np.abs(g["M"] - o["M"]) <= 1e-11 * np.abs(o["M"][..., :1]))
+    assert np.array_equal(g["msd"], o["msd"])
+
+
 def test_pfgse_energy_sets(ctx):
     """Several uniform deltas on one trajectory equal separate runs."""
     print("[test] pfgse sets", flush=True)
