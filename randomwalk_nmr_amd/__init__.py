@@ -7,5 +7,6 @@ from ._lib import RwnmrError, load  # noqa: F401
 from .core import (BC, Context, Image, bc_code, bitblock_dims, cpmg_batch_run, cpmg_run, map_run,  # noqa: F401
                    pack_bitblocks, penalty, pfgse_run, seed_and_place, sphere_pack, time_framework,
                    time_step, voronoi_labels)
+from .images import UctImage, image_stats, read_png, write_image_info, write_walkers_csv  # noqa: F401
 
 __version__ = "0.1.0"

@@ -36,6 +36,13 @@ class Walkers(ct.Structure):
                 ("penalty", P), ("delta_uniform", D), ("n", U64), ("on_device", I32)]
 
 
+class UctConfig(ct.Structure):
+    """rwnmr_uct_config (include/rwnmr/rwnmr_io.h)."""
+    _fields_ = [("dir_path", ct.c_char * 1024), ("filename", ct.c_char * 256), ("first_idx", I32),
+                ("digits", I32), ("extension", ct.c_char * 32), ("slices", I32), ("resolution", D),
+                ("voxel_division", I32), ("img_files_list", ct.c_char * 1024)]
+
+
 class CpmgInstance(ct.Structure):
     """rwnmr_cpmg_instance (include/rwnmr/rwnmr.h)."""
     _fields_ = [("image", P), ("walkers", Walkers), ("echoes", U32), ("steps_per_echo", U32),
@@ -64,6 +71,18 @@ class PfgseResult(ct.Structure):
 
 # (name, restype, argtypes) of every exported symbol the package binds.
 SIGNATURES = {
+    # image ingestion and text outputs (include/rwnmr/rwnmr_io.h)
+    "rwnmr_uct_config_defaults": (None, [P]),
+    "rwnmr_uct_config_read": (I32, [ct.c_char_p, P]),
+    "rwnmr_uct_slice_path": (I32, [P, I32, ct.c_char_p, U32]),
+    "rwnmr_uct_dims": (I32, [P, ct.POINTER(I32), ct.POINTER(I32), ct.POINTER(I32)]),
+    "rwnmr_uct_load": (I32, [P, I32, I32, I32, P]),
+    "rwnmr_png_size": (I32, [ct.c_char_p, ct.POINTER(I32), ct.POINTER(I32)]),
+    "rwnmr_png_binarize": (I32, [ct.c_char_p, I32, I32, P]),
+    "rwnmr_image_stats": (I32, [P, I32, I32, I32, D, ct.POINTER(U64), ct.POINTER(D), ct.POINTER(U64),
+                                ct.POINTER(D)]),
+    "rwnmr_write_image_info": (I32, [ct.c_char_p, ct.c_char_p, I32, I32, I32, D, I32, D, D, D]),
+    "rwnmr_write_walkers_csv": (I32, [ct.c_char_p, U64, P, P, P, P, P, P, P, P, P, P]),
     "rwnmr_version": (ct.c_char_p, []),
     "rwnmr_ctx_create": (I32, [ct.POINTER(P), I32]),
     "rwnmr_ctx_destroy": (None, [P]),
