@@ -157,6 +157,23 @@ rwnmr_status rwnmr_map_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers
 rwnmr_status rwnmr_cpmg_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
                             uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums);
 
+/* Internal-field CPMG (NMR_cpmg::image_simulation_cuda with an internal
+ * field, kernels CPMG_walk_{noflux,periodic,mirror}_field,
+ * B/NMR_Simulation/NMR_cpmg_cuda.cpp:146-293, 409-567, 683-841, driver
+ * :1006-1011, 1080-1257): per echo two phase inversions of
+ * steps_per_echo / 2 steps; the phase gains gammatau * field[x + y W + z W H]
+ * at the walker's image voxel before and after every step, with
+ * gammatau = 0.5 * (1e6 * gyromagnetic_ratio) * (1e-3 * time_interval_ms);
+ * a wall hit multiplies p_alive by the walker's delta.  echo_sums[e] =
+ * sum_w p_alive cos(phase) after echo e (echoes + 1 values).  field holds
+ * W*H*D doubles (host or device, as the walkers).  p_alive / phase are
+ * in/out per walker (NULL: start at 1 / 0 and are not returned; on device
+ * give both or neither).  shift (voxel division) is not supported here. */
+rwnmr_status rwnmr_cpmg_field_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                  uint32_t steps_per_echo, int32_t bc, const double *field,
+                                  double gyromagnetic_ratio, double time_interval_ms, double *p_alive, double *phase,
+                                  double *echo_sums);
+
 /* CPMG with a relaxivity per solid label: a hit on a voxel of label l
  * multiplies by label_delta[l] (host or device like the walkers, n_labels of
  * the image).  label_hits (nullable, [n_labels][n] uint32, in/out) counts hits

@@ -79,6 +79,18 @@ rwnmr_status rwnmr_write_walkers_csv(const char *path, uint64_t n, const int32_t
                                      const uint32_t *collisions, const double *xi_rate, const double *energy,
                                      const uint64_t *seed);
 
+/* Internal field of a uniform gradient (InternalField::fillDataArray,
+ * B/NMR_Simulation/InternalField.cpp:95-134): value = coordinate along
+ * `direction` (0 x, 1 y, 2 z) * gradient * 1e-6 * resolution in pore voxels,
+ * 0 in solid voxels; field[x + y W + z W H]. */
+rwnmr_status rwnmr_internal_field_uniform(const uint8_t *solid, int32_t width, int32_t height, int32_t depth,
+                                          double resolution, double gradient, int32_t direction, double *field);
+/* Internal field from a raw file of W*H*D native doubles
+ * (InternalField::readDataFromFile, InternalField.cpp:167-200; the file size
+ * must match). */
+rwnmr_status rwnmr_internal_field_read(const char *path, int32_t width, int32_t height, int32_t depth,
+                                       double *field);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,9 @@ def lib():
         L.orc_walk.restype = _I
         L.orc_walk.argtypes = [_P, _I, _I, _I, _P, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P,
                                _P, _P, _I64, _I64, _P, _I]
+        L.orc_walk_field.restype = _I
+        L.orc_walk_field.argtypes = [_P, _I, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D,
+                                     _I64, _I64, _P, _I]
         L.orc_cpmg_reference_loop.restype = _D
         L.orc_cpmg_reference_loop.argtypes = [_P, _I, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P,
                                               _I64, _I64, _P, _P, _P, _P, _I]
@@ -161,6 +164,33 @@ def walk(blocks, dims, x, y, z, seed, *, echoes, steps_per_echo, bc="mirror", sh
     if rc:
         raise ValueError(f"orc_walk failed ({rc})")
     return dict(x=x, y=y, z=z, seed=seed, energy=energy, hits=hits, echo_sums=es, label_hits=lh)
+
+
+def walk_field(blocks, dims, x, y, z, seed, *, echoes, steps_per_echo, field, gammatau, bc="mirror", shift=0,
+               penalty=None, p_alive=None, phase=None, hits=None, use_digit_sum=False, threads=0):
+    """Internal-field CPMG (see orc_walk_field).  Returns a dict with x, y,
+    z, seed, p_alive, phase, hits and echo_sums (sums of p_alive cos(phase))."""
+    W, H, D = dims
+    n = len(x)
+    x = _c(x, _np.int32).copy()
+    y = _c(y, _np.int32).copy()
+    z = _c(z, _np.int32).copy()
+    seed = _c(seed, _np.uint64).copy()
+    pa = _np.ones(n) if p_alive is None else _c(p_alive, _np.float64).copy()
+    ph = _np.zeros(n) if phase is None else _c(phase, _np.float64).copy()
+    hits = _np.zeros(n, _np.uint32) if hits is None else _c(hits, _np.uint32).copy()
+    pen = _np.ones(n) if penalty is None else _c(_np.broadcast_to(penalty, (n,)), _np.float64)
+    fld = _c(field, _np.float64)
+    if fld.size != W * H * D:
+        raise ValueError("field must hold W*H*D values")
+    es = _np.zeros(echoes + 1)
+    blocks = _c(blocks, _np.uint64)
+    rc = lib().orc_walk_field(_ptr(blocks), W, H, D, BC_CODES.get(bc, bc), shift, int(use_digit_sum), n, _ptr(x),
+                              _ptr(y), _ptr(z), _ptr(seed), _ptr(pa), _ptr(ph), _ptr(hits), _ptr(pen), _ptr(fld),
+                              float(gammatau), echoes, steps_per_echo, _ptr(es), threads)
+    if rc:
+        raise ValueError(f"orc_walk_field failed ({rc})")
+    return dict(x=x, y=y, z=z, seed=seed, p_alive=pa, phase=ph, hits=hits, echo_sums=es)
 
 
 def cpmg_reference_loop(blocks, dims, x, y, z, seed, penalty, *, echoes, steps_per_echo,

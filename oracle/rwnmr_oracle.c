@@ -358,6 +358,126 @@ int orc_walk(const uint64_t *blocks, int W, int H, int D, const uint8_t *labels,
     return 0;
 }
 
+/* Image voxel of a walker position under the geometry's boundary (the
+ * fold the GPU kernels apply before a bitBlock or field lookup). */
+static inline void orc_image_voxel(const orc_geom *g, int x, int y, int z, int *ix, int *iy, int *iz)
+{
+    if (g->bc == ORC_BC_PERIODIC) {
+        *ix = orc_fold_periodic(x, g->shift, g->W);
+        *iy = orc_fold_periodic(y, g->shift, g->H);
+        *iz = orc_fold_periodic(z, g->shift, g->D);
+    } else if (g->bc == ORC_BC_MIRROR) {
+        *ix = orc_fold_mirror(x, g->shift, g->W);
+        *iy = orc_fold_mirror(y, g->shift, g->H);
+        *iz = orc_fold_mirror(z, g->shift, g->D);
+    } else {
+        *ix = x >> g->shift;
+        *iy = y >> g->shift;
+        *iz = z >> g->shift;
+    }
+}
+
+/*
+ * Internal-field CPMG (B/NMR_Simulation/NMR_cpmg_cuda.cpp:146-293 noflux,
+ * 409-567 periodic, 683-841 mirror).  Per echo two phase inversions
+ * (phase = -phase), each of steps_per_echo / 2 (integer division) steps;
+ * before and after every step the phase gains gammatau * field[voxel] with
+ * voxel = ix + iy W + iz W H of the walker's image position, written as a
+ * fused multiply-add (the CUDA build contracts phase + gammatau * field);
+ * a wall hit multiplies p_alive by the walker's penalty (no energy change
+ * otherwise).  Echo value: p_alive * cos(phase); echo 0 from the initial
+ * state.  p_alive / phase are in/out (NULL: start at 1 / 0).
+ */
+int orc_walk_field(const uint64_t *blocks, int W, int H, int D, int bc, int shift, int use_digit_sum, int64_t n,
+                   int32_t *x, int32_t *y, int32_t *z, uint64_t *seed, double *p_alive, double *phase,
+                   uint32_t *hits, const double *penalty, const double *field, double gammatau, int64_t echoes,
+                   int64_t steps_per_echo, double *echo_sums, int threads)
+{
+    if (n < 0 || echoes < 0 || steps_per_echo < 0 || W <= 0 || H <= 0 || D <= 0 || !field)
+        return 1;
+    orc_geom g;
+    g.blocks = blocks;
+    g.W = W;
+    g.H = H;
+    g.D = D;
+    g.bC = (W + 3) / 4;
+    g.bR = (H + 3) / 4;
+    g.labels = NULL;
+    g.bc = bc;
+    g.shift = shift;
+    const int64_t half = steps_per_echo / 2;
+    int nt = orc_threads(threads);
+    orc_acc *acc = NULL;
+    if (echo_sums) {
+        acc = (orc_acc *)calloc((size_t)nt * (size_t)(echoes + 1), sizeof(orc_acc));
+        if (!acc)
+            return 2;
+    }
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        orc_acc *my = acc ? acc + (size_t)tid * (size_t)(echoes + 1) : NULL;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int64_t w = 0; w < n; ++w) {
+            int px = x[w], py = y[w], pz = z[w];
+            uint64_t s = seed[w];
+            double pa = p_alive ? p_alive[w] : 1.0;
+            double ph = phase ? phase[w] : 0.0;
+            uint32_t h = hits ? hits[w] : 0u;
+            const double dw = penalty ? penalty[w] : 1.0;
+            if (my)
+                orc_acc_add(&my[0], pa * cos(ph));
+            for (int64_t ec = 1; ec <= echoes; ++ec) {
+                for (int inv = 0; inv < 2; ++inv) {
+                    ph = -ph;
+                    for (int64_t st = 0; st < half; ++st) {
+                        int ix, iy, iz;
+                        orc_image_voxel(&g, px, py, pz, &ix, &iy, &iz);
+                        ph = fma(gammatau, field[(size_t)ix + (size_t)iy * W + (size_t)iz * W * H], ph);
+                        if (orc_step(&g, &px, &py, &pz, &s, use_digit_sum) >= 0) {
+                            ++h;
+                            pa = pa * dw;
+                        }
+                        orc_image_voxel(&g, px, py, pz, &ix, &iy, &iz);
+                        ph = fma(gammatau, field[(size_t)ix + (size_t)iy * W + (size_t)iz * W * H], ph);
+                    }
+                }
+                if (my)
+                    orc_acc_add(&my[ec], pa * cos(ph));
+            }
+            x[w] = px;
+            y[w] = py;
+            z[w] = pz;
+            seed[w] = s;
+            if (p_alive)
+                p_alive[w] = pa;
+            if (phase)
+                phase[w] = ph;
+            if (hits)
+                hits[w] = h;
+        }
+    }
+    if (echo_sums) {
+        for (int64_t ec = 0; ec <= echoes; ++ec) {
+            orc_acc tot = {0.0L, 0.0L};
+            for (int t = 0; t < nt; ++t) {
+                orc_acc_add(&tot, acc[(size_t)t * (echoes + 1) + ec].s);
+                orc_acc_add(&tot, acc[(size_t)t * (echoes + 1) + ec].c);
+            }
+            echo_sums[ec] = (double)(tot.s + tot.c);
+        }
+        free(acc);
+    }
+    return 0;
+}
+
 /*
  * Reference CPU CPMG loop order, kept for timing only (the CPU baseline):
  * echo-major, walker AoS records, the digit-sum mod6, a sequential

@@ -4,9 +4,10 @@ relaxivity fit).  Compute runs in librwnmr_amd.so (HIP kernels for gfx950
 behind the C ABI in include/rwnmr/); this package is the Python host mirror.
 """
 from ._lib import RwnmrError, load  # noqa: F401
-from .core import (BC, Context, Image, bc_code, bitblock_dims, cpmg_batch_run, cpmg_run, map_run,  # noqa: F401
+from .core import (BC, Context, Image, bc_code, bitblock_dims, cpmg_batch_run, cpmg_field_run, cpmg_run, map_run,  # noqa: F401
                    pack_bitblocks, penalty, pfgse_run, seed_and_place, sphere_pack, time_framework,
                    time_step, voronoi_labels)
-from .images import UctImage, image_stats, read_png, write_image_info, write_walkers_csv  # noqa: F401
+from .images import (UctImage, image_stats, internal_field_uniform, read_internal_field, read_png,  # noqa: F401
+                     write_image_info, write_walkers_csv)
 
 __version__ = "0.1.0"

@@ -104,6 +104,9 @@ SIGNATURES = {
     "rwnmr_map_run": (I32, [P, P, ct.POINTER(Walkers), U64, I32, U32]),
     "rwnmr_cpmg_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
     "rwnmr_cpmg_run_labels": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P, P]),
+    "rwnmr_cpmg_field_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, P, D, D, P, P, P]),
+    "rwnmr_internal_field_uniform": (I32, [P, I32, I32, I32, D, D, I32, P]),
+    "rwnmr_internal_field_read": (I32, [ct.c_char_p, I32, I32, I32, P]),
     "rwnmr_pfgse_run": (I32, [P, P, ct.POINTER(Walkers), P, P, P, P, U32, P, P, P, U32, I32, D, I32, U32,
                               U32, P, P, P]),
     "rwnmr_cpmg_batch_run": (I32, [P, ct.POINTER(CpmgInstance), U32, U32]),

@@ -372,6 +372,44 @@ def pfgse_run(ctx: Context, img: Image, x, y, z, seed, *, steps_inc, kx, ky, kz,
     return dict(M=M, msd=msd, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
 
 
+def cpmg_field_run(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo, field, gyromagnetic_ratio,
+                   time_interval_ms, bc="mirror", penalty=None, delta=1.0, p_alive=None, phase=None, energy=None,
+                   hits=None, echo_sums=True, sync=True):
+    """Internal-field CPMG (rwnmr_cpmg_field_run): echo sums of
+    p_alive cos(phase).  Host inputs are copied and results returned (x, y,
+    z, seed, p_alive, phase, energy, hits, echo_sums); device inputs are
+    updated in place."""
+    dev = _is_dev(x)
+    n = len(x)
+    L = load()
+    if dev:
+        _torch_sync()
+        es = echo_sums if (echo_sums is not None and echo_sums is not True and echo_sums is not False) else None
+        w = _walker_struct(x, y, z, seed, energy, hits, penalty, delta, n, 1)
+        check(L.rwnmr_cpmg_field_run(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo, bc_code(bc),
+                                     _ptr(field), float(gyromagnetic_ratio), float(time_interval_ms), _ptr(p_alive),
+                                     _ptr(phase), _ptr(es)), ctx.handle)
+        if sync:
+            ctx.synchronize()
+        return dict(x=x, y=y, z=z, seed=seed, p_alive=p_alive, phase=phase, energy=energy, hits=hits, echo_sums=es)
+    x, y, z = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+    seed = _host(seed, np.uint64)
+    energy = np.ones(n) if energy is None else _host(energy, np.float64, n)
+    hits = np.zeros(n, np.uint32) if hits is None else _host(hits, np.uint32, n)
+    pen = None if penalty is None else _host(penalty, np.float64, n)
+    pa = np.ones(n) if p_alive is None else _host(p_alive, np.float64, n)
+    ph = np.zeros(n) if phase is None else _host(phase, np.float64, n)
+    fld = np.ascontiguousarray(field, np.float64)
+    if fld.size != img.dims[0] * img.dims[1] * img.dims[2]:
+        raise ValueError("field must hold W*H*D values")
+    es = np.zeros(echoes + 1) if echo_sums else None
+    w = _walker_struct(x, y, z, seed, energy, hits, pen, delta, n, 0)
+    check(L.rwnmr_cpmg_field_run(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo, bc_code(bc),
+                                 _ptr(fld), float(gyromagnetic_ratio), float(time_interval_ms), _ptr(pa), _ptr(ph),
+                                 _ptr(es)), ctx.handle)
+    return dict(x=x, y=y, z=z, seed=seed, p_alive=pa, phase=ph, energy=energy, hits=hits, echo_sums=es)
+
+
 def cpmg_batch_run(ctx: Context, instances, *, sequential=False, sync=True):
     """Many CPMG instances in one persistent launch (rwnmr_cpmg_batch_run).
 

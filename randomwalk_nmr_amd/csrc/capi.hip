@@ -79,6 +79,8 @@ struct rwnmr_ctx {
     DevBuf x, y, z, seed, energy, hits, penalty, x0, y0, z0;
     // per-call small arrays and reductions
     DevBuf partials, sums, label_delta, label_hits, steps_inc, kvec, set_delta;
+    // internal-field CPMG staging (host-pointer calls)
+    DevBuf field, p_alive, phase;
     // batched CPMG: descriptors, tile list, reduction rows, queue head
     DevBuf batch_inst, batch_tiles, batch_rows, batch_counter;
     DevBuf flag;  // uint32 error word
@@ -569,10 +571,19 @@ rwnmr_status walk_local(rwnmr_ctx *c, const rwnmr_image *img, const WalkArgs &a,
     return RWNMR_OK;
 }
 
-// Shared CPMG driver: emode 1 (per-walker delta) or 2 (per-label delta).
+// Internal-field inputs of a CPMG run (emode 3).
+struct FieldRun {
+    const double *field;   // W*H*D, host or device like the walkers
+    double gammatau;
+    double *p_alive, *phase;   // nullable: start at 1 / 0, not returned
+};
+
+// Shared CPMG driver: emode 1 (per-walker delta), 2 (per-label delta) or 3
+// (internal field, `fr`).
 rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
                          uint32_t steps_per_echo, int32_t bc, uint32_t shift, int emode,
-                         const double *label_delta, uint32_t *label_hits, double *echo_sums)
+                         const double *label_delta, uint32_t *label_hits, double *echo_sums,
+                         const FieldRun *fr = nullptr)
 {
     rwnmr_status s;
     if (!c)
@@ -596,6 +607,43 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         return s;
     WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
     a.n_labels = img->n_labels;
+    if (emode == 3) {
+        if (!fr || !fr->field)
+            return fail(c, RWNMR_ERR_INVALID, "field run: null field");
+        if (shift != 0)
+            return fail(c, RWNMR_ERR_INVALID, "field run: voxel division (shift > 0) is not supported");
+        const uint64_t vox = (uint64_t)img->W * img->H * img->D, n = w->n;
+        a.gammatau = fr->gammatau;
+        if (w->on_device) {
+            a.field = fr->field;
+        } else {
+            RW_HIP(c, c->field.reserve(8 * vox));
+            RW_HIP(c, hipMemcpyAsync(c->field.p, fr->field, 8 * vox, hipMemcpyHostToDevice, c->stream));
+            a.field = c->field.as<double>();
+        }
+        if (w->on_device && (fr->p_alive != nullptr) != (fr->phase != nullptr))
+            return fail(c, RWNMR_ERR_INVALID, "field run: give both p_alive and phase device arrays, or neither");
+        if (w->on_device && fr->p_alive && fr->phase) {
+            a.p_alive = fr->p_alive;
+            a.phase = fr->phase;
+        } else {
+            RW_HIP(c, c->p_alive.reserve(8 * n + 8));
+            RW_HIP(c, c->phase.reserve(8 * n + 8));
+            a.p_alive = c->p_alive.as<double>();
+            a.phase = c->phase.as<double>();
+            const hipMemcpyKind kind = w->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+            const uint32_t nb = (uint32_t)((n + 255) / 256);
+            if (fr->p_alive)
+                RW_HIP(c, hipMemcpyAsync(a.p_alive, fr->p_alive, 8 * n, kind, c->stream));
+            else if (n)
+                hipLaunchKernelGGL(fill_f64_kernel, dim3(nb), dim3(256), 0, c->stream, a.p_alive, 1.0, n);
+            if (fr->phase)
+                RW_HIP(c, hipMemcpyAsync(a.phase, fr->phase, 8 * n, kind, c->stream));
+            else if (n)
+                hipLaunchKernelGGL(fill_f64_kernel, dim3(nb), dim3(256), 0, c->stream, a.phase, 0.0, n);
+            RW_HIP(c, hipGetLastError());
+        }
+    }
     if (emode == 2) {
         if (w->on_device) {
             a.label_delta = label_delta;
@@ -633,7 +681,7 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
     }
     if (echo_sums)
         RW_HIP(c, c->partials.reserve(sizeof(double) * (size_t)blocks * (size_t)(chunk + 1) + 8));
-    const bool local = emode != 2 && shift == 0 && c->locality && w->n >= c->locality_min_walkers;
+    const bool local = emode < 2 && shift == 0 && c->locality && w->n >= c->locality_min_walkers;
     if (local) {
         a.steps_per_echo = steps_per_echo;
         if ((s = walk_local(c, img, a, bc, emode, echoes, steps_per_echo, echo_sums ? sums : nullptr, chunk)) !=
@@ -672,6 +720,10 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         if (emode == 2 && label_hits)
             RW_HIP(c, hipMemcpyAsync(label_hits, a.label_hits, sizeof(uint32_t) * (size_t)img->n_labels * w->n,
                                      hipMemcpyDeviceToHost, c->stream));
+        if (emode == 3 && fr->p_alive)
+            RW_HIP(c, hipMemcpyAsync(fr->p_alive, a.p_alive, 8 * w->n, hipMemcpyDeviceToHost, c->stream));
+        if (emode == 3 && fr->phase)
+            RW_HIP(c, hipMemcpyAsync(fr->phase, a.phase, 8 * w->n, hipMemcpyDeviceToHost, c->stream));
         if ((s = stage_out(c, w, v)) != RWNMR_OK)
             return s;
         return sync_and_check(c);
@@ -735,7 +787,7 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
     for (DevBuf *b : {&c->x, &c->y, &c->z, &c->seed, &c->energy, &c->hits, &c->penalty, &c->x0, &c->y0, &c->z0,
                       &c->partials, &c->sums, &c->label_delta, &c->label_hits, &c->steps_inc, &c->kvec,
                       &c->set_delta, &c->flag, &c->batch_inst, &c->batch_tiles, &c->batch_rows,
-                      &c->batch_counter, &c->lkeys, &c->lidx, &c->lsort_tmp})
+                      &c->batch_counter, &c->lkeys, &c->lidx, &c->lsort_tmp, &c->field, &c->p_alive, &c->phase})
         b->release();
     for (int i = 0; i < 8; ++i) {
         c->sortA[i].release();
@@ -1118,6 +1170,21 @@ rwnmr_status rwnmr_cpmg_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers 
                             uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums)
 {
     return cpmg_common(c, img, w, echoes, steps_per_echo, bc, shift, 1, nullptr, nullptr, echo_sums);
+}
+
+rwnmr_status rwnmr_cpmg_field_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                  uint32_t steps_per_echo, int32_t bc, const double *field,
+                                  double gyromagnetic_ratio, double time_interval_ms, double *p_alive, double *phase,
+                                  double *echo_sums)
+{
+    FieldRun fr;
+    fr.field = field;
+    // gamma = 1e6 * giromagnetic ratio, tau = 1e-3 * time interval, half of their product per
+    // phase sample (NMR_cpmg_cuda.cpp:1010-1011 and the kernels' gammatau)
+    fr.gammatau = 0.5 * (1.0e6 * gyromagnetic_ratio) * (1.0e-3 * time_interval_ms);
+    fr.p_alive = p_alive;
+    fr.phase = phase;
+    return cpmg_common(c, img, w, echoes, steps_per_echo, bc, 0, 3, nullptr, nullptr, echo_sums, &fr);
 }
 
 rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,

@@ -9,6 +9,7 @@
 //   pore count, porosity                    B/NMR_Simulation/NMR_Simulation.cpp:675-727, 919-923
 //   pore-matrix interfaces, S/Vp            B/NMR_Simulation/NMR_Simulation.cpp:818-917
 //   ImageInfo.txt, walkers.csv              B/NMR_Simulation/NMR_Simulation.cpp:1644-1712
+//   internal field (uniform / file)         B/NMR_Simulation/InternalField.cpp:95-200
 //
 // The reference decodes slices with OpenCV's imread(path, 1): a colour
 // (3-channel, 8-bit) image, alpha dropped, 16-bit samples reduced to their
@@ -465,6 +466,39 @@ rwnmr_status rwnmr_write_walkers_csv(const char *path, uint64_t n, const int32_t
     for (uint64_t i = 0; i < n; ++i)
         f << x0[i] << "," << y0[i] << "," << z0[i] << "," << x[i] << "," << y[i] << "," << z[i] << "," << collisions[i]
           << "," << xi_rate[i] << "," << energy[i] << "," << seed[i] << std::endl;
+    return f ? RWNMR_OK : RWNMR_ERR_INVALID;
+}
+
+rwnmr_status rwnmr_internal_field_uniform(const uint8_t *solid, int32_t width, int32_t height, int32_t depth,
+                                          double resolution, double gradient, int32_t direction, double *field)
+{
+    if (!solid || !field || width <= 0 || height <= 0 || depth <= 0)
+        return RWNMR_ERR_INVALID;
+    const double step = gradient * 1.0e-6 * resolution;
+    for (int32_t z = 0; z < depth; ++z)
+        for (int32_t y = 0; y < height; ++y)
+            for (int32_t x = 0; x < width; ++x) {
+                const size_t i = ((size_t)z * height + y) * width + x;
+                const double v = direction == 0 ? x * step : direction == 1 ? y * step : direction == 2 ? z * step : 0.0;
+                field[i] = solid[i] ? 0.0 : v;
+            }
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_internal_field_read(const char *path, int32_t width, int32_t height, int32_t depth, double *field)
+{
+    if (!path || !field || width <= 0 || height <= 0 || depth <= 0)
+        return RWNMR_ERR_INVALID;
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        return RWNMR_ERR_INVALID;
+    f.seekg(0, std::ios::end);
+    const std::streamoff bytes = f.tellg();
+    const size_t n = (size_t)width * height * depth;
+    if (bytes < 0 || (size_t)bytes != n * sizeof(double))
+        return RWNMR_ERR_INVALID;
+    f.seekg(0, std::ios::beg);
+    f.read(reinterpret_cast<char *>(field), (std::streamsize)(n * sizeof(double)));
     return f ? RWNMR_OK : RWNMR_ERR_INVALID;
 }
 

@@ -413,4 +413,109 @@ __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, doub
     });
 }
 
+// Internal-field CPMG on one tile (CPMG_walk_{noflux,periodic,mirror}_field,
+// B/NMR_Simulation/NMR_cpmg_cuda.cpp:146-293, 409-567, 683-841): per echo two
+// phase inversions (phase = -phase) of steps_per_echo / 2 steps each; before
+// and after every step the phase gains gammatau * field at the walker's image
+// voxel (one fused multiply-add, as CUDA contracts phase + gammatau * field);
+// a wall hit multiplies p_alive by delta.  Echo value p_alive cos(phase).
+// Runs on the folded geometry (the field is indexed by the image voxel).
+template <int BC, int K>
+__device__ __forceinline__ void walk_tile_field(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
+                                                double (*lds)[kMaxWaves])
+{
+    const Geom g = make_geom(a);
+    const uint64_t base = tile * blockDim.x * K + threadIdx.x;
+    const uint64_t n = a.n;
+    const uint32_t echoes = a.echoes, half = a.steps_per_echo / 2;
+    const uint64_t row = (uint64_t)a.W, slab = (uint64_t)a.W * (uint64_t)a.H;
+    const double gt = a.gammatau;
+    Walker w[K];
+    double PA[K], PH[K];
+    static_for<K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const uint64_t id = base + (uint64_t)k * blockDim.x;
+        w[k].active = id < n;
+        const uint64_t j = w[k].active ? id : 0;
+        w[k].x = w[k].active ? a.x[j] : 0;
+        w[k].y = w[k].active ? a.y[j] : 0;
+        w[k].z = w[k].active ? a.z[j] : 0;
+        w[k].s = w[k].active ? a.seed[j] : 1;
+        w[k].h = (w[k].active && a.hits) ? a.hits[j] : 0u;
+        w[k].delta = (w[k].active && a.penalty) ? a.penalty[j] : a.delta_uniform;
+        w[k].e = 0.0;
+        PA[k] = w[k].active ? a.p_alive[j] : 0.0;
+        PH[k] = w[k].active ? a.phase[j] : 0.0;
+        if (w[k].active && !start_ok<BC>(g, w[k])) {
+            w[k].active = 0;
+            atomicOr(a.error_flag, 1u);
+        }
+        if (!w[k].active) {
+            w[k].x = w[k].y = w[k].z = 0;
+            PA[k] = 0.0;
+            PH[k] = 0.0;
+        }
+        init_fold<BC>(g, w[k]);
+    });
+    auto fval = [&](const Walker &v) {
+        return a.field[(uint64_t)v.ix + (uint64_t)v.iy * row + (uint64_t)v.iz * slab];
+    };
+    auto echo_value = [&]() {
+        double v = 0.0;
+        static_for<K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            v += PA[k] * cos(PH[k]);
+        });
+        return v;
+    };
+    if (part)
+        stage_echo(lds, 0, echoes, echo_value(), part, part_stride);
+    for (uint32_t echo = 1; echo <= echoes; ++echo) {
+        for (int inv = 0; inv < 2; ++inv) {
+            static_for<K>([&](auto kc) { PH[decltype(kc)::value] = -PH[decltype(kc)::value]; });
+            for (uint32_t st = 0; st < half; ++st) {
+                Move mv[K];
+                uint64_t word[K];
+                static_for<K>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    PH[k] = fma(gt, fval(w[k]), PH[k]);
+                    mv[k] = propose<BC>(g, w[k]);
+                });
+                static_for<K>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    word[k] = w[k].active ? g.blocks[mv[k].blk] : 0ull;
+                });
+                static_for<K>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    if (!((word[k] >> mv[k].bit) & 1ull)) {
+                        commit<BC>(w[k], mv[k]);
+                    } else {
+                        w[k].h += 1u;
+                        PA[k] *= w[k].delta;
+                    }
+                    PH[k] = fma(gt, fval(w[k]), PH[k]);
+                });
+            }
+        }
+        if (part)
+            stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
+    }
+    static_for<K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (w[k].active) {
+            const uint64_t id = base + (uint64_t)k * blockDim.x;
+            a.x[id] = w[k].x;
+            a.y[id] = w[k].y;
+            a.z[id] = w[k].z;
+            a.seed[id] = w[k].s;
+            a.p_alive[id] = PA[k];
+            a.phase[id] = PH[k];
+            if (a.energy)
+                a.energy[id] = PA[k] * cos(PH[k]);
+            if (a.hits)
+                a.hits[id] = w[k].h;
+        }
+    });
+}
+
 }  // namespace rwnmr

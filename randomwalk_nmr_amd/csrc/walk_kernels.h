@@ -41,6 +41,11 @@ struct WalkArgs {
     ZLayout zl;                      // its dilated address layout
     uint32_t zcache;                 // 1: per-walker 16-byte chunk cache in registers
     const uint32_t *zlabels;         // unfolded solid labels, 4 bits per voxel (per-label walks)
+    // internal-field CPMG (emode 3): field[x + y W + z W H], phase step gammatau,
+    // per-walker surviving fraction and phase (in/out)
+    const double *field;
+    double gammatau;
+    double *p_alive, *phase;
 };
 
 struct LaunchShape {

@@ -34,6 +34,16 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void walk_kernel(const WalkAr
     walk_tile<BC, EMODE, K>(a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride, lds);
 }
 
+template <int BC, int K>
+__global__ __launch_bounds__(max_threads_for_k(K)) void walk_field_kernel(const WalkArgs a)
+{
+    __shared__ double lds[kEchoChunk][kMaxWaves];
+    const uint64_t tile = tile_of_block(a);
+    if (a.tile_map && tile >= a.n_tiles)
+        return;
+    walk_tile_field<BC, K>(a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride, lds);
+}
+
 // Voxel-division path (shift > 0): walker coordinates are sub-voxel units
 // and the image voxel is (p >> shift); folds recomputed each step.
 template <int BC, int EMODE>
@@ -241,25 +251,38 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const double *partials
 template <int BC, int EMODE>
 static hipError_t launch_bc_emode(const WalkArgs &a, const LaunchShape &ls, hipStream_t st)
 {
-    if (a.shift != 0) {
-        const uint32_t tpb = ls.threads;
-        const uint64_t blocks = (a.n + tpb - 1) / tpb;
-        hipLaunchKernelGGL((walk_kernel_shift<BC, EMODE>), dim3((uint32_t)blocks), dim3(tpb), 0, st, a);
-        return hipGetLastError();
-    }
     const uint64_t per_block = (uint64_t)ls.threads * ls.k;
     const uint32_t tiles = (uint32_t)((a.n + per_block - 1) / per_block);
     if (a.tile_map && a.n_tiles != tiles)
         return hipErrorInvalidValue;
     // XCD-striped tiles: a grid of 8 * ceil(tiles / 8) workgroups (spares exit)
     const uint32_t blocks = a.tile_map ? ((tiles + kXcds - 1) / kXcds) * kXcds : tiles;
-    switch (ls.k) {
-    case 1: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 1>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 2>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 4>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    default: return hipErrorInvalidValue;
+    if constexpr (EMODE == 3) {
+        // the field walk runs at voxel resolution only
+        if (a.shift != 0 || !a.field || !a.p_alive || !a.phase)
+            return hipErrorInvalidValue;
+        switch (ls.k) {
+        case 1: hipLaunchKernelGGL((walk_field_kernel<BC, 1>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((walk_field_kernel<BC, 2>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((walk_field_kernel<BC, 4>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    } else {
+        if (a.shift != 0) {
+            const uint32_t tpb = ls.threads;
+            const uint64_t sblocks = (a.n + tpb - 1) / tpb;
+            hipLaunchKernelGGL((walk_kernel_shift<BC, EMODE>), dim3((uint32_t)sblocks), dim3(tpb), 0, st, a);
+            return hipGetLastError();
+        }
+        switch (ls.k) {
+        case 1: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 1>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 2>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+        case 4: hipLaunchKernelGGL((walk_kernel<BC, EMODE, 4>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
     }
-    return hipGetLastError();
 }
 
 uint32_t walk_grid_blocks(uint64_t n, const LaunchShape &ls, int shift)
@@ -280,6 +303,7 @@ hipError_t launch_walk(const WalkArgs &a, int bc, int emode, const LaunchShape &
     RW_CASE(kNoflux, 0) RW_CASE(kNoflux, 1) RW_CASE(kNoflux, 2)
     RW_CASE(kPeriodic, 0) RW_CASE(kPeriodic, 1) RW_CASE(kPeriodic, 2)
     RW_CASE(kMirror, 0) RW_CASE(kMirror, 1) RW_CASE(kMirror, 2)
+    RW_CASE(kNoflux, 3) RW_CASE(kPeriodic, 3) RW_CASE(kMirror, 3)
 #undef RW_CASE
     return hipErrorInvalidValue;
 }

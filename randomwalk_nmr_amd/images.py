@@ -84,3 +84,22 @@ def write_walkers_csv(path, x0, y0, z0, x, y, z, collisions, xi_rate, energy, se
                                                     (collisions, np.uint32), (xi_rate, np.float64),
                                                     (energy, np.float64), (seed, np.uint64))]
     check(load().rwnmr_write_walkers_csv(str(path).encode(), len(arrs[0]), *[_ptr(a) for a in arrs]))
+
+
+def internal_field_uniform(solid, *, resolution, gradient, direction):
+    """Field of a uniform gradient (InternalField::fillDataArray): float64
+    [D, H, W], coordinate * gradient * 1e-6 * resolution in pores, 0 in solid."""
+    solid = np.ascontiguousarray(solid, np.uint8)
+    D, H, W = solid.shape
+    out = np.zeros((D, H, W))
+    check(load().rwnmr_internal_field_uniform(_ptr(solid), W, H, D, float(resolution), float(gradient),
+                                              int(direction), _ptr(out)))
+    return out
+
+
+def read_internal_field(path, shape):
+    """Raw W*H*D doubles (InternalField::readDataFromFile) as [D, H, W]."""
+    D, H, W = shape
+    out = np.zeros((D, H, W))
+    check(load().rwnmr_internal_field_read(str(path).encode(), W, H, D, _ptr(out)))
+    return out

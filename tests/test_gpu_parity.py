@@ -199,6 +199,52 @@ def test_cpmg_labels_unfolded_and_folded_match_oracle(ctx, bc, dims, unfolded):
     assert np.array_equal(g["label_hits"], o["label_hits"].T)
 
 
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
+@pytest.mark.parametrize("shape", [(256, 1), (512, 2), (256, 4)])
+@pytest.mark.parametrize("spe", [4, 7])
+def test_internal_field_cpmg_matches_oracle(ctx, bc, shape, spe):
+    """Internal-field CPMG: positions, seeds, hits, p_alive and phases
+    bit-exact (the phase update is one fused multiply-add on both sides);
+    echo sums of p_alive cos(phase) within 1e-12 relative."""
+    print(f"[test] field bc={bc} launch={shape} spe={spe}", flush=True)
+    solid, seeds, x, y, z, pen = _case(26, 19, 23, 3.5, 0.35, 1200, 51)
+    rng = np.random.default_rng(spe)
+    field = rng.normal(0.0, 1e-3, solid.shape)
+    field[solid == 1] = 0.0
+    gyro, dt_ms = 42.576, 0.0333
+    gt = 0.5 * (1.0e6 * gyro) * (1.0e-3 * dt_ms)
+    img = R.Image.from_solid(ctx, solid)
+    ctx.set_launch(*shape)
+    try:
+        g = R.cpmg_field_run(ctx, img, x, y, z, seeds, echoes=33, steps_per_echo=spe, field=field,
+                             gyromagnetic_ratio=gyro, time_interval_ms=dt_ms, bc=bc, penalty=pen)
+    finally:
+        ctx.set_launch(256, 1)
+    o = O.walk_field(O.bitblock_pack(solid), (26, 19, 23), x, y, z, seeds, echoes=33, steps_per_echo=spe,
+                     field=field, gammatau=gt, bc=bc, penalty=pen)
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.array_equal(g["p_alive"], o["p_alive"])
+    assert np.array_equal(g["phase"], o["phase"])
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+
+
+def test_internal_field_state_carries_across_calls(ctx):
+    """Two calls of 10 + 15 echoes continuing p_alive / phase == one call of 25."""
+    print("[test] field continuation", flush=True)
+    solid, seeds, x, y, z, pen = _case(20, 20, 20, 3, 0.35, 700, 61)
+    field = np.random.default_rng(1).normal(0.0, 1e-3, solid.shape)
+    img = R.Image.from_solid(ctx, solid)
+    kw = dict(field=field, gyromagnetic_ratio=42.576, time_interval_ms=0.05, bc="mirror", penalty=pen)
+    a = R.cpmg_field_run(ctx, img, x, y, z, seeds, echoes=10, steps_per_echo=6, **kw)
+    b = R.cpmg_field_run(ctx, img, a["x"], a["y"], a["z"], a["seed"], echoes=15, steps_per_echo=6,
+                         p_alive=a["p_alive"], phase=a["phase"], hits=a["hits"], **kw)
+    c = R.cpmg_field_run(ctx, img, x, y, z, seeds, echoes=25, steps_per_echo=6, **kw)
+    for k in ("x", "y", "z", "seed", "hits", "p_alive", "phase"):
+        assert np.array_equal(b[k], c[k]), k
+    assert np.array_equal(b["echo_sums"], c["echo_sums"][10:])
+
+
 @pytest.mark.parametrize("bc", ["noflux", "mirror"])
 def test_pfgse_matches_oracle(ctx, bc):
     print(f"[test] pfgse bc={bc}", flush=True)

@@ -203,3 +203,18 @@ def test_image_info_and_walkers_csv_formats(tmp_path):
     assert lines[0] == "PositionXi,PositionYi,PositionZi,PositionXf,PositionYf,PositionZf,Collisions,XIRate,Energy,RNGSeed"
     assert lines[1] == "1,3,5,7,9,11,13,0.5,0.123457,9223372036854775813"
     assert lines[2] == "-2,4,6,8,-10,12,0,1e-07,1,17"
+
+
+def test_internal_field_uniform_and_file_round_trip(tmp_path):
+    rng = np.random.default_rng(4)
+    solid = (rng.random((4, 5, 6)) < 0.3).astype(np.uint8)
+    for direction, axis in ((0, 2), (1, 1), (2, 0)):
+        f = I.internal_field_uniform(solid, resolution=2.0, gradient=30.0, direction=direction)
+        coord = np.indices(solid.shape)[axis].astype(float)
+        want = np.where(solid == 1, 0.0, coord * (30.0 * 1e-6 * 2.0))
+        assert np.array_equal(f, want)
+    p = tmp_path / "field.bin"
+    f.tofile(p)
+    assert np.array_equal(I.read_internal_field(p, solid.shape), f)
+    with pytest.raises(R.RwnmrError):
+        I.read_internal_field(p, (4, 5, 7))

@@ -5,6 +5,7 @@ independent numpy / pure-Python restatements, brute force on small domains,
 closed forms of the random walk, the C++ standard's mt19937 value, libstdc++'s
 uniform_int_distribution, and the reference's own PNG stacks.
 """
+import math
 import os
 import subprocess
 
@@ -12,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import randomwalk_nmr_amd as R
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 M64 = (1 << 64) - 1
@@ -308,3 +310,89 @@ def test_seed_and_place_uses_pore_list():
     seeds, x, y, z = O.seed_and_place(5, 1000, pores, solid.size)
     assert np.all(solid[z, y, x] == 0)
     assert len(set(seeds.tolist())) == 1000
+
+
+# ---------------------------------------------------------- internal field
+
+def py_walk_field(solid, field, x, y, z, s, echoes, spe, bc, delta, gt):
+    """Independent pure-Python internal-field walker: per echo two phase
+    inversions of spe // 2 steps, phase += gt * field(image voxel) before and
+    after each step, echo value p_alive * cos(phase)."""
+    D, H, W = solid.shape
+    dims = (W, H, D)
+
+    def img(p):
+        if bc == "periodic":
+            return [p[i] % dims[i] for i in range(3)]
+        if bc == "mirror":
+            return [_reflect(p[i], dims[i]) for i in range(3)]
+        return list(p)
+
+    pa, ph, vals = 1.0, 0.0, [1.0]
+    for _ in range(echoes):
+        for _inv in range(2):
+            ph = -ph
+            for _ in range(spe // 2):
+                v = img((x, y, z))
+                ph = ph + gt * field[v[2], v[1], v[0]]
+                x, y, z, s, e, h = py_walk(solid, x, y, z, s, 1, bc, delta)
+                pa *= e
+                v = img((x, y, z))
+                ph = ph + gt * field[v[2], v[1], v[0]]
+        vals.append(pa * math.cos(ph))
+    return x, y, z, s, pa, ph, vals
+
+
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
+@pytest.mark.parametrize("spe", [2, 5])
+def test_field_walk_matches_pure_python(bc, spe):
+    """Field values are small integers and gt a power of two, so every
+    product and partial sum is exact and fused or separate arithmetic agree:
+    positions, p_alive, phases bit-exact; echo values to libm rounding."""
+    rng = np.random.default_rng(11 + spe)
+    D, H, W = 5, 6, 7
+    solid = (rng.random((D, H, W)) < 0.3).astype(np.uint8)
+    field = rng.integers(-40, 41, (D, H, W)).astype(np.float64)
+    pz, py, px = np.nonzero(solid == 0)
+    idx = rng.choice(len(px), 9)
+    x, y, z = px[idx].astype(np.int32), py[idx].astype(np.int32), pz[idx].astype(np.int32)
+    seeds = rng.integers(1, 2**63, 9, dtype=np.int64).astype(np.uint64)
+    gt = 2.0 ** -7
+    out = O.walk_field(O.bitblock_pack(solid), (W, H, D), x, y, z, seeds, echoes=4, steps_per_echo=spe,
+                       field=field, gammatau=gt, bc=bc, penalty=0.8)
+    tot = np.zeros(5)
+    for i in range(9):
+        rx, ry, rz, rs, pa, ph, vals = py_walk_field(solid, field, int(x[i]), int(y[i]), int(z[i]), int(seeds[i]),
+                                                     4, spe, bc, 0.8, gt)
+        assert (out["x"][i], out["y"][i], out["z"][i], int(out["seed"][i])) == (rx, ry, rz, rs)
+        assert out["p_alive"][i] == pa and out["phase"][i] == ph
+        tot += vals
+    assert np.allclose(out["echo_sums"], tot, rtol=1e-14, atol=1e-14)
+
+
+def test_field_walk_zero_field_is_the_plain_walk():
+    """No field: phase 0, echo value = p_alive = the plain walk's energy after
+    2 * (spe // 2) steps per echo."""
+    solid, _ = R.sphere_pack(20, 18, 16, 3, 5, 0.35, 4)
+    x, y, z, seeds = _pore_walkers(solid, 300, 2)
+    f = O.walk_field(O.bitblock_pack(solid), (20, 18, 16), x, y, z, seeds, echoes=15, steps_per_echo=7,
+                     field=np.zeros((16, 18, 20)), gammatau=3.0, bc="mirror", penalty=0.97)
+    p = O.walk(O.bitblock_pack(solid), (20, 18, 16), x, y, z, seeds, echoes=15, steps_per_echo=6, bc="mirror",
+               penalty=0.97)
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(f[k], p[k]), k
+    assert np.array_equal(f["p_alive"], p["energy"]) and not np.any(f["phase"])
+    assert np.array_equal(f["echo_sums"], p["echo_sums"])
+
+
+def test_field_walk_constant_field_refocuses():
+    """A uniform field dephases during the first half-echo and exactly
+    refocuses during the second (CPMG refocusing): phase 0 at every echo."""
+    solid, _ = R.sphere_pack(16, 16, 16, 3, 4, 0.4, 6)
+    x, y, z, seeds = _pore_walkers(solid, 200, 3)
+    f = O.walk_field(O.bitblock_pack(solid), (16, 16, 16), x, y, z, seeds, echoes=9, steps_per_echo=4,
+                     field=np.full((16, 16, 16), 3.0), gammatau=0.25, bc="periodic", penalty=0.9)
+    assert not np.any(f["phase"])
+    plain = O.walk(O.bitblock_pack(solid), (16, 16, 16), x, y, z, seeds, echoes=9, steps_per_echo=4,
+                   bc="periodic", penalty=0.9)
+    assert np.array_equal(f["echo_sums"], plain["echo_sums"])
