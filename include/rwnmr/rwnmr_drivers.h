@@ -93,6 +93,55 @@ int32_t rwnmr_stejskal_tanner(const double *rhs, const double *lhs, uint32_t n, 
 /* Points kept by the LHS-value threshold (setThresholdFromLHSValue, NMR_pfgse.cpp:399-419). */
 uint32_t rwnmr_pfgse_threshold_lhs(const double *lhs, uint32_t n, double value, uint32_t window);
 
+/* ---- collision-histogram ("fast") CPMG method ---- */
+
+/* Bins of a histogram of `size`: size (linear) or size + 1 (log scale; bin 0
+ * is the no-collision control entry), CollisionHistogram::createBlankHistogram
+ * (B/NMR_Simulation/CollisionHistogram.cpp:57-72). */
+int32_t rwnmr_histogram_bins(int32_t size, int32_t log_scale);
+/* Echo segments of the histogram list (NMR_Simulation::initHistogramList,
+ * B/NMR_Simulation/NMR_Simulation.cpp:1456-1479): n_histograms segments of
+ * echoes / n echoes (one if echoes < n), the remainder added to the last.
+ * first_echo / last_echo hold n_histograms entries; *n_used gets the count. */
+rwnmr_status rwnmr_histogram_list(uint32_t echoes, int32_t n_histograms, int32_t *n_used, uint32_t *first_echo,
+                                  uint32_t *last_echo);
+/* Histogram of collision rates xi = collisions / steps over n walkers
+ * (CollisionHistogram::fillHistogram, CollisionHistogram.cpp:74-250):
+ * linear: bins (i + 1/2) / size, index floor(xi size) (xi = 1 in the last
+ * bin); log: bin 0 = round(log10 min rate) - 1 counts walkers without
+ * collisions, bins 1.. = 10^(e0 + (i - 1)(0 - e0)/(size - 1)) with
+ * e0 = round(log10 min rate), index round((log10 xi - log10 bin1) / gap) + 1
+ * (non-positive -> bin 1).  Amplitudes are fractions of walkers. */
+rwnmr_status rwnmr_collision_histogram(const uint32_t *collisions, uint64_t n, uint64_t steps, int32_t size,
+                                       int32_t log_scale, double *bins, double *amps);
+/* Per-bin penalties delta(rho(xi))^(xi steps_per_echo) with xi = the bin value
+ * (NMR_cpmg::createPenaltiesVector, B/NMR_Simulation/NMR_cpmg.cpp:235-279);
+ * rho uniform, or the 8-parameter two-sigmoid of xi when sigmoid != NULL
+ * (B/Walker/walker.h:174-196); delta = 1 - (2/3) rho step_length / (D0 1e3). */
+rwnmr_status rwnmr_histogram_penalties(const double *bins, int32_t n_bins, double rho, const double *sigmoid,
+                                       double step_length, double D0, uint32_t steps_per_echo, double *penalties);
+/* Histogram-propagated decay (NMR_cpmg::histogram_simulation,
+ * NMR_cpmg.cpp:281-344): signal[0] = 1; for each segment h the bin energies
+ * start at (last signal) * amps[h][i] and are multiplied by the penalties
+ * once per echo of the segment, the signal being their sum.  amps is
+ * [n_hist][n_bins]; signal gets 1 + sum of segment lengths values. */
+rwnmr_status rwnmr_histogram_decay(const double *amps, int32_t n_hist, int32_t n_bins, const uint32_t *first_echo,
+                                   const uint32_t *last_echo, const double *penalties, double *signal);
+/* The whole histogram method on the GPU map walk (mapSimulation with
+ * histograms, B/NMR_Simulation/NMR_Simulation_cuda.cpp:726-1270, then
+ * NMR_cpmg::run_simulation's "histogram" branch, NMR_cpmg.cpp:158-165): per
+ * segment a map walk of steps_per_echo x (segment echoes) steps with fresh
+ * collision counts (walkers keep moving from segment to segment), a segment
+ * histogram; a global histogram of the summed collisions over all steps
+ * gives the bins for the penalties.  signal: echoes + 1 values (before bulk
+ * relaxation and normalisation).  bins_out ([n_bins]) and amps_out
+ * ([n_used][n_bins]) are nullable. */
+rwnmr_status rwnmr_cpmg_histogram_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                           uint32_t steps_per_echo, int32_t bc, uint32_t shift, int32_t n_histograms,
+                                           int32_t size, int32_t log_scale, double rho, const double *sigmoid,
+                                           double step_length, double D0, double *signal, double *bins_out,
+                                           double *amps_out, int32_t *n_used);
+
 #ifdef __cplusplus
 }
 #endif

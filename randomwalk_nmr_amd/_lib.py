@@ -104,6 +104,13 @@ SIGNATURES = {
     "rwnmr_map_run": (I32, [P, P, ct.POINTER(Walkers), U64, I32, U32]),
     "rwnmr_cpmg_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
     "rwnmr_cpmg_run_labels": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P, P]),
+    "rwnmr_histogram_bins": (I32, [I32, I32]),
+    "rwnmr_histogram_list": (I32, [U32, I32, ct.POINTER(I32), P, P]),
+    "rwnmr_collision_histogram": (I32, [P, U64, U64, I32, I32, P, P]),
+    "rwnmr_histogram_penalties": (I32, [P, I32, D, P, D, D, U32, P]),
+    "rwnmr_histogram_decay": (I32, [P, I32, I32, P, P, P, P]),
+    "rwnmr_cpmg_histogram_simulate": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, I32, I32, I32, D, P, D,
+                                            D, P, P, P, ct.POINTER(I32)]),
     "rwnmr_cpmg_field_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, P, D, D, P, P, P]),
     "rwnmr_internal_field_uniform": (I32, [P, I32, I32, I32, D, D, I32, P]),
     "rwnmr_internal_field_read": (I32, [ct.c_char_p, I32, I32, I32, P]),

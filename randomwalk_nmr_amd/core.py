@@ -410,6 +410,74 @@ def cpmg_field_run(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per
     return dict(x=x, y=y, z=z, seed=seed, p_alive=pa, phase=ph, energy=energy, hits=hits, echo_sums=es)
 
 
+def collision_histogram(collisions, steps, *, size, log_scale=False):
+    """(bins, amps) of walker collision rates (rwnmr_collision_histogram)."""
+    c = np.ascontiguousarray(collisions, np.uint32)
+    nb = load().rwnmr_histogram_bins(int(size), int(log_scale))
+    bins, amps = np.zeros(nb), np.zeros(nb)
+    check(load().rwnmr_collision_histogram(_ptr(c), len(c), int(steps), int(size), int(log_scale), _ptr(bins),
+                                           _ptr(amps)))
+    return bins, amps
+
+
+def histogram_list(echoes, n_histograms):
+    """[(first_echo, last_echo)] of the histogram segments."""
+    f = np.zeros(n_histograms, np.uint32)
+    l = np.zeros(n_histograms, np.uint32)
+    n = ct.c_int32()
+    check(load().rwnmr_histogram_list(int(echoes), int(n_histograms), ct.byref(n), _ptr(f), _ptr(l)))
+    return [(int(f[i]), int(l[i])) for i in range(n.value)]
+
+
+def histogram_penalties(bins, *, rho, step_length, D0, steps_per_echo, sigmoid=None):
+    b = np.ascontiguousarray(bins, np.float64)
+    out = np.zeros(len(b))
+    sg = None if sigmoid is None else np.ascontiguousarray(sigmoid, np.float64)
+    check(load().rwnmr_histogram_penalties(_ptr(b), len(b), float(rho), _ptr(sg), float(step_length), float(D0),
+                                           int(steps_per_echo), _ptr(out)))
+    return out
+
+
+def histogram_decay(amps, segments, penalties):
+    a = np.ascontiguousarray(amps, np.float64)
+    nh, nb = a.shape
+    f = np.array([s[0] for s in segments], np.uint32)
+    l = np.array([s[1] for s in segments], np.uint32)
+    p = np.ascontiguousarray(penalties, np.float64)
+    sig = np.zeros(1 + int((l - f).sum()))
+    check(load().rwnmr_histogram_decay(_ptr(a), nh, nb, _ptr(f), _ptr(l), _ptr(p), _ptr(sig)))
+    return sig
+
+
+def cpmg_histogram_simulate(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo, bc="mirror",
+                            n_histograms=1, size=100, log_scale=False, rho=0.0, sigmoid=None, step_length=1.0,
+                            D0=2.5):
+    """Collision-histogram CPMG on the GPU map walk
+    (rwnmr_cpmg_histogram_simulate).  Host walkers are copied and returned;
+    device walkers are updated in place.  Returns dict(signal, bins, amps,
+    x, y, z, seed)."""
+    dev = _is_dev(x)
+    n = len(x)
+    if dev:
+        _torch_sync()
+        w = _walker_struct(x, y, z, seed, None, None, None, 1.0, n, 1)
+    else:
+        x, y, z = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+        seed = _host(seed, np.uint64)
+        w = _walker_struct(x, y, z, seed, None, None, None, 1.0, n, 0)
+    nb = load().rwnmr_histogram_bins(int(size), int(log_scale))
+    sig = np.zeros(echoes + 1)
+    bins = np.zeros(nb)
+    amps = np.zeros((n_histograms, nb))
+    used = ct.c_int32()
+    sg = None if sigmoid is None else np.ascontiguousarray(sigmoid, np.float64)
+    check(load().rwnmr_cpmg_histogram_simulate(ctx.handle, img.handle, ct.byref(w), int(echoes), int(steps_per_echo),
+                                               bc_code(bc), 0, int(n_histograms), int(size), int(log_scale),
+                                               float(rho), _ptr(sg), float(step_length), float(D0), _ptr(sig),
+                                               _ptr(bins), _ptr(amps), ct.byref(used)), ctx.handle)
+    return dict(signal=sig, bins=bins, amps=amps[:used.value], x=x, y=y, z=z, seed=seed)
+
+
 def cpmg_batch_run(ctx: Context, instances, *, sequential=False, sync=True):
     """Many CPMG instances in one persistent launch (rwnmr_cpmg_batch_run).
 

@@ -285,4 +285,73 @@ rwnmr_status rwnmr_pfgse_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_
     return RWNMR_OK;
 }
 
+rwnmr_status rwnmr_cpmg_histogram_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                           uint32_t steps_per_echo, int32_t bc, uint32_t shift, int32_t n_histograms,
+                                           int32_t size, int32_t log_scale, double rho, const double *sigmoid,
+                                           double step_length, double D0, double *signal, double *bins_out,
+                                           double *amps_out, int32_t *n_used)
+{
+    if (!ctx || !img || !w || !signal || echoes == 0 || steps_per_echo == 0 || n_histograms <= 0 || size <= 0 ||
+        w->n == 0)
+        return RWNMR_ERR_INVALID;
+    const uint64_t n = w->n;
+    std::vector<uint32_t> first((size_t)n_histograms), last((size_t)n_histograms);
+    int32_t nh = 0;
+    rwnmr_status st = rwnmr_histogram_list(echoes, n_histograms, &nh, first.data(), last.data());
+    if (st != RWNMR_OK)
+        return st;
+    const int32_t nb = rwnmr_histogram_bins(size, log_scale);
+    std::vector<double> amps((size_t)nh * nb), seg_bins(nb), bins(nb), pen(nb);
+    std::vector<uint32_t> hits(n, 0u), total(n, 0u);
+    // the map walk counts into a hits array of the walkers' memory kind
+    uint32_t *dev_hits = nullptr;
+    if (w->on_device && hipMalloc((void **)&dev_hits, n * 4) != hipSuccess)
+        return RWNMR_ERR_ALLOC;
+    rwnmr_walkers m = *w;
+    m.energy = nullptr;
+    m.hits = w->on_device ? dev_hits : hits.data();
+    for (int32_t h = 0; h < nh && st == RWNMR_OK; ++h) {
+        const uint64_t steps = (uint64_t)steps_per_echo * (last[h] - first[h]);
+        if (w->on_device) {
+            if (hipMemset(dev_hits, 0, n * 4) != hipSuccess)
+                st = RWNMR_ERR_HIP;
+        } else {
+            std::fill(hits.begin(), hits.end(), 0u);
+        }
+        if (st == RWNMR_OK)
+            st = rwnmr_map_run(ctx, img, &m, steps, bc, shift);
+        if (st == RWNMR_OK && w->on_device) {
+            if (rwnmr_ctx_synchronize(ctx) != RWNMR_OK || hipMemcpy(hits.data(), dev_hits, n * 4,
+                                                                    hipMemcpyDeviceToHost) != hipSuccess)
+                st = RWNMR_ERR_HIP;
+        }
+        if (st == RWNMR_OK)
+            st = rwnmr_collision_histogram(hits.data(), n, steps, size, log_scale, seg_bins.data(),
+                                           amps.data() + (size_t)h * nb);
+        for (uint64_t i = 0; i < n; ++i)
+            total[i] += hits[i];
+    }
+    if (dev_hits)
+        (void)hipFree(dev_hits);
+    if (st != RWNMR_OK)
+        return st;
+    std::vector<double> gamps(nb);
+    if ((st = rwnmr_collision_histogram(total.data(), n, (uint64_t)last[nh - 1] * steps_per_echo, size, log_scale,
+                                        bins.data(), gamps.data())) != RWNMR_OK)
+        return st;
+    if ((st = rwnmr_histogram_penalties(bins.data(), nb, rho, sigmoid, step_length, D0, steps_per_echo,
+                                        pen.data())) != RWNMR_OK)
+        return st;
+    if ((st = rwnmr_histogram_decay(amps.data(), nh, nb, first.data(), last.data(), pen.data(), signal)) !=
+        RWNMR_OK)
+        return st;
+    if (bins_out)
+        std::copy(bins.begin(), bins.end(), bins_out);
+    if (amps_out)
+        std::copy(amps.begin(), amps.end(), amps_out);
+    if (n_used)
+        *n_used = nh;
+    return RWNMR_OK;
+}
+
 }  // extern "C"

@@ -245,6 +245,50 @@ def test_internal_field_state_carries_across_calls(ctx):
     assert np.array_equal(b["echo_sums"], c["echo_sums"][10:])
 
 
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
+@pytest.mark.parametrize("log_scale", [False, True])
+@pytest.mark.parametrize("device", [False, True])
+def test_histogram_method_matches_oracle(ctx, bc, log_scale, device):
+    """Collision-histogram CPMG: the GPU map walk per segment (fresh counts,
+    walkers carried over) feeding the histogram arithmetic == oracle map
+    walks feeding the oracle's restatement of the histogram method."""
+    print(f"[test] histogram method bc={bc} log={log_scale} device={device}", flush=True)
+    from oracle import histogram as OH
+    solid, seeds, x, y, z, _ = _case(24, 22, 20, 3.5, 0.35, 3000, 71)
+    echoes, spe, nh, size = 40, 4, 3, 16
+    img = R.Image.from_solid(ctx, solid)
+    if device:
+        import torch
+        dev = torch.device("cuda:0")
+        X, Y, Z = (torch.from_numpy(a.copy()).to(dev) for a in (x, y, z))
+        S = torch.from_numpy(seeds.view(np.int64).copy()).to(dev)
+        g = R.cpmg_histogram_simulate(ctx, img, X, Y, Z, S, echoes=echoes, steps_per_echo=spe, bc=bc,
+                                      n_histograms=nh, size=size, log_scale=log_scale, rho=15.0, step_length=1.0,
+                                      D0=2.5)
+        gx = X.cpu().numpy()
+    else:
+        g = R.cpmg_histogram_simulate(ctx, img, x, y, z, seeds, echoes=echoes, steps_per_echo=spe, bc=bc,
+                                      n_histograms=nh, size=size, log_scale=log_scale, rho=15.0, step_length=1.0,
+                                      D0=2.5)
+        gx = g["x"]
+    segs = OH.histogram_list(echoes, nh)
+    blocks = O.bitblock_pack(solid)
+    cx, cy, cz, cs = x, y, z, seeds
+    total = np.zeros(len(x), np.int64)
+    amps = []
+    for f, l in segs:
+        o = O.walk(blocks, (24, 22, 20), cx, cy, cz, cs, echoes=1, steps_per_echo=spe * (l - f), bc=bc,
+                   echo_sums=False)
+        cx, cy, cz, cs = o["x"], o["y"], o["z"], o["seed"]
+        total += o["hits"]
+        amps.append(OH.collision_histogram([int(h) for h in o["hits"]], spe * (l - f), size, log_scale)[1])
+    bins, _ = OH.collision_histogram([int(h) for h in total], segs[-1][1] * spe, size, log_scale)
+    sig = OH.decay(amps, segs, OH.penalties(bins, 15.0, 1.0, 2.5, spe))
+    assert np.array_equal(gx, cx)
+    assert np.array_equal(g["bins"], bins) and np.array_equal(g["amps"], np.array(amps))
+    assert np.allclose(g["signal"], sig, rtol=1e-13, atol=0)
+
+
 @pytest.mark.parametrize("bc", ["noflux", "mirror"])
 def test_pfgse_matches_oracle(ctx, bc):
     print(f"[test] pfgse bc={bc}", flush=True)

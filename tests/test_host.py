@@ -221,3 +221,45 @@ def test_step_arithmetic_host_build(tmp_path):
                    check=True)
     out = subprocess.run([exe], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.strip().endswith("OK"), out.stdout[-2000:]
+
+
+# ------------------------------------------------------- histogram method
+
+from oracle import histogram as OH  # noqa: E402
+
+
+def test_histogram_oracle_closed_forms():
+    # every walker at one rate: all mass in one bin; one-bin decay = penalty^e
+    bins, amps = OH.collision_histogram([35] * 50, 100, 10)
+    assert amps[3] == 1.0 and abs(bins[3] - 0.35) < 1e-15
+    segs = OH.histogram_list(12, 1)
+    pen = OH.penalties(bins, 10.0, 1.0, 2.5, 4)
+    sig = OH.decay([amps], segs, pen)
+    assert np.allclose(sig, pen[3] ** np.arange(13), rtol=1e-14)
+    # rate 1 lands in the last linear bin; amplitudes conserve probability
+    bins, amps = OH.collision_histogram([100, 0, 50, 99], 100, 4)
+    assert list(amps) == [0.25, 0.0, 0.25, 0.5]
+    b2, a2 = OH.collision_histogram([0, 0, 3, 40, 400, 1000], 1000, 6, log_scale=True)
+    assert abs(a2.sum() - 1.0) < 1e-15 and a2[0] == 2 / 6
+    assert b2[0] == -4.0 and abs(b2[1] - 1e-3) < 1e-18 and abs(b2[-1] - 1.0) < 1e-15
+    assert OH.histogram_list(10, 3) == [(0, 3), (3, 6), (6, 10)] and OH.histogram_list(2, 5) == [(0, 2)]
+
+
+@pytest.mark.parametrize("log_scale", [False, True])
+@pytest.mark.parametrize("size", [2, 7, 50])
+def test_histogram_library_matches_oracle(log_scale, size):
+    rng = np.random.default_rng(size + 10 * log_scale)
+    steps = 997
+    col = rng.integers(0, steps + 1, 5000).astype(np.uint32)
+    col[:300] = 0
+    col[300:320] = steps
+    b, a = R.collision_histogram(col, steps, size=size, log_scale=log_scale)
+    ob, oa = OH.collision_histogram([int(c) for c in col], steps, size, log_scale)
+    assert np.array_equal(b, ob) and np.array_equal(a, oa)
+    sig = [5.0, 10.0, 0.3, 1.0, 2.0, 0.05, 40.0, 3.0] if size == 7 else None
+    p = R.histogram_penalties(b, rho=12.0, step_length=0.8, D0=2.5, steps_per_echo=6, sigmoid=sig)
+    op = OH.penalties(ob, 12.0, 0.8, 2.5, 6, sig)
+    assert np.allclose(p, op, rtol=1e-15, atol=0)
+    segs = R.histogram_list(23, 4)
+    amps = np.stack([a, a[::-1], a, a[::-1]])
+    assert np.allclose(R.histogram_decay(amps, segs, p), OH.decay(amps, segs, op), rtol=1e-14)
