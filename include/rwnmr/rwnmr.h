@@ -251,6 +251,20 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walke
                              uint32_t n_delta, const double *kx, const double *ky, const double *kz,
                              uint32_t n_k, int32_t k_per_delta, double voxel, int32_t bc, uint32_t shift,
                              uint32_t n_sets, const double *set_delta, double *M, double *msd);
+/* PFGSE with walker-sampling statistics (NMR_PFGSE::recoverDsatWithSampling,
+ * B/NMR_Simulation/NMR_pfgse.cpp:876-1060; samples as
+ * NMR_pfgse_cuda.cpp:1350-2906 form them): walkers split into n_samples
+ * consecutive groups of N / n_samples (the remaining walkers walk but
+ * belong to no sample); M_samples gets [n_samples][sets|1][n_delta][n_k]
+ * sums per sample, M and msd the all-walker results of rwnmr_pfgse_run.
+ * Each sample is summed over whole blocks of its own, so its sums do not
+ * depend on the other samples. */
+rwnmr_status rwnmr_pfgse_run_sampled(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,
+                                     const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc,
+                                     uint32_t n_delta, const double *kx, const double *ky, const double *kz,
+                                     uint32_t n_k, int32_t k_per_delta, double voxel, int32_t bc, uint32_t n_sets,
+                                     const double *set_delta, uint32_t n_samples, double *M_samples, double *M,
+                                     double *msd);
 
 /* Laplace inversion, same names and semantics as the reference's C API
  * (B/Laplace/include/nmrinv.h:13-39, B/Laplace/src/nmrinv.cpp): one global

@@ -90,6 +90,18 @@ double rwnmr_pfgse_wave_vector(double gradient, double pulse_width, double gamma
 /* No-intercept least squares slope and its standard error
  * (LeastSquareAdjust.cpp:77-170; NMR_pfgse.cpp:678-682). */
 int32_t rwnmr_stejskal_tanner(const double *rhs, const double *lhs, uint32_t n, double *D, double *D_err);
+/* Walker-sampling statistics of one diffusion time (NMR_PFGSE::
+ * recoverDsatWithSampling, B/NMR_Simulation/NMR_pfgse.cpp:876-1060, after
+ * its optional bulk and noise terms, which the caller applies to M_samples):
+ * M_samples is [n_samples][n_k] (one energy set of rwnmr_pfgse_run_sampled);
+ * per sample LHS_k = log(max(M_k / M_0, 1e-6)); mean and population standard
+ * deviation over samples of M and LHS per k (any output may be NULL); one
+ * no-intercept Stejskal-Tanner fit per sample over the first n_points points
+ * (rhs shared), D_mean / D_std / mean fit error over samples. */
+rwnmr_status rwnmr_pfgse_sample_stats(const double *M_samples, uint32_t n_samples, uint32_t n_k, const double *rhs,
+                                      uint32_t n_points, double *mean_M, double *std_M, double *mean_lhs,
+                                      double *std_lhs, double *D_samples, double *D_mean, double *D_std,
+                                      double *D_err_mean);
 /* Points kept by the LHS-value threshold (setThresholdFromLHSValue, NMR_pfgse.cpp:399-419). */
 uint32_t rwnmr_pfgse_threshold_lhs(const double *lhs, uint32_t n, double value, uint32_t window);
 

@@ -104,6 +104,10 @@ SIGNATURES = {
     "rwnmr_map_run": (I32, [P, P, ct.POINTER(Walkers), U64, I32, U32]),
     "rwnmr_cpmg_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
     "rwnmr_cpmg_run_labels": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P, P]),
+    "rwnmr_pfgse_run_sampled": (I32, [P, P, ct.POINTER(Walkers), P, P, P, P, U32, P, P, P, U32, I32, D, I32, U32, P,
+                                      U32, P, P, P]),
+    "rwnmr_pfgse_sample_stats": (I32, [P, U32, U32, P, U32, P, P, P, P, P, ct.POINTER(D), ct.POINTER(D),
+                                       ct.POINTER(D)]),
     "rwnmr_histogram_bins": (I32, [I32, I32]),
     "rwnmr_histogram_list": (I32, [U32, I32, ct.POINTER(I32), P, P]),
     "rwnmr_collision_histogram": (I32, [P, U64, U64, I32, I32, P, P]),

@@ -410,6 +410,50 @@ def cpmg_field_run(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per
     return dict(x=x, y=y, z=z, seed=seed, p_alive=pa, phase=ph, energy=energy, hits=hits, echo_sums=es)
 
 
+def pfgse_run_sampled(ctx: Context, img: Image, x, y, z, seed, *, n_samples, steps_inc, kx, ky, kz, voxel,
+                      bc="mirror", penalty=None, delta=1.0, set_delta=None):
+    """PFGSE with walker samples (rwnmr_pfgse_run_sampled), host arrays.
+    Returns dict(M_samples [n_samples, sets|1, n_delta, n_k], M, msd and
+    the final walker state)."""
+    n = len(x)
+    x0, y0, z0 = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+    x, y, z = x0.copy(), y0.copy(), z0.copy()
+    seed = _host(seed, np.uint64)
+    energy, hits = np.ones(n), np.zeros(n, np.uint32)
+    pen = None if penalty is None else _host(penalty, np.float64, n)
+    si = np.ascontiguousarray(steps_inc, np.uint32)
+    kx, ky, kz = (np.ascontiguousarray(k, np.float64) for k in (kx, ky, kz))
+    per_delta = kx.ndim == 2
+    nd, nk = len(si), kx.shape[-1]
+    sd = None if set_delta is None else np.ascontiguousarray(set_delta, np.float64)
+    ns = 0 if sd is None else len(sd)
+    Ms = np.zeros((n_samples, max(ns, 1), nd, nk))
+    M = np.zeros((max(ns, 1), nd, nk))
+    msd = np.zeros((nd, 3))
+    w = _walker_struct(x, y, z, seed, energy, hits, pen, delta, n, 0)
+    check(load().rwnmr_pfgse_run_sampled(ctx.handle, img.handle, ct.byref(w), _ptr(x0), _ptr(y0), _ptr(z0), _ptr(si),
+                                         nd, _ptr(kx), _ptr(ky), _ptr(kz), nk, int(per_delta), float(voxel),
+                                         bc_code(bc), ns, _ptr(sd), int(n_samples), _ptr(Ms), _ptr(M), _ptr(msd)),
+          ctx.handle)
+    return dict(M_samples=Ms, M=M, msd=msd, x=x, y=y, z=z, seed=seed, energy=energy, hits=hits)
+
+
+def pfgse_sample_stats(M_samples, rhs, n_points):
+    """Walker-sampling statistics of one diffusion time (rwnmr_pfgse_sample_stats):
+    M_samples [n_samples, n_k]."""
+    Ms = np.ascontiguousarray(M_samples, np.float64)
+    S, nk = Ms.shape
+    r = np.ascontiguousarray(rhs, np.float64)
+    out = {k: np.zeros(nk) for k in ("mean_M", "std_M", "mean_lhs", "std_lhs")}
+    Ds = np.zeros(S)
+    dm, dsd, de = ct.c_double(), ct.c_double(), ct.c_double()
+    check(load().rwnmr_pfgse_sample_stats(_ptr(Ms), S, nk, _ptr(r), int(n_points), _ptr(out["mean_M"]),
+                                          _ptr(out["std_M"]), _ptr(out["mean_lhs"]), _ptr(out["std_lhs"]), _ptr(Ds),
+                                          ct.byref(dm), ct.byref(dsd), ct.byref(de)))
+    out.update(D_samples=Ds, D_mean=dm.value, D_std=dsd.value, D_err_mean=de.value)
+    return out
+
+
 def collision_histogram(collisions, steps, *, size, log_scale=False):
     """(bins, amps) of walker collision rates (rwnmr_collision_histogram)."""
     c = np.ascontiguousarray(collisions, np.uint32)

@@ -81,6 +81,7 @@ struct rwnmr_ctx {
     DevBuf partials, sums, label_delta, label_hits, steps_inc, kvec, set_delta;
     // internal-field CPMG staging (host-pointer calls)
     DevBuf field, p_alive, phase;
+    DevBuf msamples;   // PFGSE per-sample sums (host-pointer calls)
     // batched CPMG: descriptors, tile list, reduction rows, queue head
     DevBuf batch_inst, batch_tiles, batch_rows, batch_counter;
     DevBuf flag;  // uint32 error word
@@ -731,6 +732,12 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
     return RWNMR_OK;
 }
 
+rwnmr_status pfgse_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,
+                          const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc, uint32_t n_delta,
+                          const double *kx, const double *ky, const double *kz, uint32_t n_k, int32_t k_per_delta,
+                          double voxel, int32_t bc, uint32_t shift, uint32_t n_sets, const double *set_delta,
+                          double *M, double *msd, uint32_t n_samples, double *M_samples);
+
 }  // namespace
 
 extern "C" {
@@ -787,7 +794,7 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
     for (DevBuf *b : {&c->x, &c->y, &c->z, &c->seed, &c->energy, &c->hits, &c->penalty, &c->x0, &c->y0, &c->z0,
                       &c->partials, &c->sums, &c->label_delta, &c->label_hits, &c->steps_inc, &c->kvec,
                       &c->set_delta, &c->flag, &c->batch_inst, &c->batch_tiles, &c->batch_rows,
-                      &c->batch_counter, &c->lkeys, &c->lidx, &c->lsort_tmp, &c->field, &c->p_alive, &c->phase})
+                      &c->batch_counter, &c->lkeys, &c->lidx, &c->lsort_tmp, &c->field, &c->p_alive, &c->phase, &c->msamples})
         b->release();
     for (int i = 0; i < 8; ++i) {
         c->sortA[i].release();
@@ -1200,6 +1207,33 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
                              double voxel, int32_t bc, uint32_t shift, uint32_t n_sets, const double *set_delta,
                              double *M, double *msd)
 {
+    return pfgse_common(c, img, w, x0, y0, z0, steps_inc, n_delta, kx, ky, kz, n_k, k_per_delta, voxel, bc, shift,
+                        n_sets, set_delta, M, msd, 0, nullptr);
+}
+
+rwnmr_status rwnmr_pfgse_run_sampled(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,
+                                     const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc,
+                                     uint32_t n_delta, const double *kx, const double *ky, const double *kz,
+                                     uint32_t n_k, int32_t k_per_delta, double voxel, int32_t bc, uint32_t n_sets,
+                                     const double *set_delta, uint32_t n_samples, double *M_samples, double *M,
+                                     double *msd)
+{
+    if (n_samples == 0 || !M_samples)
+        return fail(c, RWNMR_ERR_INVALID, "pfgse_run_sampled: need n_samples > 0 and M_samples");
+    return pfgse_common(c, img, w, x0, y0, z0, steps_inc, n_delta, kx, ky, kz, n_k, k_per_delta, voxel, bc, 0,
+                        n_sets, set_delta, M, msd, n_samples, M_samples);
+}
+
+}  // extern "C"
+
+namespace {
+
+rwnmr_status pfgse_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,
+                          const int32_t *y0, const int32_t *z0, const uint32_t *steps_inc, uint32_t n_delta,
+                          const double *kx, const double *ky, const double *kz, uint32_t n_k, int32_t k_per_delta,
+                          double voxel, int32_t bc, uint32_t shift, uint32_t n_sets, const double *set_delta,
+                          double *M, double *msd, uint32_t n_samples, double *M_samples)
+{
     rwnmr_status s;
     if (!c)
         return RWNMR_ERR_INVALID;
@@ -1217,7 +1251,17 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
     if (w->n == 0)
         return fail(c, RWNMR_ERR_INVALID, "pfgse_run: no walkers");
     const uint32_t threads = rwnmr::kPfgseThreads;
-    const uint32_t blocks = (uint32_t)((w->n + threads - 1) / threads);
+    PfgseArgs p;
+    memset(&p, 0, sizeof(p));
+    if (n_samples) {
+        // NMR_PFGSE::recoverDsatWithSampling: walkersPerSample = N / samples
+        p.group_walkers = w->n / n_samples;
+        if (p.group_walkers == 0)
+            return fail(c, RWNMR_ERR_INVALID, "pfgse sampling: fewer walkers than samples");
+        p.blocks_per_group = (uint32_t)((p.group_walkers + threads - 1) / threads);
+        p.full_groups = n_samples;
+    }
+    const uint32_t blocks = rwnmr::pfgse_grid_blocks(p, w->n);
     const uint64_t mrows = (uint64_t)(n_sets ? n_sets : 1) * n_delta * n_k;
     const uint64_t rows = mrows + (msd ? 3ull * n_delta : 0ull);
     if (rows * blocks * 8ull > (8ull << 30))
@@ -1227,8 +1271,6 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
         return s;
     if ((s = stage_in(c, w, v)) != RWNMR_OK)
         return s;
-    PfgseArgs p;
-    memset(&p, 0, sizeof(p));
     p.n_delta = n_delta;
     p.n_k = n_k;
     p.k_delta_stride = k_per_delta ? n_k : 0u;
@@ -1302,7 +1344,28 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
     RW_HIP(c, rwnmr::launch_reduce_rows(p.partials, mrows, blocks, Mdev, c->stream));
     if (msd)
         RW_HIP(c, rwnmr::launch_reduce_rows(p.partials + mrows * blocks, 3ull * n_delta, blocks, msd_dev, c->stream));
+    double *Ms_dev = M_samples;
+    if (n_samples) {
+        // per-sample sums: the sample's own block columns, fixed order
+        if (!w->on_device) {
+            RW_HIP(c, c->msamples.reserve(8ull * mrows * n_samples + 8));
+            Ms_dev = c->msamples.as<double>();
+        }
+        std::vector<rwnmr::ReduceRow> rows_list;
+        rows_list.reserve((size_t)mrows * n_samples);
+        for (uint32_t smp = 0; smp < n_samples; ++smp)
+            for (uint64_t r = 0; r < mrows; ++r)
+                rows_list.push_back(rwnmr::ReduceRow{p.partials + r * blocks + (uint64_t)smp * p.blocks_per_group,
+                                                     p.blocks_per_group, Ms_dev + (uint64_t)smp * mrows + r});
+        RW_HIP(c, hipStreamSynchronize(c->stream));   // the row table buffer may still be read by a prior launch
+        RW_HIP(c, c->batch_rows.reserve(sizeof(rwnmr::ReduceRow) * rows_list.size()));
+        RW_HIP(c, hipMemcpy(c->batch_rows.p, rows_list.data(), sizeof(rwnmr::ReduceRow) * rows_list.size(),
+                            hipMemcpyHostToDevice));
+        RW_HIP(c, rwnmr::launch_reduce_segments(c->batch_rows.as<rwnmr::ReduceRow>(), rows_list.size(), c->stream));
+    }
     if (!w->on_device) {
+        if (n_samples)
+            RW_HIP(c, hipMemcpyAsync(M_samples, Ms_dev, 8ull * mrows * n_samples, hipMemcpyDeviceToHost, c->stream));
         RW_HIP(c, hipMemcpyAsync(M, Mdev, 8ull * mrows, hipMemcpyDeviceToHost, c->stream));
         if (msd)
             RW_HIP(c, hipMemcpyAsync(msd, msd_dev, 24ull * n_delta, hipMemcpyDeviceToHost, c->stream));
@@ -1313,7 +1376,8 @@ rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers
     return RWNMR_OK;
 }
 
-}  // extern "C"
+}  // namespace
+
 
 extern "C" uint64_t rwnmr_ckpt_bytes(uint64_t n, uint32_t n_segments, uint32_t n_labels)
 {

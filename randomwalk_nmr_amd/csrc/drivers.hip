@@ -171,6 +171,71 @@ int32_t rwnmr_stejskal_tanner(const double *rhs, const double *lhs, uint32_t n, 
     return RWNMR_OK;
 }
 
+rwnmr_status rwnmr_pfgse_sample_stats(const double *M_samples, uint32_t n_samples, uint32_t n_k, const double *rhs,
+                                      uint32_t n_points, double *mean_M, double *std_M, double *mean_lhs,
+                                      double *std_lhs, double *D_samples, double *D_mean, double *D_std,
+                                      double *D_err_mean)
+{
+    // recoverDsatWithSampling (NMR_pfgse.cpp:876-1060) after its optional bulk
+    // and noise terms: per-sample normalisation by M(k=0), LHS, mean and
+    // population deviation over samples, one no-intercept fit per sample
+    if (!M_samples || !rhs || n_samples == 0 || n_k == 0 || n_points < 2 || n_points > n_k)
+        return RWNMR_ERR_INVALID;
+    const uint32_t S = n_samples;
+    std::vector<double> lhs((size_t)S * n_k), col(S), Ds(S), Derr(S);
+    auto mean = [](const double *v, uint32_t n) {
+        double sum = 0;
+        for (uint32_t i = 0; i < n; ++i)
+            sum += v[i];
+        return sum / (double)n;
+    };
+    auto stdev = [](const double *v, uint32_t n, double m) {
+        double sum = 0.0;
+        for (uint32_t i = 0; i < n; ++i)
+            sum += (v[i] - m) * (v[i] - m);
+        return sqrt(sum / (double)n);
+    };
+    for (uint32_t s = 0; s < S; ++s) {
+        const double *Ms = M_samples + (size_t)s * n_k;
+        const double n0 = Ms[0] / Ms[0];
+        for (uint32_t k = 0; k < n_k; ++k) {
+            const double ratio = (Ms[k] / Ms[0]) / n0;
+            lhs[(size_t)s * n_k + k] = ratio > 1.0e-06 ? log(ratio) : log(1.0e-06);
+        }
+    }
+    for (uint32_t k = 0; k < n_k; ++k) {
+        for (uint32_t s = 0; s < S; ++s)
+            col[s] = M_samples[(size_t)s * n_k + k];
+        const double m = mean(col.data(), S);
+        if (mean_M)
+            mean_M[k] = m;
+        if (std_M)
+            std_M[k] = stdev(col.data(), S, m);
+        for (uint32_t s = 0; s < S; ++s)
+            col[s] = lhs[(size_t)s * n_k + k];
+        const double ml = mean(col.data(), S);
+        if (mean_lhs)
+            mean_lhs[k] = ml;
+        if (std_lhs)
+            std_lhs[k] = stdev(col.data(), S, ml);
+    }
+    for (uint32_t s = 0; s < S; ++s) {
+        rwnmr_status st = rwnmr_stejskal_tanner(rhs, lhs.data() + (size_t)s * n_k, n_points, &Ds[s], &Derr[s]);
+        if (st != RWNMR_OK)
+            return st;
+    }
+    const double dm = mean(Ds.data(), S);
+    if (D_samples)
+        std::copy(Ds.begin(), Ds.end(), D_samples);
+    if (D_mean)
+        *D_mean = dm;
+    if (D_std)
+        *D_std = stdev(Ds.data(), S, dm);
+    if (D_err_mean)
+        *D_err_mean = mean(Derr.data(), S);
+    return RWNMR_OK;
+}
+
 uint32_t rwnmr_pfgse_threshold_lhs(const double *lhs, uint32_t n, double value, uint32_t window)
 {
     // setThresholdFromLHSValue (NMR_pfgse.cpp:399-419): the first point

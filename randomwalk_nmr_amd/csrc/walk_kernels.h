@@ -81,9 +81,46 @@ struct PfgseArgs {
                                      // [n_delta][3] squared-displacement sums
     uint32_t k_rot;                  // 1: every k row is an arithmetic progression (n_k >= 2):
                                      // cos(k_j . dr) by complex rotation instead of cos()
+    // walker samples: full_groups groups of group_walkers consecutive walkers,
+    // each walked by blocks_per_group blocks of its own (so per-sample sums are
+    // sums of whole partial columns); walkers after them follow in plain
+    // blocks.  group_walkers == 0: plain blocks for every walker.
+    uint64_t group_walkers;
+    uint32_t blocks_per_group, full_groups;
 };
 
+// Walker index of thread threadIdx.x of block b under the sample grouping of
+// p (see PfgseArgs); `inside` is false for padding threads.
+__host__ __device__ inline uint64_t pfgse_walker(const PfgseArgs &p, uint64_t b, uint32_t t, uint32_t threads,
+                                                 uint64_t n, bool &inside)
+{
+    if (p.group_walkers == 0) {
+        const uint64_t id = b * threads + t;
+        inside = id < n;
+        return id;
+    }
+    const uint64_t grouped = (uint64_t)p.blocks_per_group * p.full_groups;
+    if (b < grouped) {
+        const uint64_t g = b / p.blocks_per_group, lb = b % p.blocks_per_group;
+        const uint64_t local = lb * threads + t;
+        inside = local < p.group_walkers;
+        return g * p.group_walkers + local;
+    }
+    const uint64_t id = p.group_walkers * p.full_groups + (b - grouped) * threads + t;
+    inside = id < n;
+    return id;
+}
+
 uint32_t walk_grid_blocks(uint64_t n, const LaunchShape &ls, int shift);
+
+// Blocks of a PFGSE launch of n walkers under the grouping of p.
+inline uint32_t pfgse_grid_blocks(const PfgseArgs &p, uint64_t n)
+{
+    if (p.group_walkers == 0)
+        return (uint32_t)((n + kPfgseThreads - 1) / kPfgseThreads);
+    const uint64_t rest = n - p.group_walkers * p.full_groups;
+    return (uint32_t)((uint64_t)p.blocks_per_group * p.full_groups + (rest + kPfgseThreads - 1) / kPfgseThreads);
+}
 hipError_t launch_pfgse(const WalkArgs &a, const PfgseArgs &p, int bc, uint32_t threads, hipStream_t st);
 hipError_t launch_walk(const WalkArgs &a, int bc, int emode, const LaunchShape &ls, hipStream_t st);
 hipError_t launch_reduce_rows(const double *partials, uint64_t rows, uint32_t cols, double *out,

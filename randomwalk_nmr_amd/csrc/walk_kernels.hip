@@ -138,10 +138,11 @@ template <int BC, bool ROT>
 __global__ __launch_bounds__(kPfgseThreads) void pfgse_kernel(const WalkArgs a, const PfgseArgs p)
 {
     __shared__ double lds[kMaxSets * kPfgseKChunk][kPfgseWaves];
-    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool inside;
+    const uint64_t id = pfgse_walker(p, blockIdx.x, threadIdx.x, blockDim.x, a.n, inside);
     const Geom g = make_geom(a);
     Walker w;
-    w.active = id < a.n;
+    w.active = inside;
     const uint64_t j = w.active ? id : 0;
     w.x = w.active ? a.x[j] : 0;
     w.y = w.active ? a.y[j] : 0;
@@ -210,7 +211,7 @@ hipError_t launch_pfgse(const WalkArgs &a, const PfgseArgs &p, int bc, uint32_t 
         return hipErrorInvalidValue;
     if (a.zwords)
         return launch_zpfgse(a, p, st);
-    const uint32_t blocks = (uint32_t)((a.n + threads - 1) / threads);
+    const uint32_t blocks = pfgse_grid_blocks(p, a.n);
 #define RW_PF(B)                                                                                             \
     if (bc == B) {                                                                                           \
         if (p.k_rot)                                                                                         \

@@ -41,8 +41,9 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     __shared__ uint2 fix[6];
     zbuild_table(a.zl, tab, fix);
     __syncthreads();
-    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = id < a.n;
+    bool inside;
+    const uint64_t id = pfgse_walker(p, blockIdx.x, threadIdx.x, blockDim.x, a.n, inside);
+    const bool active = inside;
     const uint64_t j = active ? id : 0;
     int32_t X = active ? a.x[j] : 0, Y = active ? a.y[j] : 0, Z = active ? a.z[j] : 0;
     uint64_t s = active ? a.seed[j] : 1ull;
@@ -121,8 +122,7 @@ hipError_t launch_zpfgse(const WalkArgs &a, const PfgseArgs &p, hipStream_t st)
         return hipSuccess;
     if (a.zwords == nullptr || a.shift != 0)
         return hipErrorInvalidValue;
-    const uint32_t blocks = (uint32_t)((a.n + kPfgseThreads - 1) / kPfgseThreads);
-    const dim3 g(blocks), t(kPfgseThreads);
+    const dim3 g(pfgse_grid_blocks(p, a.n)), t(kPfgseThreads);
     if (a.zl.wrapfix) {
         if (p.k_rot)
             hipLaunchKernelGGL((zpfgse_kernel<true, true>), g, t, 0, st, a, p);

@@ -290,6 +290,40 @@ def test_histogram_method_matches_oracle(ctx, bc, log_scale, device):
 
 
 @pytest.mark.parametrize("bc", ["noflux", "mirror"])
+@pytest.mark.parametrize("n,n_samples", [(3000, 10), (2999, 7), (700, 3)])
+@pytest.mark.parametrize("unfolded", [2, 0])
+def test_pfgse_sampled_matches_oracle_per_sample(ctx, bc, n, n_samples, unfolded):
+    """Walker samples of N // n_samples consecutive walkers (the remainder
+    walks in no sample): each sample's M(k, Delta) == the oracle on that
+    sample alone (1e-11 M0); all-walker M and every walker's state exact."""
+    print(f"[test] pfgse sampled bc={bc} n={n} samples={n_samples} unfolded={unfolded}", flush=True)
+    solid, seeds, x, y, z, _ = _case(28, 24, 20, 4, 0.35, n, 81)
+    steps_inc = np.array([3, 9, 27], np.int64)
+    kx = np.stack([np.linspace(0.0, 1.2 / np.sqrt(c), 10) for c in np.cumsum(steps_inc)])
+    zero = np.zeros_like(kx)
+    sets = np.array([1.0, 0.98])
+    ctx.set_unfolded(unfolded)
+    try:
+        img = R.Image.from_solid(ctx, solid)
+        g = R.pfgse_run_sampled(ctx, img, x, y, z, seeds, n_samples=n_samples, steps_inc=steps_inc, kx=kx, ky=zero,
+                                kz=zero, voxel=1.0, bc=bc, set_delta=sets)
+    finally:
+        ctx.set_unfolded(2)
+    blocks = O.bitblock_pack(solid)
+    full = O.pfgse(blocks, (28, 24, 20), x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=zero, kz=zero, voxel=1.0,
+                   bc=bc, set_delta=sets)
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(g[k], full[k]), k
+    assert np.all(np.abs(g["M"] - full["M"]) <= 1e-11 * np.abs(full["M"][..., :1]))
+    wps = n // n_samples
+    for smp in range(n_samples):
+        sl = slice(smp * wps, (smp + 1) * wps)
+        o = O.pfgse(blocks, (28, 24, 20), x[sl], y[sl], z[sl], seeds[sl], steps_inc=steps_inc, kx=kx, ky=zero,
+                    kz=zero, voxel=1.0, bc=bc, set_delta=sets)
+        assert np.all(np.abs(g["M_samples"][smp] - o["M"]) <= 1e-11 * np.abs(o["M"][..., :1])), smp
+
+
+@pytest.mark.parametrize("bc", ["noflux", "mirror"])
 def test_pfgse_matches_oracle(ctx, bc):
     print(f"[test] pfgse bc={bc}", flush=True)
     solid, seeds, x, y, z, pen = _case(32, 32, 32, 5, 0.35, 1500, 21)

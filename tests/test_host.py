@@ -263,3 +263,32 @@ def test_histogram_library_matches_oracle(log_scale, size):
     segs = R.histogram_list(23, 4)
     amps = np.stack([a, a[::-1], a, a[::-1]])
     assert np.allclose(R.histogram_decay(amps, segs, p), OH.decay(amps, segs, op), rtol=1e-14)
+
+
+# ------------------------------------------------- PFGSE sample statistics
+
+from oracle import pfgse_stats as OPS  # noqa: E402
+
+
+def test_pfgse_sample_stats_oracle_pins():
+    k = np.linspace(0, 2.0, 12)
+    rhs = -(k ** 2) * 3.0
+    M = 1e5 * np.exp(0.7 * rhs)                    # exact Gaussian attenuation, D = 0.7
+    st = OPS.sample_stats(np.stack([M] * 5), rhs, 8)
+    assert np.allclose(st["D_samples"], 0.7, rtol=1e-14) and st["D_std"] == 0.0
+    assert np.all(st["std_M"] == 0.0) and abs(st["D_err_mean"]) < 1e-12
+
+
+def test_pfgse_sample_stats_library_matches_oracle():
+    rng = np.random.default_rng(2)
+    k = np.linspace(0, 1.5, 16)
+    rhs = -(k ** 2) * 2.0
+    M = 1e4 * np.exp(0.9 * rhs)[None, :] * (1 + 0.01 * rng.standard_normal((7, 16)))
+    M[:, 0] = 1e4 * (1 + 0.01 * rng.standard_normal(7))
+    M[3, 12:] = 1e-9                                # below the 1e-6 LHS floor
+    got = R.pfgse_sample_stats(M, rhs, 10)
+    want = OPS.sample_stats(M, rhs, 10)
+    for key in ("mean_M", "std_M", "mean_lhs", "std_lhs", "D_samples"):
+        assert np.allclose(got[key], want[key], rtol=1e-12, atol=1e-12), key
+    for key in ("D_mean", "D_std", "D_err_mean"):
+        assert abs(got[key] - want[key]) <= 1e-12 * abs(want[key]) + 1e-15, key
