@@ -105,6 +105,35 @@ rwnmr_status rwnmr_pfgse_sample_stats(const double *M_samples, uint32_t n_sample
 /* Points kept by the LHS-value threshold (setThresholdFromLHSValue, NMR_pfgse.cpp:399-419). */
 uint32_t rwnmr_pfgse_threshold_lhs(const double *lhs, uint32_t n, double value, uint32_t window);
 
+/* ---- multitau (one CPMG per echo time) ---- */
+
+/* multitau.config TAU_MIN / TAU_MAX / TAU_POINTS / TAU_SCALE / TAU_VALUES /
+ * COMPLETE_DECAY. */
+typedef struct {
+    double tau_min, tau_max;   /* ms */
+    uint32_t tau_points;
+    int32_t tau_scale;         /* 0 linear, 1 log, 2 manual (tau_values) */
+    const double *tau_values;  /* manual: [tau_points] ms */
+    int32_t complete_decay;    /* 1: each CPMG runs the cpmg observation time */
+} rwnmr_multitau_params;
+
+/* Echo-time sequence (NMR_multitau::setTauSequence,
+ * B/NMR_Simulation/NMR_multitau.cpp:73-110): tau from linspace / logspace
+ * (start + step i, base 10) / the manual list; steps = ceil(tau / dt), made
+ * even, and forced strictly increasing (else previous + 2); echo time =
+ * steps dt.  steps / echo_times hold tau_points values. */
+rwnmr_status rwnmr_multitau_sequence(const rwnmr_multitau_params *p, double time_interval, uint32_t *steps,
+                                     double *echo_times);
+/* NMR_multitau::run (NMR_multitau.cpp:119-204): for every echo time one
+ * CPMG (rwnmr_cpmg_simulate with steps_per_echo = that time's steps and the
+ * exposure = that echo time, or the cpmg observation time with
+ * complete_decay), each from the initial walker state w0 (left unchanged);
+ * signal[i] = the normalised first-echo amplitude of run i
+ * (multitau_decay.csv).  echo_times / signal hold tau_points values. */
+rwnmr_status rwnmr_multitau_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, const rwnmr_walkers *w0,
+                                     const rwnmr_multitau_params *mp, const rwnmr_cpmg_params *cp,
+                                     double *echo_times, double *signal);
+
 /* ---- collision-histogram ("fast") CPMG method ---- */
 
 /* Bins of a histogram of `size`: size (linear) or size + 1 (log scale; bin 0

@@ -13,4 +13,7 @@ from .core import (BC, Context, Image, bc_code, bitblock_dims, collision_histogr
 from .images import (UctImage, image_stats, internal_field_uniform, read_internal_field, read_png,  # noqa: F401
                      write_image_info, write_walkers_csv)
 
+from .config import ConfigFile  # noqa: F401
+from .drivers import cpmg_simulate, multitau_sequence, multitau_simulate, pfgse_simulate  # noqa: F401
+
 __version__ = "0.1.0"

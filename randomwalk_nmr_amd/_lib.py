@@ -63,6 +63,12 @@ class PfgseParams(ct.Structure):
                 ("threshold_window", U32)]
 
 
+class MultitauParams(ct.Structure):
+    """rwnmr_multitau_params (include/rwnmr/rwnmr_drivers.h)."""
+    _fields_ = [("tau_min", D), ("tau_max", D), ("tau_points", U32), ("tau_scale", I32), ("tau_values", P),
+                ("complete_decay", I32)]
+
+
 class PfgseResult(ct.Structure):
     """rwnmr_pfgse_result (include/rwnmr/rwnmr_drivers.h)."""
     _fields_ = [("Mkt", P), ("rhs", P), ("lhs", P), ("D_sat", P), ("D_sat_err", P), ("points", P), ("msd", P),
@@ -108,6 +114,19 @@ SIGNATURES = {
                                       U32, P, P, P]),
     "rwnmr_pfgse_sample_stats": (I32, [P, U32, U32, P, U32, P, P, P, P, P, ct.POINTER(D), ct.POINTER(D),
                                        ct.POINTER(D)]),
+    "rwnmr_multitau_sequence": (I32, [ct.POINTER(MultitauParams), D, P, P]),
+    "rwnmr_multitau_simulate": (I32, [P, P, ct.POINTER(Walkers), ct.POINTER(MultitauParams),
+                                      ct.POINTER(CpmgParams), P, P]),
+    "rwnmr_config_create": (I32, [ct.POINTER(P)]),
+    "rwnmr_config_destroy": (None, [P]),
+    "rwnmr_config_read": (I32, [P, ct.c_char_p]),
+    "rwnmr_config_set": (I32, [P, ct.c_char_p, ct.c_char_p]),
+    "rwnmr_config_has": (I32, [P, ct.c_char_p]),
+    "rwnmr_config_string": (I32, [P, ct.c_char_p, ct.c_char_p, U32]),
+    "rwnmr_config_double": (I32, [P, ct.c_char_p, ct.POINTER(D)]),
+    "rwnmr_config_int": (I32, [P, ct.c_char_p, ct.POINTER(I64)]),
+    "rwnmr_config_bool": (I32, [P, ct.c_char_p, ct.POINTER(I32)]),
+    "rwnmr_config_list": (I32, [P, ct.c_char_p, I32, P, U32, ct.POINTER(U32)]),
     "rwnmr_histogram_bins": (I32, [I32, I32]),
     "rwnmr_histogram_list": (I32, [U32, I32, ct.POINTER(I32), P, P]),
     "rwnmr_collision_histogram": (I32, [P, U64, U64, I32, I32, P, P]),

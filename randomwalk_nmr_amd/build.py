@@ -19,7 +19,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["walk_kernels.hip", "zwalk_kernels.hip", "batch_kernels.hip", "backward_kernels.hip",
                "locality_kernels.hip", "capi.hip", "drivers.hip"]
-CXX_SOURCES = ["laplace.cpp", "host_sim.cpp", "image_io.cpp", "histogram.cpp"]
+CXX_SOURCES = ["laplace.cpp", "host_sim.cpp", "image_io.cpp", "histogram.cpp", "config.cpp"]
 
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
              # packed FP32 instructions are kept out of every kernel

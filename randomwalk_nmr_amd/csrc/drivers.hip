@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/rwnmr/rwnmr.h"
@@ -417,6 +418,107 @@ rwnmr_status rwnmr_cpmg_histogram_simulate(rwnmr_ctx *ctx, const rwnmr_image *im
     if (n_used)
         *n_used = nh;
     return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_multitau_sequence(const rwnmr_multitau_params *p, double time_interval, uint32_t *steps,
+                                     double *echo_times)
+{
+    if (!p || !steps || !echo_times || p->tau_points == 0 || time_interval <= 0.0 ||
+        (p->tau_scale == 2 && !p->tau_values) || (p->tau_scale != 2 && p->tau_points < 2))
+        return RWNMR_ERR_INVALID;
+    const uint32_t n = p->tau_points;
+    std::vector<double> tau(n);
+    if (p->tau_scale == 2) {
+        std::copy(p->tau_values, p->tau_values + n, tau.begin());
+    } else if (p->tau_scale == 1) {
+        const double a = log10(p->tau_min), b = log10(p->tau_max);
+        const double step = (b - a) / ((double)n - 1.0);
+        for (uint32_t i = 0; i < n; ++i)
+            tau[i] = pow(10.0, a + step * i);
+    } else {
+        const double step = (p->tau_max - p->tau_min) / ((double)n - 1.0);
+        for (uint32_t i = 0; i < n; ++i)
+            tau[i] = p->tau_min + step * i;
+    }
+    uint32_t min_steps = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        int s = (int)std::ceil(tau[i] / time_interval);
+        if (s % 2 != 0)
+            s++;
+        if ((uint32_t)s <= min_steps)
+            s = (int)min_steps + 2;
+        if (s <= 0)
+            return RWNMR_ERR_INVALID;
+        steps[i] = (uint32_t)s;
+        min_steps = (uint32_t)s;
+        echo_times[i] = s * time_interval;
+    }
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_multitau_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, const rwnmr_walkers *w0,
+                                     const rwnmr_multitau_params *mp, const rwnmr_cpmg_params *cp,
+                                     double *echo_times, double *signal)
+{
+    if (!ctx || !img || !w0 || !mp || !cp || !echo_times || !signal || w0->n == 0)
+        return RWNMR_ERR_INVALID;
+    const uint32_t n_tau = mp->tau_points;
+    std::vector<uint32_t> steps(n_tau);
+    const double dt = rwnmr_time_step(cp->voxel_size, cp->D0);
+    rwnmr_status st = rwnmr_multitau_sequence(mp, dt, steps.data(), echo_times);
+    if (st != RWNMR_OK)
+        return st;
+    const uint64_t n = w0->n;
+    // every CPMG starts from the initial walker state (NMR_cpmg resets the walkers)
+    std::vector<int32_t> hx, hy, hz;
+    std::vector<uint64_t> hs;
+    int32_t *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    uint64_t *ds = nullptr;
+    if (w0->on_device) {
+        if (hipMalloc((void **)&dx, 4 * n) != hipSuccess || hipMalloc((void **)&dy, 4 * n) != hipSuccess ||
+            hipMalloc((void **)&dz, 4 * n) != hipSuccess || hipMalloc((void **)&ds, 8 * n) != hipSuccess) {
+            (void)hipFree(dx), (void)hipFree(dy), (void)hipFree(dz), (void)hipFree(ds);
+            return RWNMR_ERR_ALLOC;
+        }
+    }
+    for (uint32_t i = 0; i < n_tau && st == RWNMR_OK; ++i) {
+        rwnmr_cpmg_params p = *cp;
+        p.steps_per_echo = steps[i];
+        p.observation_time = mp->complete_decay ? cp->observation_time : echo_times[i];
+        rwnmr_walkers w = *w0;
+        w.energy = nullptr;
+        w.hits = nullptr;
+        if (w0->on_device) {
+            if (hipMemcpy(dx, w0->x, 4 * n, hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipMemcpy(dy, w0->y, 4 * n, hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipMemcpy(dz, w0->z, 4 * n, hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipMemcpy(ds, w0->seed, 8 * n, hipMemcpyDeviceToDevice) != hipSuccess) {
+                st = RWNMR_ERR_HIP;
+                break;
+            }
+            w.x = dx, w.y = dy, w.z = dz, w.seed = ds;
+        } else {
+            hx.assign(w0->x, w0->x + n);
+            hy.assign(w0->y, w0->y + n);
+            hz.assign(w0->z, w0->z + n);
+            hs.assign(w0->seed, w0->seed + n);
+            w.x = hx.data(), w.y = hy.data(), w.z = hz.data(), w.seed = hs.data();
+        }
+        uint32_t echoes = 0;
+        uint64_t total = 0;
+        if ((st = rwnmr_time_framework(p.observation_time, p.voxel_size, p.D0, p.steps_per_echo, &echoes, &total)) !=
+            RWNMR_OK)
+            break;
+        std::vector<double> times(echoes + 1), sig(echoes + 1);
+        uint32_t npts = 0;
+        st = rwnmr_cpmg_simulate(ctx, img, &w, &p, echoes + 1, times.data(), sig.data(), &npts);
+        if (st == RWNMR_OK)
+            signal[i] = npts > 1 ? sig[1] : sig[0];
+    }
+    if (w0->on_device) {
+        (void)hipFree(dx), (void)hipFree(dy), (void)hipFree(dz), (void)hipFree(ds);
+    }
+    return st;
 }
 
 }  // extern "C"

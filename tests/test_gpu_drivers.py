@@ -1,0 +1,117 @@
+"""Sequence drivers on the GPU (rwnmr_cpmg_simulate, rwnmr_pfgse_simulate,
+rwnmr_multitau_simulate) against the oracle walk plus the host arithmetic
+restated here: time framework (pinned in test_host), accumulated signal
+times, bulk factor, normalisation, RHS / LHS, the Stejskal-Tanner slope,
+D from the mean squared displacement, the multitau echo-time sequence."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import randomwalk_nmr_amd as R
+from oracle import histogram as OH
+
+pytestmark = pytest.mark.gpu
+
+VOX, D0, T2B = 1.0, 2.5, 2800.0
+
+
+def _case(seed, n=2500, side=(26, 22, 24)):
+    W, H, D = side
+    solid, _ = R.sphere_pack(W, H, D, 3.5, 3.5, 0.35, seed)
+    seeds, x, y, z = R.seed_and_place(solid, seed + 3, n)
+    return solid, seeds, x, y, z
+
+
+def _times(echoes, spe):
+    dt = R.time_step(VOX, D0)
+    t, out = 0.0, [0.0]
+    for _ in range(echoes):
+        t += spe * dt
+        out.append(t)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("bc", ["noflux", "mirror"])
+@pytest.mark.parametrize("sigmoid", [False, True])
+def test_cpmg_simulate_matches_oracle(ctx, bc, sigmoid):
+    print(f"[test] cpmg_simulate bc={bc} sigmoid={sigmoid}", flush=True)
+    solid, seeds, x, y, z = _case(5)
+    img = R.Image.from_solid(ctx, solid)
+    T, spe, rho = 30.0, 3, 18.0
+    sig = [5.0, 10.0, 0.3, 12.0, 2.0, 0.05, 0.5, 9.0] if sigmoid else None
+    g = R.cpmg_simulate(ctx, img, x, y, z, seeds, observation_time=T, steps_per_echo=spe, voxel_size=VOX, D0=D0,
+                        bulk_time=T2B, apply_bulk=True, rho=rho, sigmoid=sig, bc=bc)
+    echoes, steps = R.time_framework(T, VOX, D0, spe)
+    blocks = O.bitblock_pack(solid)
+    if sigmoid:
+        m = O.walk(blocks, solid.shape[::-1], x, y, z, seeds, echoes=1, steps_per_echo=steps, bc=bc,
+                   echo_sums=False)
+        pen = np.array([R.penalty(OH.sigmoid_rho(sig, h / steps), VOX, D0) for h in m["hits"]])
+    else:
+        pen = np.full(len(x), R.penalty(rho, VOX, D0))
+    o = O.walk(blocks, solid.shape[::-1], x, y, z, seeds, echoes=echoes, steps_per_echo=spe, bc=bc, penalty=pen)
+    t = _times(echoes, spe)
+    s = np.exp(-1.0 / T2B * t) * o["echo_sums"]
+    s = (1.0 / s[0]) * s
+    assert np.array_equal(g["times"], t)
+    assert np.allclose(g["signal"], s, rtol=1e-12, atol=0)
+    assert np.array_equal(g["x"], o["x"]) and np.array_equal(g["seed"], o["seed"])
+
+
+@pytest.mark.parametrize("threshold", [0, 6])
+def test_pfgse_simulate_matches_oracle(ctx, threshold):
+    print(f"[test] pfgse_simulate threshold={threshold}", flush=True)
+    solid, seeds, x, y, z = _case(9, n=3000)
+    img = R.Image.from_solid(ctx, solid)
+    deltas = np.array([0.5, 1.5, 4.0])
+    K = 12
+    kx = np.stack([np.linspace(0.0, 2.5 / math.sqrt(D0 * d), K) for d in deltas])
+    zero = np.zeros_like(kx)
+    pw = 0.1
+    g = R.pfgse_simulate(ctx, img, x, y, z, seeds, exposure_times=deltas, kx=kx, ky=zero, kz=zero, voxel_size=VOX,
+                         D0=D0, bulk_time=T2B, apply_bulk=True, bc="mirror", pulse_width=pw, rho=8.0,
+                         threshold_samples=threshold)
+    steps = np.array([R.time_framework(d, VOX, D0, 1)[1] for d in deltas], np.int64)
+    inc = np.diff(np.concatenate([[0], steps]))
+    o = O.pfgse(O.bitblock_pack(solid), solid.shape[::-1], x, y, z, seeds, steps_inc=inc, kx=kx, ky=zero, kz=zero,
+                voxel=VOX, bc="mirror", penalty=R.penalty(8.0, VOX, D0))
+    assert np.array_equal(g["steps"], steps)
+    n_used = threshold if threshold else K
+    for i, d in enumerate(deltas):
+        M = o["M"][0, i] * math.exp(-1.0 / T2B * d)
+        assert np.allclose(g["Mkt"][0, i], M, rtol=1e-11, atol=1e-11 * abs(M[0]))
+        rhs = -1.0 * kx[i] ** 2 * (d - pw / 3.0)
+        ratio = M / M[0]
+        lhs = np.log(np.maximum(ratio, 1e-6))
+        b = float(np.dot(rhs[:n_used], lhs[:n_used]) / np.dot(rhs[:n_used], rhs[:n_used]))
+        assert g["points"][0, i] == n_used
+        assert abs(g["D_sat"][0, i] - b) <= 1e-9 * abs(b)
+        dmsd = o["msd"][i].sum() * VOX ** 2 / len(x) / (6.0 * d)
+        assert abs(g["D_msd"][i] - dmsd) <= 1e-14 * dmsd
+
+
+@pytest.mark.parametrize("scale", ["log", "manual"])
+def test_multitau_matches_oracle(ctx, scale):
+    print(f"[test] multitau scale={scale}", flush=True)
+    solid, seeds, x, y, z = _case(13, n=2000)
+    img = R.Image.from_solid(ctx, solid)
+    tau = dict(tau_min=0.1, tau_max=2.0, tau_points=5, scale=scale)
+    if scale == "manual":
+        tau["tau_values"] = [0.2, 0.2, 0.9]
+    cpmg = dict(observation_time=50.0, steps_per_echo=1, voxel_size=VOX, D0=D0, bulk_time=T2B, apply_bulk=True,
+                rho=20.0, bc="mirror")
+    times, sig = R.multitau_simulate(ctx, img, x, y, z, seeds, tau=tau, cpmg=cpmg)
+    dt = R.time_step(VOX, D0)
+    steps, t_expect = R.multitau_sequence(dt, **tau)
+    assert np.array_equal(times, t_expect)
+    blocks = O.bitblock_pack(solid)
+    for i, s in enumerate(steps):
+        echoes, _ = R.time_framework(float(t_expect[i]), VOX, D0, int(s))
+        assert echoes == 1
+        o = O.walk(blocks, solid.shape[::-1], x, y, z, seeds, echoes=1, steps_per_echo=int(s), bc="mirror",
+                   penalty=R.penalty(20.0, VOX, D0))
+        t1 = 0.0 + int(s) * dt
+        want = (math.exp(-1.0 / T2B * t1) * o["echo_sums"][1]) * (1.0 / o["echo_sums"][0])
+        assert abs(sig[i] - want) <= 1e-12 * want, i

@@ -292,3 +292,54 @@ def test_pfgse_sample_stats_library_matches_oracle():
         assert np.allclose(got[key], want[key], rtol=1e-12, atol=1e-12), key
     for key in ("D_mean", "D_std", "D_err_mean"):
         assert abs(got[key] - want[key]) <= 1e-12 * abs(want[key]) + 1e-15, key
+
+
+# ------------------------------------------------------ config files
+
+def test_config_files_reference_grammar(tmp_path):
+    d = tmp_path / "default.config"
+    d.write_text("--- RWNMR Configuration\n"
+                 "-- 'NAME': Output folder with simulation results, saved at 'db'\n"
+                 "NAME: NMR_simulation\nD0: 2.5\nRHO: {0.0}\nWALKERS: 50000\nBC: noflux\nGPU_USAGE: true\n"
+                 "MAX_GRADIENT: {3000.0, 0.0, 0.0}\n")
+    u = tmp_path / "user.config"
+    u.write_text("NAME: rock_A\nRHO: {12.5, 3.0}\nWALKERS: 1000000\nBC: mirror\nGPU_USAGE: false\n")
+    c = R.ConfigFile(d, u)
+    assert c.string("NAME") == "rock_A" and c.int("WALKERS") == 1000000 and c.string("BC") == "mirror"
+    assert c.double("D0") == 2.5 and c.bool("GPU_USAGE") is False
+    # RHO appends across files (the defaults' value comes first)
+    assert list(c.list("RHO", append=True)) == [0.0, 12.5, 3.0] and list(c.list("RHO")) == [12.5, 3.0]
+    assert list(c.list("MAX_GRADIENT")) == [3000.0, 0.0, 0.0]
+    # the comment line's ": " made a key candidate, as in the reference's reader
+    assert "-- 'NAME'" in c and "MISSING" not in c
+    with pytest.raises(R.RwnmrError):
+        c.double("BC")
+
+
+def _py_tau_steps(taus, dt):
+    out, prev = [], 0
+    for t in taus:
+        s = math.ceil(t / dt)
+        s += s % 2
+        if s <= prev:
+            s = prev + 2
+        out.append(s)
+        prev = s
+    return out
+
+
+@pytest.mark.parametrize("scale", ["linear", "log", "manual"])
+def test_multitau_sequence(scale):
+    dt = 1.0 / 15.0
+    kw = dict(tau_min=0.05, tau_max=3.0, tau_points=9, scale=scale)
+    if scale == "manual":
+        kw["tau_values"] = [0.1, 0.1, 0.35, 2.0, 1.0]
+        taus = kw["tau_values"]
+    elif scale == "log":
+        taus = [10 ** (math.log10(0.05) + (math.log10(3.0) - math.log10(0.05)) / 8 * i) for i in range(9)]
+    else:
+        taus = [0.05 + (3.0 - 0.05) / 8 * i for i in range(9)]
+    steps, times = R.multitau_sequence(dt, **kw)
+    assert list(steps) == _py_tau_steps(taus, dt)
+    assert np.all(np.diff(steps.astype(int)) > 0) and np.all(steps % 2 == 0)
+    assert np.array_equal(times, steps * dt)
