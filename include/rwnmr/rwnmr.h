@@ -289,6 +289,11 @@ void get_used_t2_bins(double *out_used_t2_bins);
 /* Sizes of the arrays the getters above fill (the reference leaves these to
  * the caller's knowledge of its config). */
 int rwnmr_inverter_num_used_raw_bins(void);
+/* log10 range of the L-curve lambdas (cpmg.config MIN_LAMBDA / MAX_LAMBDA,
+ * NMRInverterConfig of B/Laplace/include/nmrinv_core.h; the reference's C
+ * API has no setter and keeps the default -4 .. 2).  Applies to the next
+ * initialize_inverter*. */
+void rwnmr_inverter_set_lambda_range(double min_log10, double max_log10);
 int rwnmr_inverter_num_t2_bins(void);
 int rwnmr_inverter_num_lambdas(void);
 

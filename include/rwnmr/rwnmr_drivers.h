@@ -104,6 +104,10 @@ rwnmr_status rwnmr_pfgse_sample_stats(const double *M_samples, uint32_t n_sample
                                       double *D_err_mean);
 /* Points kept by the LHS-value threshold (setThresholdFromLHSValue, NMR_pfgse.cpp:399-419). */
 uint32_t rwnmr_pfgse_threshold_lhs(const double *lhs, uint32_t n, double value, uint32_t window);
+/* The same with noise (setThresholdFromLHSWindow, NMR_pfgse.cpp:405-439):
+ * the first point where the mean of the last `window` LHS values falls
+ * below log(value); n when n < window. */
+uint32_t rwnmr_pfgse_threshold_lhs_window(const double *lhs, uint32_t n, double value, uint32_t window);
 
 /* ---- multitau (one CPMG per echo time) ---- */
 

@@ -43,6 +43,23 @@ int64_t rwnmr_count_pores(const uint8_t *solid, int32_t W, int32_t H, int32_t D)
 int32_t rwnmr_seed_and_place(const uint8_t *solid, int32_t W, int32_t H, int32_t D, uint64_t master_seed,
                              uint64_t n, uint64_t *seeds, int32_t *x, int32_t *y, int32_t *z);
 
+/* The whole walker set-up of an NMR_Simulation (RNG seeding :86-95,
+ * setWalkers :324-410, applyVoxelDivision :192-268 of
+ * B/NMR_Simulation/NMR_Simulation.cpp) from rwnmr.config WALKERS_PLACEMENT:
+ * mode 0 "random" = rwnmr_seed_and_place; 1 "point": every walker at
+ * (W/2, H/2, D/2), an error if that voxel is solid; 2 "cubic": walkers
+ * uniform over the pore voxels of [c - deviation, c + deviation) per axis,
+ * c = (dim - 1) / 2 (an error if there are none).  shift > 0
+ * (uct.config VOXEL_DIVISION) then moves every walker to
+ * pos * 2^shift + a draw in [0, 2^shift - 1] per axis.  occupancy gets
+ * walkers / pore voxels; draws (nullable) the number of 32-bit generator
+ * outputs consumed, discard included, so a caller can continue the stream
+ * (std::mt19937(seed mod 2^32), discard(draws)) as the reference's later
+ * draws (PFGSE noise) do. */
+int32_t rwnmr_setup_walkers(const uint8_t *solid, int32_t W, int32_t H, int32_t D, uint64_t master_seed, uint64_t n,
+                            int32_t mode, int32_t deviation, uint32_t shift, uint64_t *seeds, int32_t *x,
+                            int32_t *y, int32_t *z, double *occupancy, uint64_t *draws);
+
 /* dt = voxel^2 / (6 D0) in ms (setTimeInterval, NMR_Simulation.cpp:179-183). */
 double rwnmr_time_step(double voxel_size, double D0);
 

@@ -17,6 +17,8 @@ Parity status of each oracle routine (how it is pinned, see the tests):
     walk (CPMG, map, labels)       pinned (closed forms + invariants)
     pfgse                          pinned (free diffusion closed form, k=0 column)
     mt19937 / Lemire placement     pinned (C++ standard value, numpy, libstdc++)
+    setup_walkers (point / cubic,  pinned (brute-force reachable sets on tiny
+      voxel division)                images, draw counts, seed_and_place)
     laplace                        pinned (scipy.optimize.nnls, closed forms)
 """
 from __future__ import annotations
@@ -281,6 +283,90 @@ def reference_pore_cap(n_pores: int, volume: int) -> int:
     (B/NMR_Simulation/NMR_Simulation.cpp:915-928)."""
     porosity = float(n_pores) / float(volume)
     return int(porosity * float(volume))
+
+
+def setup_walkers(solid, master_seed: int, n: int, mode: str = "random", deviation: int = 0, shift: int = 0):
+    """NMR_Simulation's walker set-up, one draw at a time in Python on the
+    oracle's mt19937 (B/NMR_Simulation/NMR_Simulation.cpp: generator :86-95,
+    setWalkers :324-410, createWalkers :1139-1204, createWalkersIDList
+    :1086-1136, placeWalkersInCubicSpace :1395-1454 over the cube of
+    countPoresInCubicSpace / createPoreList(v1, v2) :728-816, :982-1069 with
+    v1 = c - deviation, v2 = c + deviation, c = (dim - 1) / 2, upper bound
+    exclusive and clipped to the image; applyVoxelDivision :192-268).
+    Returns (seeds, x, y, z, draws) or None when the reference places no
+    walker (point on a solid voxel, empty cube)."""
+    solid = _np.asarray(solid)
+    D, H, W = solid.shape
+    g = MT19937(master_seed)
+    for _ in range(4096):
+        g.next()
+    draws = 4096
+    seeds = _np.zeros(n, _np.uint64)
+    for i in range(n):
+        hi, lo = g.next(), g.next()
+        seeds[i] = (hi << 32) | lo
+        draws += 2
+    x = _np.zeros(n, _np.int64)
+    y = _np.zeros(n, _np.int64)
+    z = _np.zeros(n, _np.int64)
+
+    if mode == "random":
+        pores = pore_list(solid)
+        cap = reference_pore_cap(len(pores), solid.size)
+        picks = []
+        for i in range(n):
+            picks.append(g.lemire(cap + 1))
+        draws += _lemire_outputs(master_seed, 4096 + 2 * n, [cap + 1] * n)
+        for i, idx in enumerate(picks):
+            idx = min(idx, len(pores) - 1)
+            x[i], y[i], z[i] = pores[idx]
+    elif mode == "point":
+        cx, cy, cz = W // 2, H // 2, D // 2
+        if solid[cz, cy, cx]:
+            return None
+        x[:], y[:], z[:] = cx, cy, cz
+    else:
+        lo = [(W - 1) // 2 - deviation, (H - 1) // 2 - deviation, (D - 1) // 2 - deviation]
+        hi = [min((W - 1) // 2 + deviation, W), min((H - 1) // 2 + deviation, H), min((D - 1) // 2 + deviation, D)]
+        cube = []
+        if min(lo) >= 0:
+            for zz in range(lo[2], hi[2]):
+                for yy in range(lo[1], hi[1]):
+                    for xx in range(lo[0], hi[0]):
+                        if not solid[zz, yy, xx]:
+                            cube.append((xx, yy, zz))
+        if not cube:
+            return None
+        picks = [g.lemire(len(cube) + 1) for _ in range(n)]
+        draws += _lemire_outputs(master_seed, 4096 + 2 * n, [len(cube) + 1] * n)
+        for i, idx in enumerate(picks):
+            x[i], y[i], z[i] = cube[min(idx, len(cube) - 1)]
+    if shift:
+        f = 1 << shift
+        start = draws
+        for i in range(n):
+            x[i] = x[i] * f + g.lemire(f)
+            y[i] = y[i] * f + g.lemire(f)
+            z[i] = z[i] * f + g.lemire(f)
+        draws += _lemire_outputs(master_seed, start, [f] * (3 * n))
+    return seeds, x.astype(_np.int32), y.astype(_np.int32), z.astype(_np.int32), draws
+
+
+def _lemire_outputs(master_seed: int, skip: int, ranges) -> int:
+    """32-bit outputs Lemire's method takes for the given ranges, starting
+    `skip` outputs into the stream: one per draw, plus one per rejection
+    (low word of output * range under (2^32 - range) mod range)."""
+    g = MT19937(master_seed)
+    for _ in range(skip):
+        g.next()
+    used = 0
+    for r in ranges:
+        thr = ((1 << 32) - r) % r
+        while True:
+            used += 1
+            if ((g.next() * r) & 0xFFFFFFFF) >= thr:
+                break
+    return used
 
 
 def seed_and_place(master_seed: int, n: int, pores: _np.ndarray, volume: int):

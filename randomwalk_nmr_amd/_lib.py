@@ -152,11 +152,28 @@ SIGNATURES = {
     "rwnmr_pfgse_wave_vector": (D, [D, D, D]),
     "rwnmr_stejskal_tanner": (I32, [P, P, U32, ct.POINTER(D), ct.POINTER(D)]),
     "rwnmr_pfgse_threshold_lhs": (U32, [P, U32, D, U32]),
+    "rwnmr_pfgse_threshold_lhs_window": (U32, [P, U32, D, U32]),
+    # output tree (include/rwnmr/rwnmr_output.h)
+    "rwnmr_make_dirs": (I32, [ct.c_char_p]),
+    "rwnmr_write_sim_info": (I32, [ct.c_char_p, ct.c_char_p, ct.c_char_p, ct.c_char_p, D, D, I32, D, U64, I32,
+                                   ct.c_char_p]),
+    "rwnmr_write_bin_image": (I32, [ct.c_char_p, P, I32, I32, I32]),
+    "rwnmr_write_cpmg_decay": (I32, [ct.c_char_p, U32, P, P, P]),
+    "rwnmr_write_two_columns": (I32, [ct.c_char_p, ct.c_char_p, U32, P, P]),
+    "rwnmr_write_histogram_list": (I32, [ct.c_char_p, I32, I32, P, P]),
+    "rwnmr_write_pfgse_parameters": (I32, [ct.c_char_p, D, D, D, D, D, D, U32, P, U32]),
+    "rwnmr_write_pfgse_gradient": (I32, [ct.c_char_p, U32, P, P, P, P, P, P]),
+    "rwnmr_write_pfgse_echoes": (I32, [ct.c_char_p, U32, P, P, P, P, P, P, P]),
+    "rwnmr_write_pfgse_msd": (I32, [ct.c_char_p, P, P, P, P]),
+    "rwnmr_write_pfgse_results_header": (I32, [ct.c_char_p]),
+    "rwnmr_append_pfgse_results": (I32, [ct.c_char_p, D, P, U32]),
     # host set-up (include/rwnmr/rwnmr_host.h)
     "rwnmr_synth_sphere_pack": (I32, [I32, I32, I32, D, D, D, U64, P, ct.POINTER(D)]),
     "rwnmr_synth_voronoi_labels": (I32, [I32, I32, I32, I32, I32, U64, P, P]),
     "rwnmr_count_pores": (I64, [P, I32, I32, I32]),
     "rwnmr_seed_and_place": (I32, [P, I32, I32, I32, U64, U64, P, P, P, P]),
+    "rwnmr_setup_walkers": (I32, [P, I32, I32, I32, U64, U64, I32, I32, U32, P, P, P, P, ct.POINTER(D),
+                                  ct.POINTER(U64)]),
     "rwnmr_time_step": (D, [D, D]),
     "rwnmr_time_framework": (I32, [D, D, D, U32, ct.POINTER(U32), ct.POINTER(U64)]),
     "rwnmr_penalty": (D, [D, D, D]),

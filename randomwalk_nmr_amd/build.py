@@ -19,7 +19,10 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["walk_kernels.hip", "zwalk_kernels.hip", "batch_kernels.hip", "backward_kernels.hip",
                "locality_kernels.hip", "capi.hip", "drivers.hip"]
-CXX_SOURCES = ["laplace.cpp", "host_sim.cpp", "image_io.cpp", "histogram.cpp", "config.cpp"]
+CXX_SOURCES = ["laplace.cpp", "host_sim.cpp", "image_io.cpp", "histogram.cpp", "config.cpp", "output.cpp"]
+# the command-line program, linked against the library
+APP_SOURCE = "rwnmr_app.cpp"
+APP = os.path.join(HERE, "rwnmr")
 
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
              # packed FP32 instructions are kept out of every kernel
@@ -62,6 +65,10 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-fopenmp", "-lz"])
     os.replace(tmp, LIB)
+    tmp = APP + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include"),
+          os.path.join(CSRC, APP_SOURCE), "-o", tmp, "-L", HERE, "-lrwnmr_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, APP)
     return LIB
 
 

@@ -226,6 +226,24 @@ def seed_and_place(solid, master_seed, n):
     return seeds, x, y, z
 
 
+PLACEMENTS = {"random": 0, "point": 1, "cubic": 2}
+
+
+def setup_walkers(solid, master_seed, n, *, placement="random", deviation=0, shift=0):
+    """The reference's whole walker set-up: seeds, placement mode
+    ("random" / "point" / "cubic") and voxel division (include/rwnmr/rwnmr_host.h).
+    Returns dict(seed, x, y, z, occupancy, draws)."""
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    D, H, W = solid.shape
+    seeds = np.zeros(n, np.uint64)
+    x, y, z = (np.zeros(n, np.int32) for _ in range(3))
+    occ, draws = ct.c_double(), ct.c_uint64()
+    check(load().rwnmr_setup_walkers(_ptr(solid), W, H, D, int(master_seed), int(n), PLACEMENTS[placement],
+                                     int(deviation), int(shift), _ptr(seeds), _ptr(x), _ptr(y), _ptr(z),
+                                     ct.byref(occ), ct.byref(draws)))
+    return dict(seed=seeds, x=x, y=y, z=z, occupancy=occ.value, draws=draws.value)
+
+
 def time_step(voxel, D0):
     return load().rwnmr_time_step(float(voxel), float(D0))
 

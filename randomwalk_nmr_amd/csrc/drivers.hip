@@ -257,6 +257,34 @@ uint32_t rwnmr_pfgse_threshold_lhs(const double *lhs, uint32_t n, double value, 
     return (uint32_t)idx;
 }
 
+uint32_t rwnmr_pfgse_threshold_lhs_window(const double *lhs, uint32_t n, double value, uint32_t window)
+{
+    // setThresholdFromLHSWindow (NMR_pfgse.cpp:405-439): a moving mean of
+    // `window` LHS values, the first point where it falls below log(value)
+    if (n < window || window == 0)
+        return n;
+    std::vector<double> win(lhs, lhs + window);
+    auto wmean = [&]() {
+        double s = 0.0;
+        for (double v : win)
+            s += v;
+        return s / (double)win.size();
+    };
+    const double lv = log(value);
+    int idx = (int)window;
+    bool greater = !(wmean() < lv);
+    while (idx < (int)n && greater) {
+        win[(uint32_t)idx % window] = lhs[idx];
+        if (wmean() < lv)
+            greater = false;
+        else
+            ++idx;
+    }
+    if (greater)
+        --idx;
+    return (uint32_t)idx;
+}
+
 rwnmr_status rwnmr_pfgse_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w,
                                   const rwnmr_pfgse_params *p, const double *exposure_times, uint32_t n_times,
                                   const double *kx, const double *ky, const double *kz, uint32_t n_k,

@@ -184,6 +184,115 @@ int32_t rwnmr_seed_and_place(const uint8_t *solid, int32_t W, int32_t H, int32_t
     return RWNMR_OK;
 }
 
+int32_t rwnmr_setup_walkers(const uint8_t *solid, int32_t W, int32_t H, int32_t D, uint64_t master_seed, uint64_t n,
+                            int32_t mode, int32_t deviation, uint32_t shift, uint64_t *seeds, int32_t *x,
+                            int32_t *y, int32_t *z, double *occupancy, uint64_t *draws)
+{
+    if (!solid || W <= 0 || H <= 0 || D <= 0 || mode < 0 || mode > 2 || shift > 8 ||
+        (n && (!seeds || !x || !y || !z)))
+        return RWNMR_ERR_INVALID;
+    const uint64_t V = (uint64_t)W * H * D;
+    // Counting generator: every 32-bit output is counted so the caller can
+    // resume the same stream (mt19937 + discard) for later draws.
+    struct Counted {
+        std::mt19937 g;
+        uint64_t used;
+        typedef uint32_t result_type;
+        uint32_t operator()()
+        {
+            ++used;
+            return (uint32_t)g();
+        }
+    } g{std::mt19937((uint32_t)master_seed), 4096};
+    g.g.discard(4096);
+    uint64_t pores_total = 0;
+    for (uint64_t v = 0; v < V; ++v)
+        pores_total += solid[v] ? 0 : 1;
+    if (occupancy)
+        *occupancy = (double)n / (double)pores_total;   // updateWalkerOccupancy, :1081-1084
+    if (mode == 0) {
+        // setWalkers(n) (:365-384): createWalkers, pore list, ID list, placement
+        int32_t st = rwnmr_seed_and_place(solid, W, H, D, master_seed, n, seeds, x, y, z);
+        if (st != RWNMR_OK)
+            return st;
+        // replay the draws to count them (seed words, then the index draws)
+        g.used = 4096 + 2 * n;
+        for (uint64_t i = 0; i < 2 * n; ++i)
+            g.g();
+        const double porosity = (double)pores_total / (double)V;
+        const uint32_t cap = (uint32_t)(porosity * (double)V);
+        for (uint64_t i = 0; i < n; ++i)
+            lemire_draw(g, cap + 1u);
+    } else {
+        // createWalkers (:1139-1204): two outputs per seed, high word first
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t hi = g();
+            const uint64_t lo = g();
+            seeds[i] = (hi << 32) | lo;
+        }
+        if (mode == 1) {
+            // "point" (:328-337, placeWalkersInSamePoint :1335-1393): every
+            // walker at the image centre, which must be a pore voxel
+            const int32_t cx = W / 2, cy = H / 2, cz = D / 2;
+            if (solid[(uint64_t)cz * W * H + (uint64_t)cy * W + cx])
+                return RWNMR_ERR_INVALID;
+            for (uint64_t i = 0; i < n; ++i) {
+                x[i] = cx;
+                y[i] = cy;
+                z[i] = cz;
+            }
+        } else {
+            // "cubic" (:338-349, countPoresInCubicSpace :728-816, createPoreList
+            // :982-1069, placeWalkersInCubicSpace :1395-1454): pores of
+            // [c - dev, c + dev) per axis (upper end clipped to the image,
+            // a negative lower end leaves the cube empty), c = (dim - 1) / 2,
+            // in z, y, x order; draws uniform over [0, count] inclusive (the
+            // index equal to the count selects the last pore here)
+            const int64_t c[3] = {(W - 1) / 2, (H - 1) / 2, (D - 1) / 2};
+            const int64_t dim[3] = {W, H, D};
+            int64_t lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = c[a] - deviation;
+                hi[a] = std::min<int64_t>(c[a] + deviation, dim[a]);
+                if (lo[a] < 0)
+                    hi[a] = lo[a];
+            }
+            std::vector<uint32_t> cube;
+            for (int64_t zz = lo[2]; zz < hi[2]; ++zz)
+                for (int64_t yy = lo[1]; yy < hi[1]; ++yy)
+                    for (int64_t xx = lo[0]; xx < hi[0]; ++xx) {
+                        const uint64_t v = (uint64_t)zz * W * H + (uint64_t)yy * W + (uint64_t)xx;
+                        if (!solid[v])
+                            cube.push_back((uint32_t)v);
+                    }
+            if (cube.empty())
+                return n ? RWNMR_ERR_INVALID : RWNMR_OK;
+            for (uint64_t i = 0; i < n; ++i) {
+                uint32_t idx = lemire_draw(g, (uint32_t)cube.size() + 1u);
+                if (idx >= cube.size())
+                    idx = (uint32_t)cube.size() - 1;
+                const uint32_t v = cube[idx];
+                x[i] = (int32_t)(v % (uint32_t)W);
+                y[i] = (int32_t)((v / (uint32_t)W) % (uint32_t)H);
+                z[i] = (int32_t)(v / ((uint32_t)W * (uint32_t)H));
+            }
+        }
+    }
+    if (shift > 0) {
+        // applyVoxelDivision (:192-268): pos * 2^shift + a draw in
+        // [0, 2^shift - 1] per axis, x then y then z, walker by walker
+        const uint32_t f = 1u << shift;
+        for (uint64_t i = 0; i < n; ++i) {
+            x[i] = (int32_t)(x[i] * f + lemire_draw(g, f));
+            y[i] = (int32_t)(y[i] * f + lemire_draw(g, f));
+            z[i] = (int32_t)(z[i] * f + lemire_draw(g, f));
+        }
+    }
+    if (draws)
+        *draws = g.used;
+    return RWNMR_OK;
+}
+
 double rwnmr_time_step(double voxel_size, double D0)
 {
     // B/NMR_Simulation/NMR_Simulation.cpp:179-183: dt = eps^2 / (6 D0)
