@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch", default="", help="threads,k (skips the autotuner)")
+    ap.add_argument("--unfolded", type=int, default=-1,
+                    help="unfolded-image kernel variant (rwnmr_ctx_set_unfolded); -1 keeps the library default")
     ap.add_argument("--workload", default="cpmg512", choices=["cpmg512", "small", "pfgse", "batch", "fit"])
     ap.add_argument("--ga-time-limit", type=float, default=150.0, help="fit workload: GA wall-time cap (s)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -94,6 +96,8 @@ def main():
     delta = R.penalty(rho, voxel, D0)
 
     ctx = R.Context(0)
+    if args.unfolded >= 0:
+        ctx.set_unfolded(args.unfolded)
     img = R.Image.from_solid(ctx, solid)
     dev = torch.device("cuda:0")
     X0, Y0, Z0 = (torch.from_numpy(a).to(dev) for a in (x0, y0, z0))
@@ -107,9 +111,10 @@ def main():
         ctx.set_launch(tpb, k)
     elif not args.no_autotune:
         t = time.time()
-        tpb, k, ms = ctx.autotune(img, X0, Y0, Z0, SEED0, steps_per_echo=spe, bc="mirror", trial_echoes=100,
+        # one locality re-sort interval: the walkers' spatial order decays over it
+        tpb, k, ms = ctx.autotune(img, X0, Y0, Z0, SEED0, steps_per_echo=spe, bc="mirror", trial_echoes=1000,
                                   delta=delta)
-        log(f"autotune: threads {tpb} x walkers/lane {k} ({ms:.2f} ms per 100 echoes) in {time.time() - t:.1f}s")
+        log(f"autotune: threads {tpb} x walkers/lane {k} ({ms:.2f} ms per 1000 echoes) in {time.time() - t:.1f}s")
     tpb, k = ctx.get_launch()
 
     def one_step():
@@ -204,7 +209,7 @@ def main():
                                                                     "walkers_per_lane": k}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "zwalk_kernel<CPMG, chunk cache> on the unfolded mirror image",
+                     "kernel": "zwalk_kernel<CPMG, 16-byte chunk cache, byte-key direction table, direction drawn a step ahead> on the unfolded mirror image",
                      "avg_kernel_ms": avg_kernel_s * 1e3, "launches": launches,
                      "algorithmic_bytes_per_launch": steps_per_launch * BYTES_PER_STEP},
         "cpu_baseline": cpu,

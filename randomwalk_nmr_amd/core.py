@@ -78,9 +78,12 @@ class Context:
         check(load().rwnmr_ctx_set_locality(self.handle, int(enable), int(resort_every), int(min_walkers)),
               self.handle)
 
-    def set_unfolded(self, mode: int = 2):
+    def set_unfolded(self, mode: int = 3):
         """Periodic / mirror walks: 0 folded kernels, 1 unfolded image (no
-        per-step fold), 2 unfolded + per-walker 16-byte chunk cache (default)."""
+        per-step fold), 2 unfolded + per-walker 16-byte chunk cache, 3 and 4
+        the chunk tested by one shift and the next direction drawn a step
+        ahead, through the 4 KiB byte-key table (3, default) or the 64 KiB
+        wide direction table (4)."""
         check(load().rwnmr_ctx_set_unfolded(self.handle, int(mode)), self.handle)
 
     def set_timing(self, enable: bool = True):

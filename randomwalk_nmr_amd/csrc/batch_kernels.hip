@@ -76,9 +76,9 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
             zbuild_table(I.zl, tab, fix);
             __syncthreads();
             if (I.zl.wrapfix)
-                zwalk_tile<1, K, true, true>(a, bt.tile, part, I.n_tiles, lds, tab, fix);
+                zwalk_tile<1, K, true, 1>(a, bt.tile, part, I.n_tiles, lds, tab, fix);
             else
-                zwalk_tile<1, K, false, true>(a, bt.tile, part, I.n_tiles, lds, tab, fix);
+                zwalk_tile<1, K, false, 1>(a, bt.tile, part, I.n_tiles, lds, tab, fix);
             continue;
         }
         switch (I.bc) {

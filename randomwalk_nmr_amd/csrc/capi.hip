@@ -105,8 +105,11 @@ struct rwnmr_ctx {
     DevBuf lkeys, lidx, lsort_tmp;
     DevBuf sortA[8], sortB[8];
     // periodic / mirror walks on the unfolded image (built per image on first use):
-    // 0 folded kernels, 1 unfolded, 2 unfolded with the per-walker chunk cache
-    int unfolded = 2;
+    // 0 folded kernels, 1 unfolded, 2 unfolded with the per-walker 128-bit
+    // chunk cache, 3 and 4 with the chunk tested by one shift and the next
+    // direction drawn a step ahead, through the 4 KiB byte-key table (3, the
+    // default) or the 64 KiB wide direction table (4)
+    int unfolded = 3;
 };
 
 // Unfolded image of one boundary condition (dilated addresses).
@@ -443,7 +446,7 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
     if (const ZImage *z = zimage_for(c, img, bc)) {
         a.zwords = z->words;
         a.zl = z->zl;
-        a.zcache = c->unfolded >= 2 ? 1u : 0u;
+        a.zcache = c->unfolded >= 2 ? (uint32_t)(c->unfolded - 1) : 0u;
         a.zlabels = zlab;
     }
 }
@@ -835,8 +838,8 @@ rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *c, int32_t enable)
 {
     if (!c)
         return RWNMR_ERR_INVALID;
-    if (enable < 0 || enable > 2)
-        return fail(c, RWNMR_ERR_INVALID, "set_unfolded: mode must be 0, 1 or 2");
+    if (enable < 0 || enable > 4)
+        return fail(c, RWNMR_ERR_INVALID, "set_unfolded: mode must be 0 .. 4");
     c->unfolded = enable;
     return RWNMR_OK;
 }
@@ -1154,7 +1157,10 @@ rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *c, const rwnmr_image *img, const rwnm
                 cleanup();
                 return s;
             }
-            if (c->kernel_ms < best_t) {
+            // more walkers per lane must win by more than 3%: short trials on a
+            // cool device favour them, sustained runs do not
+            const double margin = k > best.k ? 0.97 : 1.0;
+            if (c->kernel_ms < best_t * margin) {
                 best_t = c->kernel_ms;
                 best = LaunchShape{t, k};
             }

@@ -31,6 +31,18 @@ RW_HD uint64_t xorshift64(uint64_t s)
     return s;
 }
 
+// The state before one xorshift64 step: each y = x ^ (x << a) is undone by
+// x = y ^ (y << a) ^ (y << 2a) ^ ... (and likewise for >>), in reverse order.
+// Lets a walk that draws its next direction one step ahead return the state
+// of the steps it actually took.
+RW_HD uint64_t xorshift64_inverse(uint64_t s)
+{
+    s ^= (s << 17) ^ (s << 34) ^ (s << 51);
+    s ^= (s >> 7) ^ (s >> 14) ^ (s >> 21) ^ (s >> 28) ^ (s >> 35) ^ (s >> 42) ^ (s >> 49) ^ (s >> 56) ^ (s >> 63);
+    s ^= (s << 13) ^ (s << 26) ^ (s << 39) ^ (s << 52);
+    return s;
+}
+
 // s % 6 from 32-bit operations only.  s mod 2 is the low bit.  s mod 3 uses
 // 2^k == (-1)^k (mod 3): with p = popcount(s) and e = popcount of the even
 // bits, s == e + 2 (p - e) = 2p - e (mod 3), a number in [0, 128]; its

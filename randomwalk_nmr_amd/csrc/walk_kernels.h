@@ -39,7 +39,7 @@ struct WalkArgs {
     uint32_t *error_flag;            // device word, bit 0: walker started outside the image
     const uint32_t *zwords;          // unfolded image (periodic / mirror fast path), else null
     ZLayout zl;                      // its dilated address layout
-    uint32_t zcache;                 // 1: per-walker 16-byte chunk cache in registers
+    uint32_t zcache;                 // chunk cache in registers: 0 none, 1 16-byte (word select), 2/3 16-byte, pipelined
     const uint32_t *zlabels;         // unfolded solid labels, 4 bits per voxel (per-label walks)
     // internal-field CPMG (emode 3): field[x + y W + z W H], phase step gammatau,
     // per-walker surviving fraction and phase (in/out)
@@ -56,7 +56,7 @@ struct LaunchShape {
 // Largest block for K walkers per lane: the register budget per lane shrinks
 // as the block grows (512 VGPRs per SIMD lane / waves per SIMD), so wider K
 // runs in smaller blocks to keep every walker in registers.
-__host__ __device__ constexpr uint32_t max_threads_for_k(int k) { return k >= 4 ? 256u : (k == 2 ? 512u : 1024u); }
+__host__ __device__ constexpr uint32_t max_threads_for_k(int k) { return k >= 4 ? 512u : 1024u; }
 
 static constexpr uint32_t kPfgseThreads = 256;
 static constexpr int kPfgseWaves = (int)kPfgseThreads / 64;

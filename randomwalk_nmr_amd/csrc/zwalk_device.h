@@ -8,7 +8,7 @@
 // the pair indexes a 6-entry direction table in LDS (axis mask, dilated
 // addend, packed displacement); the dilated add gives the candidate voxel
 // address with the periodic wrap included; then the wall bit: with the
-// register chunk cache (RCACHE) each walker keeps the 16-byte chunk
+// register chunk cache (CACHE != 0) each walker keeps the 16-byte chunk
 // (8 x 4 x 4 voxels) of its last lookup and reads memory only when the
 // candidate leaves it (about one step in five), else one 4-byte gather per
 // step; then either the move (address and packed displacement) or a wall hit
@@ -48,20 +48,108 @@ __device__ __forceinline__ void zbuild_table(const ZLayout &zl, uint4 *tab, uint
     }
 }
 
-// Candidate address of one move from A: advances the RNG state s, reads the
-// direction entry (16 bytes per entry in tab_bytes, 8 in fix_bytes) and
-// returns the dilated-add result; dp receives the packed displacement.
+// Wide direction table: one 16-byte entry per (state & 1, byte sum of the
+// state), index (state & 1) * 2048 + sb, so the step reads its entry without
+// reducing sb mod 3 (sb <= 2040; 64 KiB of LDS).  Periods that are powers of
+// two only (no wrap fix-up table).  Copied from the 6-entry table `small`
+// (built and synced before); all threads of the block write; the caller
+// syncs.
+constexpr uint32_t kZWideEntries = 4096;
+
+__device__ __forceinline__ void zbuild_table_wide(const uint4 *small, uint4 *tab)
+{
+    for (uint32_t e = threadIdx.x; e < kZWideEntries; e += blockDim.x) {
+        const uint32_t sb = e & 2047u;
+        tab[e] = small[2u * (sb % 3u) + (e >> 11)];
+    }
+}
+
+// Byte-key table: for key = (state & 1) * 2048 + byte sum of the state, the
+// byte offset (2 * (sb % 3) + (state & 1)) * 16 of the direction entry in
+// the 6-entry table (4 KiB of LDS).  All threads of the block write; the
+// caller syncs.
+constexpr uint32_t kZKeyEntries = 4096;
+
+__device__ __forceinline__ void zbuild_key_table(uint8_t *keys)
+{
+    for (uint32_t e = threadIdx.x; e < kZKeyEntries; e += blockDim.x)
+        keys[e] = (uint8_t)(16u * (2u * ((e & 2047u) % 3u) + (e >> 11)));
+}
+
+// The direction entry of the next move: advances the RNG state s and reads
+// the table entry (16 bytes per entry in tab_bytes, 8 in fix_bytes) of
+// state % 6.  TAB 0: the 6-entry table, s % 3 by arithmetic; 1: tab_bytes is
+// the wide table (zbuild_table_wide); 2: the 6-entry table reached through
+// the byte-key table `keys` (zbuild_key_table).
+struct ZEntry {
+    uint32_t m, add, dp, fx, fy;   // axis mask, dilated addend, packed displacement, wrap fix-up
+};
+
+template <bool WRAPFIX, int TAB = 0>
+__device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, const char *fix_bytes, ZEntry &e,
+                                           const uint8_t *keys = nullptr)
+{
+    static_assert(!(TAB == 1 && WRAPFIX), "the wide table has no wrap fix-up");
+    s = xorshift64(s);
+    const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+    uint32_t off;
+    if (TAB == 2) {
+        // key = byte sum + (state & 1) * 2048, the parity entering as the
+        // accumulator of the first byte sum
+        const uint32_t key = __builtin_amdgcn_sad_u8(lo, 0u, __builtin_amdgcn_sad_u8(hi, 0u, (lo << 11) & 0x800u));
+        off = keys[key];
+    } else if (TAB == 1) {
+        // byte sum sb <= 2040 == state (mod 3)
+        const uint32_t sb = __builtin_amdgcn_sad_u8(hi, 0u, __builtin_amdgcn_sad_u8(lo, 0u, 0u));
+        off = (sb << 4) | ((lo << 15) & 0x8000u);
+    } else {
+        const uint32_t sb = __builtin_amdgcn_sad_u8(hi, 0u, __builtin_amdgcn_sad_u8(lo, 0u, 0u));
+        // q = sb / 3 by a 24-bit multiply; entry 2 * (sb - 3q) + (state & 1)
+        // at 16 bytes per entry
+        const uint32_t q = __umul24(sb, 2731u) >> 13;
+        off = (uint32_t)__mul24((int)q, -96) + ((sb << 5) | ((lo & 1u) << 4));
+    }
+    const uint4 t = *reinterpret_cast<const uint4 *>(tab_bytes + off);
+    e.m = t.x;
+    e.add = t.y;
+    e.dp = t.z;
+    if (WRAPFIX) {
+        const uint2 f = *reinterpret_cast<const uint2 *>(fix_bytes + (off >> 1));
+        e.fx = f.x;
+        e.fy = f.y;
+    }
+}
+
+// Candidate address of the move of entry e from A (dilated add, periodic
+// wrap included; WRAPFIX: non-power-of-two period fix-up).
 template <bool WRAPFIX>
+__device__ __forceinline__ uint32_t zmove_entry(uint32_t A, const ZEntry &e)
+{
+    uint32_t c = (A & ~e.m) | (((A | ~e.m) + e.add) & e.m);
+    if (WRAPFIX)
+        c = (c & e.m) == e.fx ? ((c & ~e.m) | e.fy) : c;
+    return c;
+}
+
+// Candidate address of one move from A: advances the RNG state s, reads the
+// direction entry and returns the dilated-add result; dp receives the packed
+// displacement.
+template <bool WRAPFIX, bool WIDE = false>
 __device__ __forceinline__ uint32_t zpropose(uint64_t &s, uint32_t A, const char *tab_bytes, const char *fix_bytes,
                                              int32_t &dp)
 {
+    static_assert(!(WIDE && WRAPFIX), "the wide table has no wrap fix-up");
     s = xorshift64(s);
     const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
-    // byte sum sb <= 2040 == state (mod 3); q = sb / 3 by a 24-bit multiply;
-    // table entry 2 * (sb - 3q) + (state & 1) at 16 bytes per entry
+    // byte sum sb <= 2040 == state (mod 3)
     const uint32_t sb = __builtin_amdgcn_sad_u8(hi, 0u, __builtin_amdgcn_sad_u8(lo, 0u, 0u));
-    const uint32_t q = __umul24(sb, 2731u) >> 13;
-    const uint32_t off = (uint32_t)__mul24((int)q, -96) + ((sb << 5) | ((lo & 1u) << 4));
+    uint32_t off;
+    if (WIDE) {
+        off = (sb << 4) | ((lo << 15) & 0x8000u);
+    } else {
+        const uint32_t q = __umul24(sb, 2731u) >> 13;
+        off = (uint32_t)__mul24((int)q, -96) + ((sb << 5) | ((lo & 1u) << 4));
+    }
     const uint4 t = *reinterpret_cast<const uint4 *>(tab_bytes + off);
     const uint32_t m = t.x;
     uint32_t c = (A & ~m) | (((A | ~m) + t.y) & m);
@@ -96,6 +184,35 @@ __device__ __forceinline__ uint32_t zchunk_word(uint32_t c, const uint4 &QV)
     return (c & 32u) ? hi : lo;
 }
 
+// Chunk caches of CW bits (64: 4 x 4 x 4 voxels, 128: 8 x 4 x 4) read by
+// chunk id (c >> log2 CW, the load address is the id times the chunk bytes)
+// and tested with one 64-bit shift, whose hardware shift count is c & 63.
+template <int CW>
+__device__ __forceinline__ void zchunk_fill_n(const char *words, uint32_t c, uint32_t &QB, uint4 &QV)
+{
+    static_assert(CW == 64 || CW == 128, "chunk of 64 or 128 bits");
+    const uint32_t cid = c >> (CW == 64 ? 6 : 7);
+    if (cid != QB) {
+        if (CW == 64) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(words + (cid << 3));
+            QV.x = v.x;
+            QV.y = v.y;
+        } else {
+            QV = *reinterpret_cast<const uint4 *>(words + (cid << 4));
+        }
+        QB = cid;
+    }
+}
+
+template <int CW>
+__device__ __forceinline__ uint32_t zchunk_wall(uint32_t c, const uint4 &QV)
+{
+    uint64_t half = ((uint64_t)QV.y << 32) | QV.x;
+    if (CW == 128 && (c & 64u))
+        half = ((uint64_t)QV.w << 32) | QV.z;
+    return (uint32_t)(half >> (c & 63u)) & 1u;
+}
+
 // Solid label (4 bits per voxel, same address order) of address c.
 __device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
 {
@@ -109,11 +226,16 @@ __device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
 // delta (ldelta: the label deltas in LDS; labels from a.zlabels; per-label
 // hit counts in a.label_hits).  Echo partials go to part[row * part_stride]
 // (nullable).  All threads of the block call it.
-template <int EMODE, int K, bool WRAPFIX, bool RCACHE>
+// CACHE 0: no chunk cache; 1: 128-bit chunk, word select (the original
+// form); 64 / 128: chunk of that many bits with the shift test.  TAB: the
+// direction table form of zdirection (1: tab is the wide table; 2: keys is
+// the byte-key table).
+template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kMaxWaves], const uint4 *tab, const uint2 *fix,
-                                           const double *ldelta = nullptr)
+                                           const double *ldelta = nullptr, const uint8_t *keys = nullptr)
 {
+    constexpr bool WIDE = TAB == 1;
     constexpr int NL = EMODE == 2 ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
@@ -122,7 +244,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     const int n_labels = a.n_labels;
 
     uint32_t A[K], C[K], Hc[K];
-    uint32_t QB[K];   // RCACHE: 128-voxel chunk (address >> 7) held in QV
+    uint32_t QB[K];   // chunk cache: chunk id (address >> 6 or >> 7) held in QV
     uint4 QV[K];
     int32_t P[K], DP[K], X[K], Y[K], Z[K];
     uint64_t S[K];
@@ -170,13 +292,28 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     };
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
+    // CACHE 64 / 128 draw each walker's next direction one step ahead (T),
+    // between the chunk load of a step and its use, so the RNG, the table
+    // read and the load overlap; the state is rewound by one step at the end.
+    constexpr bool PIPE = CACHE >= 64;
+    ZEntry T[K];
+    if (PIPE)
+        static_for<K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            zdirection<WRAPFIX, TAB>(S[k], tab_bytes, fix_bytes, T[k], keys);
+        });
     auto step = [&]() {
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            C[k] = zpropose<WRAPFIX>(S[k], A[k], tab_bytes, fix_bytes, DP[k]);
+            if (PIPE) {
+                C[k] = zmove_entry<WRAPFIX>(A[k], T[k]);
+                DP[k] = (int32_t)T[k].dp;
+            } else {
+                C[k] = zpropose<WRAPFIX, WIDE>(S[k], A[k], tab_bytes, fix_bytes, DP[k]);
+            }
         });
-        uint32_t word[K];
-        if (RCACHE) {
+        uint32_t wallv[K];
+        if (CACHE == 1) {
             // each walker keeps the 16-byte chunk (8 x 4 x 4 voxels) of its last
             // lookup; only candidates outside it read memory
             static_for<K>([&](auto kc) {
@@ -185,18 +322,44 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                word[k] = zchunk_word(C[k], QV[k]);
+                wallv[k] = __builtin_amdgcn_ubfe(zchunk_word(C[k], QV[k]), C[k], 1u);   // offset = C & 31
+            });
+        } else if (CACHE == 64 || CACHE == 128) {
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                zchunk_fill_n<CACHE == 64 ? 64 : 128>(words, C[k], QB[k], QV[k]);
+            });
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                zdirection<WRAPFIX, TAB>(S[k], tab_bytes, fix_bytes, T[k], keys);
+            });
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                wallv[k] = zchunk_wall<CACHE == 64 ? 64 : 128>(C[k], QV[k]);
             });
         } else {
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
                 // c < 2^bits, so the byte offset of its word is inside the volume
-                word[k] = *reinterpret_cast<const uint32_t *>(words + ((C[k] >> 3) & ~3u));
+                const uint32_t word = *reinterpret_cast<const uint32_t *>(words + ((C[k] >> 3) & ~3u));
+                wallv[k] = __builtin_amdgcn_ubfe(word, C[k], 1u);
             });
         }
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            const uint32_t wall = __builtin_amdgcn_ubfe(word[k], C[k], 1u);   // offset = C & 31
+            const uint32_t wall = wallv[k];
+            if (CACHE >= 64 && EMODE == 1) {
+                // two short divergent paths (4 VALU in all) instead of the
+                // selects and unconditional multiply the compiler would make
+                if (__builtin_expect(wall != 0u, 0)) {
+                    Hc[k] += 1u;
+                    E[k] *= DL[k];
+                } else {
+                    A[k] = C[k];
+                    P[k] += DP[k];
+                }
+                return;
+            }
             A[k] = wall ? A[k] : C[k];
             P[k] += wall ? 0 : DP[k];
             Hc[k] += wall;
@@ -233,6 +396,8 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
     }
     flush();
+    if (PIPE)
+        static_for<K>([&](auto kc) { S[decltype(kc)::value] = xorshift64_inverse(S[decltype(kc)::value]); });
 
     static_for<K>([&](auto kc) {
         constexpr int k = decltype(kc)::value;

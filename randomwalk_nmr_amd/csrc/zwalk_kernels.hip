@@ -11,22 +11,32 @@
 
 namespace rwnmr {
 
-template <int EMODE, int K, bool WRAPFIX, bool RCACHE>
-__global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkArgs a)
+// Two walkers per lane fit 64 VGPRs, so 8 waves per SIMD stay resident and
+// a million walkers take a single round of 1024-thread blocks.
+template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB>
+__global__ __launch_bounds__(max_threads_for_k(K), (K == 2 && CACHE >= 64) ? 8 : 1) void zwalk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kMaxWaves];
-    __shared__ uint4 tab[6];
+    __shared__ uint4 tab[TAB == 1 ? kZWideEntries : 6];
+    __shared__ uint4 small[6];
     __shared__ uint2 fix[6];
+    __shared__ uint8_t keys[TAB == 2 ? kZKeyEntries : 4];
     __shared__ double ldelta[kMaxLabels];
     const uint64_t tile = tile_of_block(a);
     if (a.tile_map && tile >= a.n_tiles)
         return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
-    zbuild_table(a.zl, tab, fix);
+    zbuild_table(a.zl, TAB == 1 ? small : tab, fix);
+    if (TAB == 2)
+        zbuild_key_table(keys);
     if (EMODE == 2 && threadIdx.x < kMaxLabels)
         ldelta[threadIdx.x] = (int)threadIdx.x < a.n_labels ? a.label_delta[threadIdx.x] : 1.0;
     __syncthreads();
-    zwalk_tile<EMODE, K, WRAPFIX, RCACHE>(a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride, lds,
-                                          tab, fix, ldelta);
+    if (TAB == 1) {
+        zbuild_table_wide(small, tab);
+        __syncthreads();
+    }
+    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB>(a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride,
+                                              lds, tab, fix, ldelta, keys);
 }
 
 // PFGSE on the unfolded image: the walk between diffusion times counts hits
@@ -224,24 +234,41 @@ hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, ui
     return hipGetLastError();
 }
 
-template <int EMODE, bool WRAPFIX, bool RCACHE>
+template <int EMODE, bool WRAPFIX, int CACHE, int TAB>
 static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
     switch (ls.k) {
-    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, RCACHE>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, RCACHE>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX, RCACHE>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, CACHE, TAB>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX, CACHE, TAB>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+// a.zcache: 0 no chunk cache; 1 128-bit chunk with word select; 2 and 3 a
+// 128-bit chunk with the shift test and the direction drawn one step ahead,
+// through the byte-key table (2) or, for power-of-two periods, the wide
+// direction table (3; the 6-entry table otherwise).
 template <int EMODE>
 static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
-    if (a.zl.wrapfix)
-        return a.zcache ? launch_z<EMODE, true, true>(a, ls, blocks, st) : launch_z<EMODE, true, false>(a, ls, blocks, st);
-    return a.zcache ? launch_z<EMODE, false, true>(a, ls, blocks, st) : launch_z<EMODE, false, false>(a, ls, blocks, st);
+    if (a.zl.wrapfix) {
+        switch (a.zcache) {
+        case 0: return launch_z<EMODE, true, 0, 0>(a, ls, blocks, st);
+        case 1: return launch_z<EMODE, true, 1, 0>(a, ls, blocks, st);
+        case 2: return launch_z<EMODE, true, 128, 2>(a, ls, blocks, st);
+        case 3: return launch_z<EMODE, true, 128, 0>(a, ls, blocks, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    switch (a.zcache) {
+    case 0: return launch_z<EMODE, false, 0, 0>(a, ls, blocks, st);
+    case 1: return launch_z<EMODE, false, 1, 0>(a, ls, blocks, st);
+    case 2: return launch_z<EMODE, false, 128, 2>(a, ls, blocks, st);
+    case 3: return launch_z<EMODE, false, 128, 1>(a, ls, blocks, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hipStream_t st)

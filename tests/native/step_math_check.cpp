@@ -171,6 +171,18 @@ int main()
     }
     for (uint64_t v = 0; v < 1000000; ++v)
         EXPECT(mod6(v) == v % 6u, "mod6 small %llu\n", (unsigned long long)v);
+    // xorshift64_inverse undoes one step: along the stream, on small values
+    // and single bits
+    s = 1234567891011ull;
+    for (int i = 0; i < 2000000; ++i) {
+        const uint64_t t = xorshift64(s);
+        EXPECT(xorshift64_inverse(t) == s, "inverse %llu\n", (unsigned long long)s);
+        s = t;
+    }
+    for (uint64_t v = 0; v < 100000; ++v)
+        EXPECT(xorshift64_inverse(xorshift64(v)) == v, "inverse small %llu\n", (unsigned long long)v);
+    for (int b = 0; b < 64; ++b)
+        EXPECT(xorshift64(xorshift64_inverse(1ull << b)) == (1ull << b), "inverse bit %d\n", b);
     const uint64_t edges[] = {~0ull, ~0ull - 1, 1ull << 63, (1ull << 63) - 1, 0xAAAAAAAAAAAAAAAAull,
                               0x5555555555555555ull, 0xFFFFFFFFull, 0x100000000ull};
     for (uint64_t v : edges)

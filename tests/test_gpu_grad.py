@@ -112,7 +112,7 @@ def test_backward_unfolded_equals_folded_replay(ctx, bc):
     labels = R.voronoi_labels(solid, 10, 8, 9)
     seeds, x, y, z = R.seed_and_place(solid, 10, 2500)
     grads, sums = [], []
-    for mode in (2, 0):
+    for mode in (3, 0):
         ctx.set_unfolded(mode)
         try:
             img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
@@ -123,6 +123,6 @@ def test_backward_unfolded_equals_folded_replay(ctx, bc):
             c = torch.tensor(np.random.default_rng(3).standard_normal(m.echoes + 1))
             grads.append(np.asarray(m.backward_delta(c)))
         finally:
-            ctx.set_unfolded(2)
+            ctx.set_unfolded(3)
     assert np.array_equal(sums[0], sums[1])
     assert np.all(np.abs(grads[0] - grads[1]) <= 1e-12 * np.abs(grads[1]).max())

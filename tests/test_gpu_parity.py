@@ -81,7 +81,7 @@ def test_locality_order_and_brick_layout_match_oracle(ctx, bc, layout, shape):
 
 @pytest.mark.parametrize("bc", ["periodic", "mirror"])
 @pytest.mark.parametrize("dims", [(32, 16, 8), (16, 16, 16), (30, 21, 17), (9, 1, 5), (2, 3, 64)])
-@pytest.mark.parametrize("unfolded", [2, 1, 0])
+@pytest.mark.parametrize("unfolded", [4, 3, 2, 1, 0])
 @pytest.mark.parametrize("shape", [(256, 1), (512, 2), (256, 4)])
 def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
     """Periodic / mirror CPMG and map walks on the unfolded image with and
@@ -102,7 +102,7 @@ def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
         g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=37, steps_per_echo=5, bc=bc, penalty=pen)
         m = R.map_run(ctx, img, x, y, z, seeds, steps=37 * 5, bc=bc)
     finally:
-        ctx.set_unfolded(2)
+        ctx.set_unfolded(3)
         ctx.set_launch(256, 1)
     o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=37, steps_per_echo=5, bc=bc, penalty=pen)
     _assert_walkers(g, o)
@@ -113,18 +113,21 @@ def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
 
 @pytest.mark.parametrize("bc", ["periodic", "mirror"])
 @pytest.mark.parametrize("spe", [1, 600])
-def test_unfolded_long_echoes_and_locality_match_oracle(ctx, bc, spe):
+@pytest.mark.parametrize("unfolded", [4, 3, 2])
+def test_unfolded_long_echoes_and_locality_match_oracle(ctx, bc, spe, unfolded):
     """Steps per echo above the packed-displacement flush interval (511) and
     the Z-order re-sort of the unfolded walk on a power-of-two image."""
-    print(f"[test] unfolded locality bc={bc} spe={spe}", flush=True)
+    print(f"[test] unfolded locality bc={bc} spe={spe} unfolded={unfolded}", flush=True)
     solid, seeds, x, y, z, pen = _case(32, 32, 32, 3.5, 0.35, 4001, 23)
     echoes = 9 if spe > 1 else 400
     ctx.set_locality(True, resort_every=4, min_walkers=0)
+    ctx.set_unfolded(unfolded)
     try:
         img = R.Image.from_solid(ctx, solid)
         g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=echoes, steps_per_echo=spe, bc=bc, penalty=pen)
     finally:
         ctx.set_locality(True, resort_every=1000, min_walkers=65536)
+        ctx.set_unfolded(3)
     o = O.walk(O.bitblock_pack(solid), (32, 32, 32), x, y, z, seeds, echoes=echoes, steps_per_echo=spe, bc=bc,
                penalty=pen)
     _assert_walkers(g, o)
@@ -176,7 +179,7 @@ def test_cpmg_labels_match_oracle(ctx):
 
 @pytest.mark.parametrize("bc", ["periodic", "mirror"])
 @pytest.mark.parametrize("dims", [(28, 28, 28), (32, 16, 8)])
-@pytest.mark.parametrize("unfolded", [2, 1, 0])
+@pytest.mark.parametrize("unfolded", [4, 3, 2, 1, 0])
 def test_cpmg_labels_unfolded_and_folded_match_oracle(ctx, bc, dims, unfolded):
     """Per-label CPMG on the unfolded image (4-bit label volume) and on the
     folded kernel: walkers, energies and per-label hit counts bit-exact."""
@@ -191,7 +194,7 @@ def test_cpmg_labels_unfolded_and_folded_match_oracle(ctx, bc, dims, unfolded):
         g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=21, steps_per_echo=5, bc=bc, label_delta=ld,
                        label_hits=True)
     finally:
-        ctx.set_unfolded(2)
+        ctx.set_unfolded(3)
     o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=21, steps_per_echo=5, bc=bc,
                labels=labels, label_delta=ld, label_hits=True)
     _assert_walkers(g, o)
@@ -291,7 +294,7 @@ def test_histogram_method_matches_oracle(ctx, bc, log_scale, device):
 
 @pytest.mark.parametrize("bc", ["noflux", "mirror"])
 @pytest.mark.parametrize("n,n_samples", [(3000, 10), (2999, 7), (700, 3)])
-@pytest.mark.parametrize("unfolded", [2, 0])
+@pytest.mark.parametrize("unfolded", [3, 0])
 def test_pfgse_sampled_matches_oracle_per_sample(ctx, bc, n, n_samples, unfolded):
     """Walker samples of N // n_samples consecutive walkers (the remainder
     walks in no sample): each sample's M(k, Delta) == the oracle on that
@@ -308,7 +311,7 @@ def test_pfgse_sampled_matches_oracle_per_sample(ctx, bc, n, n_samples, unfolded
         g = R.pfgse_run_sampled(ctx, img, x, y, z, seeds, n_samples=n_samples, steps_inc=steps_inc, kx=kx, ky=zero,
                                 kz=zero, voxel=1.0, bc=bc, set_delta=sets)
     finally:
-        ctx.set_unfolded(2)
+        ctx.set_unfolded(3)
     blocks = O.bitblock_pack(solid)
     full = O.pfgse(blocks, (28, 24, 20), x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=zero, kz=zero, voxel=1.0,
                    bc=bc, set_delta=sets)
@@ -365,7 +368,7 @@ def test_pfgse_k_set_per_diffusion_time(ctx):
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("kind", ["arithmetic", "random"])
-@pytest.mark.parametrize("unfolded", [2, 0])
+@pytest.mark.parametrize("unfolded", [3, 0])
 def test_pfgse_phase_paths_and_kernels_match_oracle(ctx, bc, kind, unfolded):
     """Arithmetic k rows (phases by complex rotation) and irregular k rows
     (cos per k), on the unfolded-image and folded PFGSE kernels, with a
@@ -390,7 +393,7 @@ def test_pfgse_phase_paths_and_kernels_match_oracle(ctx, bc, kind, unfolded):
         g = R.pfgse_run(ctx, img, x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=ky, kz=kz, voxel=0.8, bc=bc,
                         set_delta=sets)
     finally:
-        ctx.set_unfolded(2)
+        ctx.set_unfolded(3)
     o = O.pfgse(O.bitblock_pack(solid), (27, 19, 33), x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=ky, kz=kz,
                 voxel=0.8, bc=bc, set_delta=sets)
     for k in ("x", "y", "z", "seed", "hits"):
