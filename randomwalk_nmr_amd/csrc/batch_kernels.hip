@@ -27,7 +27,9 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
     __shared__ double lds[kEchoChunk][kMaxWaves];
     __shared__ uint4 tab[6];
     __shared__ uint2 fix[6];
+    __shared__ uint8_t keys[kZKeyEntries];   // the byte-key table: the same for every instance
     __shared__ uint32_t next;
+    zbuild_key_table(keys);
     for (;;) {
         if (threadIdx.x == 0)
             next = atomicAdd(b.counter, 1u);
@@ -69,16 +71,17 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         a.error_flag = b.error_flag;
         a.zwords = I.zwords;
         a.zl = I.zl;
-        a.zcache = 1;
+        a.zcache = 2;
         double *part = I.partials + bt.tile;
         if (I.zwords) {
             // the previous tile's table reads ended at its last block barrier
+            // (the key table was synced by the barriers of the first tile)
             zbuild_table(I.zl, tab, fix);
             __syncthreads();
             if (I.zl.wrapfix)
-                zwalk_tile<1, K, true, 1>(a, bt.tile, part, I.n_tiles, lds, tab, fix);
+                zwalk_tile<1, K, true, 128, 2>(a, bt.tile, part, I.n_tiles, lds, tab, fix, nullptr, keys);
             else
-                zwalk_tile<1, K, false, 1>(a, bt.tile, part, I.n_tiles, lds, tab, fix);
+                zwalk_tile<1, K, false, 128, 2>(a, bt.tile, part, I.n_tiles, lds, tab, fix, nullptr, keys);
             continue;
         }
         switch (I.bc) {

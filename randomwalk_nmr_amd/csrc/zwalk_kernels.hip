@@ -49,7 +49,9 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     __shared__ double lds[kMaxSets * kPfgseKChunk][kPfgseWaves];
     __shared__ uint4 tab[6];
     __shared__ uint2 fix[6];
+    __shared__ uint8_t keys[kZKeyEntries];
     zbuild_table(a.zl, tab, fix);
+    zbuild_key_table(keys);
     __syncthreads();
     bool inside;
     const uint64_t id = pfgse_walker(p, blockIdx.x, threadIdx.x, blockDim.x, a.n, inside);
@@ -74,6 +76,10 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     const char *words = reinterpret_cast<const char *>(a.zwords);
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
+    // the next direction is drawn a step ahead, during the chunk load (the
+    // state is rewound by one step at the end)
+    ZEntry T;
+    zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
     auto flush = [&]() {
         int32_t dx, dy, dz;
         zunpack(P, dx, dy, dz);
@@ -89,10 +95,11 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
         while (left) {
             const uint32_t ns = left < kZFlush - since ? left : kZFlush - since;
             for (uint32_t st = 0; st < ns; ++st) {
-                int32_t dp;
-                const uint32_t c = zpropose<WRAPFIX>(s, A, tab_bytes, fix_bytes, dp);
-                zchunk_fill(words, c, QB, QV);
-                const uint32_t wall = __builtin_amdgcn_ubfe(zchunk_word(c, QV), c, 1u);
+                const uint32_t c = zmove_entry<WRAPFIX>(A, T);
+                const int32_t dp = (int32_t)T.dp;
+                zchunk_fill_n<128>(words, c, QB, QV);
+                zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
+                const uint32_t wall = zchunk_wall<128>(c, QV);
                 A = wall ? A : c;
                 P += wall ? 0 : dp;
                 h += wall;
@@ -114,6 +121,7 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
         if (p.msd_rows)
             pfgse_msd(p, i, dX, dY, dZ, active, lds);
     }
+    s = xorshift64_inverse(s);
     if (active) {
         a.x[id] = X;
         a.y[id] = Y;
