@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch", default="", help="threads,k (skips the autotuner)")
+    ap.add_argument("--bc", default="mirror", choices=["noflux", "periodic", "mirror"],
+                    help="boundary condition of the 512^3 CPMG workload (the headline is mirror)")
     ap.add_argument("--unfolded", type=int, default=-1,
                     help="unfolded-image kernel variant (rwnmr_ctx_set_unfolded); -1 keeps the library default")
     ap.add_argument("--workload", default="cpmg512", choices=["cpmg512", "small", "pfgse", "batch", "fit"])
@@ -112,7 +114,7 @@ def main():
     elif not args.no_autotune:
         t = time.time()
         # one locality re-sort interval: the walkers' spatial order decays over it
-        tpb, k, ms = ctx.autotune(img, X0, Y0, Z0, SEED0, steps_per_echo=spe, bc="mirror", trial_echoes=1000,
+        tpb, k, ms = ctx.autotune(img, X0, Y0, Z0, SEED0, steps_per_echo=spe, bc=args.bc, trial_echoes=1000,
                                   delta=delta)
         log(f"autotune: threads {tpb} x walkers/lane {k} ({ms:.2f} ms per 1000 echoes) in {time.time() - t:.1f}s")
     tpb, k = ctx.get_launch()
@@ -123,7 +125,7 @@ def main():
         Z.copy_(Z0)
         SEED.copy_(SEED0)
         EN.fill_(1.0)
-        R.cpmg_run(ctx, img, X, Y, Z, SEED, echoes=E, steps_per_echo=spe, bc="mirror", delta=delta, energy=EN,
+        R.cpmg_run(ctx, img, X, Y, Z, SEED, echoes=E, steps_per_echo=spe, bc=args.bc, delta=delta, energy=EN,
                    echo_sums=SUMS, sync=True)
 
     for i in range(args.warmup):
@@ -155,7 +157,7 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f)
-        if prof.get("launch") == f"{tpb},{k}":
+        if prof.get("launch") == f"{tpb},{k}" and args.bc == "mirror":
             traffic = prof.get("hbm_bytes_per_launch")
 
     sums = SUMS.cpu().numpy()
@@ -170,7 +172,7 @@ def main():
         ns = min(10_000, N)
         # all host threads: first 10,000 walkers over the full 10,000 echoes
         secs, es_cpu, xo, yo, zo = O.cpmg_reference_loop(blocks, (S, S, S), x0[:ns], y0[:ns], z0[:ns], seeds[:ns],
-                                                         delta, echoes=E, steps_per_echo=spe, bc="mirror",
+                                                         delta, echoes=E, steps_per_echo=spe, bc=args.bc,
                                                          use_digit_sum=True, threads=threads)
         cpu_rate = ns * spe * E / secs
         log(f"cpu baseline: {threads} threads {cpu_rate:.3e} walker-steps/s ({secs:.1f}s)")
@@ -179,7 +181,7 @@ def main():
         # one thread: first 1,000 walkers over 2,000 echoes
         n1, e1 = min(1000, N), min(2000, E)
         secs1, *_ = O.cpmg_reference_loop(blocks, (S, S, S), x0[:n1], y0[:n1], z0[:n1], seeds[:n1], delta,
-                                          echoes=e1, steps_per_echo=spe, bc="mirror", use_digit_sum=True,
+                                          echoes=e1, steps_per_echo=spe, bc=args.bc, use_digit_sum=True,
                                           threads=1)
         rate1 = n1 * spe * e1 / secs1
         log(f"cpu baseline: 1 thread {rate1:.3e} walker-steps/s ({secs1:.1f}s)")
@@ -204,12 +206,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"CPMG {S}^3 sphere pack (r 6-16, porosity {phi:.3f}), {N} walkers, {E} echoes x "
-                               f"{spe} steps, mirror BC, voxel {voxel} um, rho {rho} um/s",
+                               f"{spe} steps, {args.bc} BC, voxel {voxel} um, rho {rho} um/s",
                    "walker_steps_per_step": walker_steps, "launch": {"threads_per_block": tpb,
                                                                     "walkers_per_lane": k}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "zwalk_kernel<CPMG, 16-byte chunk cache, byte-key direction table, direction drawn a step ahead> on the unfolded mirror image",
+                     "kernel": ("walk_kernel<noflux, CPMG> on the brick-ordered bitBlock image" if args.bc == "noflux" else
+                                "zwalk_kernel<CPMG, 16-byte chunk cache, byte-key direction table, direction drawn a step "
+                                f"ahead> on the unfolded {args.bc} image"),
                      "avg_kernel_ms": avg_kernel_s * 1e3, "launches": launches,
                      "algorithmic_bytes_per_launch": steps_per_launch * BYTES_PER_STEP},
         "cpu_baseline": cpu,

@@ -79,7 +79,8 @@ class Context:
               self.handle)
 
     def set_unfolded(self, mode: int = 3):
-        """Periodic / mirror walks: 0 folded kernels, 1 unfolded image (no
+        """Periodic / mirror walks (and noflux with modes 3, 4 on an image
+        of sides >= 2, on the periodic unfolded image): 0 folded kernels, 1 unfolded image (no
         per-step fold), 2 unfolded + per-walker 16-byte chunk cache, 3 and 4
         the chunk tested by one shift and the next direction drawn a step
         ahead, through the 4 KiB byte-key table (3, default) or the 64 KiB

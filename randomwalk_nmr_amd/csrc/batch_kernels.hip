@@ -34,12 +34,14 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         if (threadIdx.x == 0)
             next = atomicAdd(b.counter, 1u);
         __syncthreads();
-        const uint32_t t = next;
+        // uniform across the block: scalar loads of the tile and instance
+        // descriptors (kept out of VGPRs and scratch)
+        const uint32_t t = __builtin_amdgcn_readfirstlane(next);
         __syncthreads();  // every thread has read `next` before it is reused
         if (t >= b.n_tiles)
             break;
         const BatchTile bt = b.tiles[t];
-        const BatchInstance I = b.inst[bt.inst];
+        const BatchInstance I = b.inst[__builtin_amdgcn_readfirstlane(bt.inst)];
         WalkArgs a;
         a.blocks = I.blocks;
         a.labels_blocked = nullptr;
@@ -72,6 +74,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         a.zwords = I.zwords;
         a.zl = I.zl;
         a.zcache = 2;
+        a.znoflux = 0;   // noflux instances run the folded walk here
         double *part = I.partials + bt.tile;
         if (I.zwords) {
             // the previous tile's table reads ended at its last block barrier

@@ -433,21 +433,32 @@ const uint32_t *zlabels_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
     return z.lstate == 1 ? z.labels : nullptr;
 }
 
-// Use the unfolded-image kernel for this walk when one applies.
-void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, WalkArgs &a)
+// Use the unfolded-image kernel for this walk when one applies.  noflux_ok:
+// the caller's kernel also runs noflux walks on the periodic unfolded image
+// (the CPMG and map walks, with the pipelined forms 3 and 4 and every image
+// size >= 2).
+void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, WalkArgs &a, bool noflux_ok = false)
 {
     a.zwords = nullptr;
     a.zlabels = nullptr;
+    a.znoflux = 0;
     if (emode < 0 || emode > 2 || a.shift != 0)
         return;
+    int zbc = bc;
+    if (bc == rwnmr::kNoflux) {
+        if (!noflux_ok || c->unfolded < 3 || img->W < 2 || img->H < 2 || img->D < 2)
+            return;
+        zbc = rwnmr::kPeriodic;
+    }
     const uint32_t *zlab = nullptr;
-    if (emode == 2 && !(zlab = zlabels_for(c, img, bc)))
+    if (emode == 2 && !(zlab = zlabels_for(c, img, zbc)))
         return;
-    if (const ZImage *z = zimage_for(c, img, bc)) {
+    if (const ZImage *z = zimage_for(c, img, zbc)) {
         a.zwords = z->words;
         a.zl = z->zl;
         a.zcache = c->unfolded >= 2 ? (uint32_t)(c->unfolded - 1) : 0u;
         a.zlabels = zlab;
+        a.znoflux = bc == rwnmr::kNoflux ? 1u : 0u;
     }
 }
 
@@ -665,7 +676,7 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
             }
         }
     }
-    attach_zimage(c, img, bc, emode, a);
+    attach_zimage(c, img, bc, emode, a, true);
     const uint32_t blocks = rwnmr::walk_grid_blocks(w->n, c->shape, (int)shift);
     a.partial_stride = blocks;
     double *sums = nullptr;
@@ -1053,7 +1064,7 @@ rwnmr_status rwnmr_map_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *
     WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
     a.energy = nullptr;  // the map walk never touches energies
     a.echoes = 1;
-    attach_zimage(c, img, bc, 0, a);
+    attach_zimage(c, img, bc, 0, a, true);
     uint64_t left = steps;
     while (left > 0) {
         const uint32_t now = (uint32_t)std::min<uint64_t>(left, 1ull << 30);

@@ -268,10 +268,46 @@ struct ZDir {
     uint32_t fixv;    // replacement field value
 };
 
+// Noflux move on the periodic unfolded image (period = image size, every
+// size >= 2): the forward move (m, add, dp) crosses the face when
+// ((c & mp) | (A & mm)) == tgt (c: the candidate, A: the current address;
+// going up the field wraps to 0 or, for a period that is not a power of
+// two, reaches the period; going down the walker is at 0); the reference
+// then turns around (B/NMR_Simulation/NMR_cpmg_cuda.cpp:1847-1902): addend
+// add2, displacement dp2.  Walkers never leave the image, so no wrap
+// fix-up is needed.
+struct ZFlip {
+    uint32_t m, add;
+    int32_t dp;
+    uint32_t tgt, add2;
+    int32_t dp2;
+    uint32_t mp, mm;
+};
+
 // s % 6 residue for table index idx (inverse of the CRT pairing).
 RW_HD uint32_t zdir_residue(uint32_t idx)
 {
     return (0x521430u >> (4u * idx)) & 0xFu;   // idx 0..5 -> 0, 3, 4, 1, 2, 5
+}
+
+RW_HD ZFlip zdir_noflux(const ZLayout &L, uint32_t idx)
+{
+    const uint32_t r = zdir_residue(idx);
+    const int ax = move_axis(r), sg = move_sign(r);
+    const uint32_t m = L.m[ax], low = m & (~m + 1u);
+    const int32_t unit = ax == 0 ? 1 : (ax == 1 ? 1 << 10 : 1 << 20);
+    const uint32_t k = (uint32_t)__builtin_popcount(m);
+    const uint32_t p = L.period[ax];
+    ZFlip f;
+    f.m = m;
+    f.add = sg > 0 ? low : m;
+    f.dp = sg * unit;
+    f.add2 = sg > 0 ? m : low;
+    f.dp2 = -sg * unit;
+    f.mp = sg > 0 ? m : 0u;
+    f.mm = sg > 0 ? 0u : m;
+    f.tgt = (sg > 0 && (1ull << k) != (uint64_t)p) ? zdeposit(p, m) : 0u;
+    return f;
 }
 
 RW_HD ZDir zdir_make(const ZLayout &L, uint32_t idx)

@@ -70,10 +70,24 @@ __device__ __forceinline__ void zbuild_table_wide(const uint4 *small, uint4 *tab
 // caller syncs.
 constexpr uint32_t kZKeyEntries = 4096;
 
-__device__ __forceinline__ void zbuild_key_table(uint8_t *keys)
+// bytes: the entry size of the direction table the offsets point into (16,
+// or 32 for the noflux table).
+__device__ __forceinline__ void zbuild_key_table(uint8_t *keys, uint32_t bytes = 16u)
 {
     for (uint32_t e = threadIdx.x; e < kZKeyEntries; e += blockDim.x)
-        keys[e] = (uint8_t)(16u * (2u * ((e & 2047u) % 3u) + (e >> 11)));
+        keys[e] = (uint8_t)(bytes * (2u * ((e & 2047u) % 3u) + (e >> 11)));
+}
+
+// Noflux direction table of layout zl (periodic image): two 16-byte halves
+// per direction, {m, add, dp, tgt} and {add2, dp2, mp, mm} (ZFlip).
+// Threads 0..5 write; the caller syncs.
+__device__ __forceinline__ void zbuild_table_noflux(const ZLayout &zl, uint4 *tab)
+{
+    if (threadIdx.x < 6) {
+        const ZFlip f = zdir_noflux(zl, threadIdx.x);
+        tab[2 * threadIdx.x] = make_uint4(f.m, f.add, (uint32_t)f.dp, f.tgt);
+        tab[2 * threadIdx.x + 1] = make_uint4(f.add2, (uint32_t)f.dp2, f.mp, f.mm);
+    }
 }
 
 // The direction entry of the next move: advances the RNG state s and reads
@@ -82,13 +96,15 @@ __device__ __forceinline__ void zbuild_key_table(uint8_t *keys)
 // the wide table (zbuild_table_wide); 2: the 6-entry table reached through
 // the byte-key table `keys` (zbuild_key_table).
 struct ZEntry {
-    uint32_t m, add, dp, fx, fy;   // axis mask, dilated addend, packed displacement, wrap fix-up
+    uint32_t m, add, dp, fx, fy;      // axis mask, dilated addend, packed displacement, wrap fix-up
+    uint32_t tgt, add2, dp2, mp, mm;  // noflux: face test and the turned-around move (ZFlip)
 };
 
-template <bool WRAPFIX, int TAB = 0>
+template <bool WRAPFIX, int TAB = 0, bool NOFLUX = false>
 __device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, const char *fix_bytes, ZEntry &e,
                                            const uint8_t *keys = nullptr)
 {
+    static_assert(!NOFLUX || (TAB == 2 && !WRAPFIX), "noflux: byte-key table, no wrap fix-up");
     static_assert(!(TAB == 1 && WRAPFIX), "the wide table has no wrap fix-up");
     s = xorshift64(s);
     const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
@@ -113,6 +129,14 @@ __device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, c
     e.m = t.x;
     e.add = t.y;
     e.dp = t.z;
+    if (NOFLUX) {
+        const uint4 u = *reinterpret_cast<const uint4 *>(tab_bytes + off + 16u);
+        e.tgt = t.w;
+        e.add2 = u.x;
+        e.dp2 = u.y;
+        e.mp = u.z;
+        e.mm = u.w;
+    }
     if (WRAPFIX) {
         const uint2 f = *reinterpret_cast<const uint2 *>(fix_bytes + (off >> 1));
         e.fx = f.x;
@@ -129,6 +153,18 @@ __device__ __forceinline__ uint32_t zmove_entry(uint32_t A, const ZEntry &e)
     if (WRAPFIX)
         c = (c & e.m) == e.fx ? ((c & ~e.m) | e.fy) : c;
     return c;
+}
+
+// Noflux candidate of entry e from A: the forward move, or the turned-around
+// one when the forward move crosses a face of the image; dp receives the
+// packed displacement of the move taken.
+__device__ __forceinline__ uint32_t zmove_noflux(uint32_t A, const ZEntry &e, int32_t &dp)
+{
+    const uint32_t c = (A & ~e.m) | (((A | ~e.m) + e.add) & e.m);
+    const bool crossed = ((c & e.mp) | (A & e.mm)) == e.tgt;
+    const uint32_t cf = (A & ~e.m) | (((A | ~e.m) + e.add2) & e.m);
+    dp = (int32_t)(crossed ? e.dp2 : e.dp);
+    return crossed ? cf : c;
 }
 
 // Candidate address of one move from A: advances the RNG state s, reads the
@@ -230,12 +266,15 @@ __device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
 // form); 64 / 128: chunk of that many bits with the shift test.  TAB: the
 // direction table form of zdirection (1: tab is the wide table; 2: keys is
 // the byte-key table).
-template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0>
+// NOFLUX: noflux walk on the periodic unfolded image (tab: the noflux table,
+// keys built for 32-byte entries); walkers must start inside the image.
+template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kMaxWaves], const uint4 *tab, const uint2 *fix,
                                            const double *ldelta = nullptr, const uint8_t *keys = nullptr)
 {
     constexpr bool WIDE = TAB == 1;
+    static_assert(!NOFLUX || CACHE >= 64, "noflux: pipelined chunk-cache walk only");
     constexpr int NL = EMODE == 2 ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
@@ -259,6 +298,13 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         X[k] = act[k] ? a.x[j] : 0;
         Y[k] = act[k] ? a.y[j] : 0;
         Z[k] = act[k] ? a.z[j] : 0;
+        if (NOFLUX && act[k] &&
+            !((unsigned)X[k] < (unsigned)a.W && (unsigned)Y[k] < (unsigned)a.H && (unsigned)Z[k] < (unsigned)a.D)) {
+            // a noflux walker must start inside the image (as the folded walk)
+            act[k] = false;
+            atomicOr(a.error_flag, 1u);
+            X[k] = Y[k] = Z[k] = 0;
+        }
         S[k] = act[k] ? a.seed[j] : 1ull;
         E[k] = act[k] ? (a.energy ? a.energy[j] : 1.0) : 0.0;   // idle lanes add 0 to echo sums
         Hc[k] = (act[k] && a.hits) ? a.hits[j] : 0u;
@@ -300,12 +346,14 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     if (PIPE)
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            zdirection<WRAPFIX, TAB>(S[k], tab_bytes, fix_bytes, T[k], keys);
+            zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys);
         });
     auto step = [&]() {
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            if (PIPE) {
+            if (NOFLUX) {
+                C[k] = zmove_noflux(A[k], T[k], DP[k]);
+            } else if (PIPE) {
                 C[k] = zmove_entry<WRAPFIX>(A[k], T[k]);
                 DP[k] = (int32_t)T[k].dp;
             } else {
@@ -331,7 +379,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                zdirection<WRAPFIX, TAB>(S[k], tab_bytes, fix_bytes, T[k], keys);
+                zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys);
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;

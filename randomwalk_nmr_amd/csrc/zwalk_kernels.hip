@@ -11,13 +11,11 @@
 
 namespace rwnmr {
 
-// Two walkers per lane fit 64 VGPRs, so 8 waves per SIMD stay resident and
-// a million walkers take a single round of 1024-thread blocks.
-template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB>
-__global__ __launch_bounds__(max_threads_for_k(K), (K == 2 && CACHE >= 64) ? 8 : 1) void zwalk_kernel(const WalkArgs a)
+template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false>
+__global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kMaxWaves];
-    __shared__ uint4 tab[TAB == 1 ? kZWideEntries : 6];
+    __shared__ uint4 tab[TAB == 1 ? kZWideEntries : (NOFLUX ? 12 : 6)];
     __shared__ uint4 small[6];
     __shared__ uint2 fix[6];
     __shared__ uint8_t keys[TAB == 2 ? kZKeyEntries : 4];
@@ -25,9 +23,12 @@ __global__ __launch_bounds__(max_threads_for_k(K), (K == 2 && CACHE >= 64) ? 8 :
     const uint64_t tile = tile_of_block(a);
     if (a.tile_map && tile >= a.n_tiles)
         return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
-    zbuild_table(a.zl, TAB == 1 ? small : tab, fix);
+    if (NOFLUX)
+        zbuild_table_noflux(a.zl, tab);
+    else
+        zbuild_table(a.zl, TAB == 1 ? small : tab, fix);
     if (TAB == 2)
-        zbuild_key_table(keys);
+        zbuild_key_table(keys, NOFLUX ? 32u : 16u);
     if (EMODE == 2 && threadIdx.x < kMaxLabels)
         ldelta[threadIdx.x] = (int)threadIdx.x < a.n_labels ? a.label_delta[threadIdx.x] : 1.0;
     __syncthreads();
@@ -35,8 +36,8 @@ __global__ __launch_bounds__(max_threads_for_k(K), (K == 2 && CACHE >= 64) ? 8 :
         zbuild_table_wide(small, tab);
         __syncthreads();
     }
-    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB>(a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride,
-                                              lds, tab, fix, ldelta, keys);
+    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX>(a, tile, a.partials ? a.partials + tile : nullptr,
+                                                      a.partial_stride, lds, tab, fix, ldelta, keys);
 }
 
 // PFGSE on the unfolded image: the walk between diffusion times counts hits
@@ -242,13 +243,13 @@ hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, ui
     return hipGetLastError();
 }
 
-template <int EMODE, bool WRAPFIX, int CACHE, int TAB>
+template <int EMODE, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false>
 static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
     switch (ls.k) {
-    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, CACHE, TAB>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX, CACHE, TAB>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -261,6 +262,8 @@ static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t bl
 template <int EMODE>
 static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
+    if (a.znoflux)   // noflux on the periodic image: pipelined byte-key walk only
+        return a.zcache >= 2 ? launch_z<EMODE, false, 128, 2, true>(a, ls, blocks, st) : hipErrorInvalidValue;
     if (a.zl.wrapfix) {
         switch (a.zcache) {
         case 0: return launch_z<EMODE, true, 0, 0>(a, ls, blocks, st);

@@ -79,15 +79,16 @@ def test_locality_order_and_brick_layout_match_oracle(ctx, bc, layout, shape):
     _assert_echoes(g["echo_sums"], o["echo_sums"])
 
 
-@pytest.mark.parametrize("bc", ["periodic", "mirror"])
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("dims", [(32, 16, 8), (16, 16, 16), (30, 21, 17), (9, 1, 5), (2, 3, 64)])
 @pytest.mark.parametrize("unfolded", [4, 3, 2, 1, 0])
 @pytest.mark.parametrize("shape", [(256, 1), (512, 2), (256, 4)])
 def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
-    """Periodic / mirror CPMG and map walks on the unfolded image with and
-    without the per-walker chunk cache (power-of-two periods: plain dilated
-    wrap; others: wrap fix; a dimension of 1) and on the folded kernel:
-    walkers bit-exact, echo sums to 1e-12, for every launch shape."""
+    """Noflux / periodic / mirror CPMG and map walks on the unfolded image
+    with and without the per-walker chunk cache (power-of-two periods: plain
+    dilated wrap; others: wrap fix; noflux: face test and turn-around on the
+    periodic image, folded kernel when a dimension is 1) and on the folded
+    kernel: walkers bit-exact, echo sums to 1e-12, for every launch shape."""
     print(f"[test] unfolded={unfolded} bc={bc} dims={dims} launch={shape}", flush=True)
     W, H, D = dims
     rng = np.random.default_rng(W * 10000 + H * 100 + D)
@@ -99,6 +100,12 @@ def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
     ctx.set_launch(*shape)
     try:
         img = R.Image.from_solid(ctx, solid)
+        if bc == "noflux" and min(dims) < 2:
+            # noflux is refused on an image with a dimension of 1 (the walk
+            # would step outside it)
+            with pytest.raises(R.RwnmrError):
+                R.cpmg_run(ctx, img, x, y, z, seeds, echoes=2, steps_per_echo=1, bc=bc, penalty=pen)
+            return
         g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=37, steps_per_echo=5, bc=bc, penalty=pen)
         m = R.map_run(ctx, img, x, y, z, seeds, steps=37 * 5, bc=bc)
     finally:
@@ -111,7 +118,7 @@ def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
         assert np.array_equal(m[k], o[k]), k
 
 
-@pytest.mark.parametrize("bc", ["periodic", "mirror"])
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("spe", [1, 600])
 @pytest.mark.parametrize("unfolded", [4, 3, 2])
 def test_unfolded_long_echoes_and_locality_match_oracle(ctx, bc, spe, unfolded):
@@ -177,7 +184,7 @@ def test_cpmg_labels_match_oracle(ctx):
     assert np.array_equal(g["label_hits"], o["label_hits"].T)
 
 
-@pytest.mark.parametrize("bc", ["periodic", "mirror"])
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("dims", [(28, 28, 28), (32, 16, 8)])
 @pytest.mark.parametrize("unfolded", [4, 3, 2, 1, 0])
 def test_cpmg_labels_unfolded_and_folded_match_oracle(ctx, bc, dims, unfolded):
