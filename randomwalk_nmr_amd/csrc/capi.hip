@@ -435,8 +435,8 @@ const uint32_t *zlabels_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
 
 // Use the unfolded-image kernel for this walk when one applies.  noflux_ok:
 // the caller's kernel also runs noflux walks on the periodic unfolded image
-// (the CPMG and map walks, with the pipelined forms 3 and 4 and every image
-// size >= 2).
+// (the CPMG, map and PFGSE walks, with the pipelined forms 3 and 4 and every
+// image size >= 2).
 void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, WalkArgs &a, bool noflux_ok = false)
 {
     a.zwords = nullptr;
@@ -1348,7 +1348,7 @@ rwnmr_status pfgse_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w
         p.k_rot = k_rows_arithmetic(hk.data(), nkv, n_k, k_per_delta ? n_delta : 1u) ? 1u : 0u;
     }
     WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
-    attach_zimage(c, img, bc, 1, a);
+    attach_zimage(c, img, bc, 1, a, true);
     double *Mdev = M, *msd_dev = msd;
     if (!w->on_device) {
         RW_HIP(c, c->sums.reserve(8ull * rows + 8));

@@ -44,21 +44,32 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
 // (register chunk cache, packed displacement flushed every kZFlush steps and
 // at every diffusion time), then pow(delta, hits) per energy set and the
 // shared measurement (pfgse_measure).  One walker per thread.
-template <bool WRAPFIX, bool ROT>
+// NOFLUX: noflux walk on the periodic unfolded image (face test and
+// turn-around, zmove_noflux); walkers must start inside the image.
+template <bool WRAPFIX, bool ROT, bool NOFLUX = false>
 __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a, const PfgseArgs p)
 {
     __shared__ double lds[kMaxSets * kPfgseKChunk][kPfgseWaves];
-    __shared__ uint4 tab[6];
+    __shared__ uint4 tab[NOFLUX ? 12 : 6];
     __shared__ uint2 fix[6];
     __shared__ uint8_t keys[kZKeyEntries];
-    zbuild_table(a.zl, tab, fix);
-    zbuild_key_table(keys);
+    if (NOFLUX)
+        zbuild_table_noflux(a.zl, tab);
+    else
+        zbuild_table(a.zl, tab, fix);
+    zbuild_key_table(keys, NOFLUX ? 32u : 16u);
     __syncthreads();
     bool inside;
     const uint64_t id = pfgse_walker(p, blockIdx.x, threadIdx.x, blockDim.x, a.n, inside);
-    const bool active = inside;
-    const uint64_t j = active ? id : 0;
+    bool active = inside;
+    uint64_t j = active ? id : 0;
     int32_t X = active ? a.x[j] : 0, Y = active ? a.y[j] : 0, Z = active ? a.z[j] : 0;
+    if (NOFLUX && active && !((unsigned)X < (unsigned)a.W && (unsigned)Y < (unsigned)a.H && (unsigned)Z < (unsigned)a.D)) {
+        active = false;   // a noflux walker must start inside the image
+        atomicOr(a.error_flag, 1u);
+        j = 0;
+        X = Y = Z = 0;
+    }
     uint64_t s = active ? a.seed[j] : 1ull;
     uint32_t hits = (active && a.hits) ? a.hits[j] : 0u;
     const int32_t x0 = active ? p.x0[j] : 0, y0 = active ? p.y0[j] : 0, z0 = active ? p.z0[j] : 0;
@@ -80,7 +91,7 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     // the next direction is drawn a step ahead, during the chunk load (the
     // state is rewound by one step at the end)
     ZEntry T;
-    zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
+    zdirection<WRAPFIX, 2, NOFLUX>(s, tab_bytes, fix_bytes, T, keys);
     auto flush = [&]() {
         int32_t dx, dy, dz;
         zunpack(P, dx, dy, dz);
@@ -96,10 +107,16 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
         while (left) {
             const uint32_t ns = left < kZFlush - since ? left : kZFlush - since;
             for (uint32_t st = 0; st < ns; ++st) {
-                const uint32_t c = zmove_entry<WRAPFIX>(A, T);
-                const int32_t dp = (int32_t)T.dp;
+                int32_t dp;
+                uint32_t c;
+                if (NOFLUX) {
+                    c = zmove_noflux(A, T, dp);
+                } else {
+                    c = zmove_entry<WRAPFIX>(A, T);
+                    dp = (int32_t)T.dp;
+                }
                 zchunk_fill_n<128>(words, c, QB, QV);
-                zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
+                zdirection<WRAPFIX, 2, NOFLUX>(s, tab_bytes, fix_bytes, T, keys);
                 const uint32_t wall = zchunk_wall<128>(c, QV);
                 A = wall ? A : c;
                 P += wall ? 0 : dp;
@@ -142,7 +159,12 @@ hipError_t launch_zpfgse(const WalkArgs &a, const PfgseArgs &p, hipStream_t st)
     if (a.zwords == nullptr || a.shift != 0)
         return hipErrorInvalidValue;
     const dim3 g(pfgse_grid_blocks(p, a.n)), t(kPfgseThreads);
-    if (a.zl.wrapfix) {
+    if (a.znoflux) {
+        if (p.k_rot)
+            hipLaunchKernelGGL((zpfgse_kernel<false, true, true>), g, t, 0, st, a, p);
+        else
+            hipLaunchKernelGGL((zpfgse_kernel<false, false, true>), g, t, 0, st, a, p);
+    } else if (a.zl.wrapfix) {
         if (p.k_rot)
             hipLaunchKernelGGL((zpfgse_kernel<true, true>), g, t, 0, st, a, p);
         else
