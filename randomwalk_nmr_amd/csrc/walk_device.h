@@ -44,6 +44,48 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// v from the lane given by the DPP control CTRL (row_mirror 0x140: lane
+// 15 - i of the row; row_half_mirror 0x141: lane 7 - i of the half row;
+// quad_perm 0x4E: lane ^ 2; quad_perm 0xB1: lane ^ 1), both 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false),
+                            __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false));
+}
+
+// One butterfly step of the reduce-scatter: items [0, n) and [n, 2n) of v;
+// the lane keeps the half its bit selects and adds the other half of the
+// partner lane (which has the opposite bit and the same earlier bits).
+template <int CTRL, int N>
+__device__ __forceinline__ void wave_halve(double *v, bool bit)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double keep = bit ? v[i + N] : v[i];
+        const double send = bit ? v[i] : v[i + N];
+        v[i] = keep + dpp_f64<CTRL>(send);
+    }
+}
+
+// Sums over the 64 lanes of 8 values per lane at once (a reduce-scatter: 3
+// halving steps inside rows, then a lane pair and the 4 rows): every lane
+// returns the wave total of value (lane >> 1) & 7.  Fixed order, so
+// deterministic; about 7 VALU per value instead of a 6-step shuffle each.
+__device__ __forceinline__ double wave_reduce_scatter8(double (&v)[8])
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    wave_halve<0x140, 4>(v, (lane & 8) != 0);   // row mirror: bit 3
+    wave_halve<0x141, 2>(v, (lane & 4) != 0);   // half-row mirror: bit 2
+    wave_halve<0x4E, 1>(v, (lane & 2) != 0);    // lane ^ 2: bit 1
+    double r = v[0];
+    r += dpp_f64<0xB1>(r);                      // lane ^ 1
+    r += __shfl_xor(r, 16, kWave);
+    r += __shfl_xor(r, 32, kWave);
+    return r;
+}
+
 // Image geometry as plain values (uniform across the grid).
 struct Geom {
     const uint64_t *blocks;
@@ -222,7 +264,7 @@ __device__ __forceinline__ void stage_echo(double (*lds)[kMaxWaves], uint32_t ec
 // complex rotation, e^{i(ph0 + j phd)} = e^{i ph0} (e^{i phd})^j, 4 FMAs per k
 // instead of a cos (rounding grows ~2 ulp per k, far inside the 1e-11 M(0)
 // tolerance).  All threads of the block call it.
-template <bool ROT>
+template <bool ROT, int MAXSETS = kMaxSets>
 __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, double dX, double dY, double dZ,
                                               const double *e, int nsets, bool active,
                                               double (*lds)[kPfgseWaves])
@@ -240,22 +282,36 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
     }
     for (uint32_t k0 = 0; k0 < p.n_k; k0 += kPfgseKChunk) {
         const uint32_t kc = p.n_k - k0 < (uint32_t)kPfgseKChunk ? p.n_k - k0 : (uint32_t)kPfgseKChunk;
-        for (uint32_t kk = 0; kk < kc; ++kk) {
-            double cs;
-            if (ROT) {
-                cs = c;
-                const double cn = c * c1 - s * s1;
-                s = s * c1 + c * s1;
-                c = cn;
-            } else {
-                const uint64_t kk_i = krow + k0 + kk;
-                cs = cos((p.kx[kk_i] * dX + p.ky[kk_i] * dY + p.kz[kk_i] * dZ) * p.voxel);
+        // groups of 8 k: the cosines of the group, then one reduce-scatter
+        // per energy set (lane 2j of lanes 0..15 holds the wave sum of k j)
+        for (uint32_t g0 = 0; g0 < kc; g0 += 8) {
+            double cs[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t kk = g0 + (uint32_t)j;
+                cs[j] = 0.0;
+                if (kk < kc) {   // uniform: the rotation advances once per k
+                    if (ROT) {
+                        cs[j] = c;
+                        const double cn = c * c1 - s * s1;
+                        s = s * c1 + c * s1;
+                        c = cn;
+                    } else {
+                        const uint64_t kk_i = krow + k0 + kk;
+                        cs[j] = cos((p.kx[kk_i] * dX + p.ky[kk_i] * dY + p.kz[kk_i] * dZ) * p.voxel);
+                    }
+                }
             }
-            static_for<kMaxSets>([&](auto qc) {
+            static_for<MAXSETS>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 if (q < nsets) {
-                    const double v = wave_sum(active ? cs * e[q] : 0.0);
-                    if (lane == 0)
+                    double t[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        t[j] = active ? cs[j] * e[q] : 0.0;
+                    const double v = wave_reduce_scatter8(t);
+                    const uint32_t kk = g0 + (uint32_t)(lane >> 1);
+                    if ((lane & 0x31) == 0 && kk < kc)
                         lds[q * kPfgseKChunk + kk][wave] = v;
                 }
             });

@@ -45,8 +45,9 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
 // at every diffusion time), then pow(delta, hits) per energy set and the
 // shared measurement (pfgse_measure).  One walker per thread.
 // NOFLUX: noflux walk on the periodic unfolded image (face test and
-// turn-around, zmove_noflux); walkers must start inside the image.
-template <bool WRAPFIX, bool ROT, bool NOFLUX = false>
+// turn-around, zmove_noflux); walkers must start inside the image.  MAXSETS:
+// the most energy sets the instance handles (1, 2 or kMaxSets; registers).
+template <bool WRAPFIX, bool ROT, bool NOFLUX, int MAXSETS>
 __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a, const PfgseArgs p)
 {
     __shared__ double lds[kMaxSets * kPfgseKChunk][kPfgseWaves];
@@ -75,8 +76,8 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     const int32_t x0 = active ? p.x0[j] : 0, y0 = active ? p.y0[j] : 0, z0 = active ? p.z0[j] : 0;
     const double own_delta = (active && a.penalty) ? a.penalty[j] : a.delta_uniform;
     const int nsets = p.n_sets > 0 ? (int)p.n_sets : 1;
-    double e[kMaxSets];
-    static_for<kMaxSets>([&](auto qc) {
+    double e[MAXSETS];
+    static_for<MAXSETS>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         e[q] = !active ? 0.0 : ((q == 0 && a.energy) ? a.energy[j] : 1.0);
     });
@@ -129,13 +130,13 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
         }
         flush();
         hits += h;
-        static_for<kMaxSets>([&](auto qc) {
+        static_for<MAXSETS>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if (q < nsets)
                 e[q] *= pow(p.n_sets > 0 ? p.set_delta[q] : own_delta, (double)h);
         });
         const double dX = (double)(X - x0), dY = (double)(Y - y0), dZ = (double)(Z - z0);
-        pfgse_measure<ROT>(p, i, dX, dY, dZ, e, nsets, active, lds);
+        pfgse_measure<ROT, MAXSETS>(p, i, dX, dY, dZ, e, nsets, active, lds);
         if (p.msd_rows)
             pfgse_msd(p, i, dX, dY, dZ, active, lds);
     }
@@ -152,28 +153,39 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     }
 }
 
+template <bool WRAPFIX, bool ROT, bool NOFLUX>
+static void launch_zpfgse_sets(const WalkArgs &a, const PfgseArgs &p, dim3 g, dim3 t, hipStream_t st)
+{
+    if (p.n_sets <= 1)
+        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, 1>), g, t, 0, st, a, p);
+    else if (p.n_sets <= 2)
+        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, 2>), g, t, 0, st, a, p);
+    else
+        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, kMaxSets>), g, t, 0, st, a, p);
+}
+
 hipError_t launch_zpfgse(const WalkArgs &a, const PfgseArgs &p, hipStream_t st)
 {
     if (a.n == 0)
         return hipSuccess;
-    if (a.zwords == nullptr || a.shift != 0)
+    if (a.zwords == nullptr || a.shift != 0 || p.n_sets > (uint32_t)kMaxSets)
         return hipErrorInvalidValue;
     const dim3 g(pfgse_grid_blocks(p, a.n)), t(kPfgseThreads);
     if (a.znoflux) {
         if (p.k_rot)
-            hipLaunchKernelGGL((zpfgse_kernel<false, true, true>), g, t, 0, st, a, p);
+            launch_zpfgse_sets<false, true, true>(a, p, g, t, st);
         else
-            hipLaunchKernelGGL((zpfgse_kernel<false, false, true>), g, t, 0, st, a, p);
+            launch_zpfgse_sets<false, false, true>(a, p, g, t, st);
     } else if (a.zl.wrapfix) {
         if (p.k_rot)
-            hipLaunchKernelGGL((zpfgse_kernel<true, true>), g, t, 0, st, a, p);
+            launch_zpfgse_sets<true, true, false>(a, p, g, t, st);
         else
-            hipLaunchKernelGGL((zpfgse_kernel<true, false>), g, t, 0, st, a, p);
+            launch_zpfgse_sets<true, false, false>(a, p, g, t, st);
     } else {
         if (p.k_rot)
-            hipLaunchKernelGGL((zpfgse_kernel<false, true>), g, t, 0, st, a, p);
+            launch_zpfgse_sets<false, true, false>(a, p, g, t, st);
         else
-            hipLaunchKernelGGL((zpfgse_kernel<false, false>), g, t, 0, st, a, p);
+            launch_zpfgse_sets<false, false, false>(a, p, g, t, st);
     }
     return hipGetLastError();
 }
