@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void zbackward_kernel(const WalkArgs a, const 
             int32_t dp;
             const uint32_t c = zpropose<WRAPFIX>(s, A, tab_bytes, fix_bytes, dp);
             zchunk_fill(words, c, QB, QV);
-            if (!__builtin_amdgcn_ubfe(zchunk_word(c, QV), c, 1u)) {
+            if (!zchunk_bit(c, QV)) {
                 A = c;
             } else {
                 const uint32_t lab = zlabel(a.zlabels, c);

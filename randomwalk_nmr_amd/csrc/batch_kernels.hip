@@ -24,7 +24,7 @@ namespace rwnmr {
 template <int K>
 __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
 {
-    __shared__ double lds[kEchoChunk][kMaxWaves];
+    __shared__ double lds[kEchoChunk][kEchoCols];
     __shared__ uint4 tab[6];
     __shared__ uint2 fix[6];
     __shared__ uint8_t keys[kZKeyEntries];   // the byte-key table: the same for every instance

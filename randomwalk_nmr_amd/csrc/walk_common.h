@@ -180,6 +180,11 @@ RW_HD uint32_t morton3(uint32_t bx, uint32_t by, uint32_t bz)
 // both wrapping at 2^k.  A period that is not a power of two needs one fix
 // after the add (field == dep(P) -> 0 going up; field == all ones ->
 // dep(P - 1) going down).
+// Bit order: each aligned run of 64 addresses is one 64-bit value (word
+// 2i the low half, 2i + 1 the high half) holding address a at bit
+// 63 - (a & 63), so a left shift by a & 63 (the hardware count of a 64-bit
+// shift) brings the voxel's bit to the sign bit: the wall test is a shift
+// and a sign compare.
 
 struct ZLayout {
     uint32_t m[3];       // address bits of x, y, z

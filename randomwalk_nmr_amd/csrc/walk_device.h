@@ -26,6 +26,9 @@ namespace rwnmr {
 static constexpr int kWave = 64;
 static constexpr int kEchoChunk = 32;   // echoes staged in LDS between block syncs
 static constexpr int kMaxWaves = 16;    // up to 1024 threads per block
+// one staged echo row: a partial per half-wave, plus one pad column so the
+// rows of consecutive echoes start on different LDS banks
+static constexpr int kEchoCols = 2 * kMaxWaves + 1;
 
 // f(integral_constant<0>), ..., f(integral_constant<N-1>), fully inlined.
 template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
@@ -46,13 +49,31 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // v from the lane given by the DPP control CTRL (row_mirror 0x140: lane
 // 15 - i of the row; row_half_mirror 0x141: lane 7 - i of the half row;
-// quad_perm 0x4E: lane ^ 2; quad_perm 0xB1: lane ^ 1), both 32-bit halves.
-template <int CTRL>
+// quad_perm 0x4E: lane ^ 2; quad_perm 0xB1: lane ^ 1; row_bcast:15 0x142:
+// lane 15 of the previous row, written only in the rows ROWS selects), both
+// 32-bit halves.  Every source lane of these controls exists, so no old value
+// is needed (the unwritten rows of row_bcast:15 hold junk).  Callers run
+// with the whole wave active.
+template <int CTRL, int ROWS = 0xF>
 __device__ __forceinline__ double dpp_f64(double v)
 {
     const int lo = __double2loint(v), hi = __double2hiint(v);
-    return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false),
-                            __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false));
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, CTRL, ROWS, 0xF, true),
+                            __builtin_amdgcn_mov_dpp(lo, CTRL, ROWS, 0xF, true));
+}
+
+// Sums of v over the half-waves, lanes 0-31 in lane 31 and lanes 32-63 in
+// lane 63 (other lanes hold partial sums): the 16 lanes of each row by four
+// DPP steps (lane ^ 1, lane ^ 2, half-row mirror, row mirror), then rows 0
+// and 2 into rows 1 and 3 (row_bcast:15).  Fixed order, so deterministic;
+// 15 VALU and no LDS round trips.
+__device__ __forceinline__ double half_wave_sum(double v)
+{
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    return v + dpp_f64<0x142, 0xA>(v);
 }
 
 // One butterfly step of the reduce-scatter: items [0, n) and [n, 2n) of v;
@@ -230,23 +251,24 @@ __device__ __forceinline__ int start_ok(const Geom &g, const Walker &w)
 
 // Stage one echo's block sum; every kEchoChunk echoes (and at the last echo)
 // the block writes its partials out[row * row_stride] for the staged rows.
+// Each half-wave stages its own partial, so a wave spends 15 VALU per echo.
 // All threads of the block must call this.
-__device__ __forceinline__ void stage_echo(double (*lds)[kMaxWaves], uint32_t echo, uint32_t last_echo, double v,
+__device__ __forceinline__ void stage_echo(double (*lds)[kEchoCols], uint32_t echo, uint32_t last_echo, double v,
                                            double *out, uint64_t row_stride)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
-    v = wave_sum(v);
+    v = half_wave_sum(v);
     const int slot = echo % kEchoChunk;
-    if (lane == 0)
-        lds[slot][wave] = v;
+    if ((lane & 31) == 31)
+        lds[slot][2 * wave + (lane >> 5)] = v;
     if (slot == kEchoChunk - 1 || echo == last_echo) {
         __syncthreads();
         const int cnt = slot + 1;
         if ((int)threadIdx.x < cnt) {
             double s = 0.0;
-            for (int q = 0; q < nw; ++q)
+            for (int q = 0; q < 2 * nw; ++q)
                 s += lds[threadIdx.x][q];
             const uint64_t row = (uint64_t)(echo - slot) + threadIdx.x;
             out[row * row_stride] = s;
@@ -361,7 +383,7 @@ __device__ __forceinline__ void pfgse_msd(const PfgseArgs &p, uint32_t i, double
 // delta.  Echo partials go to part[row * part_stride] (nullable).
 template <int BC, int EMODE, int K>
 __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
-                                          double (*lds)[kMaxWaves])
+                                          double (*lds)[kEchoCols])
 {
     constexpr int NL = EMODE == 2 ? kMaxLabels : 1;
     const Geom g = make_geom(a);
@@ -478,7 +500,7 @@ __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, doub
 // Runs on the folded geometry (the field is indexed by the image voxel).
 template <int BC, int K>
 __device__ __forceinline__ void walk_tile_field(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
-                                                double (*lds)[kMaxWaves])
+                                                double (*lds)[kEchoCols])
 {
     const Geom g = make_geom(a);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;

@@ -27,7 +27,7 @@ namespace rwnmr {
 template <int BC, int EMODE, int K>
 __global__ __launch_bounds__(max_threads_for_k(K)) void walk_kernel(const WalkArgs a)
 {
-    __shared__ double lds[kEchoChunk][kMaxWaves];
+    __shared__ double lds[kEchoChunk][kEchoCols];
     const uint64_t tile = tile_of_block(a);
     if (a.tile_map && tile >= a.n_tiles)
         return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void walk_kernel(const WalkAr
 template <int BC, int K>
 __global__ __launch_bounds__(max_threads_for_k(K)) void walk_field_kernel(const WalkArgs a)
 {
-    __shared__ double lds[kEchoChunk][kMaxWaves];
+    __shared__ double lds[kEchoChunk][kEchoCols];
     const uint64_t tile = tile_of_block(a);
     if (a.tile_map && tile >= a.n_tiles)
         return;
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void walk_field_kernel(const 
 template <int BC, int EMODE>
 __global__ __launch_bounds__(1024) void walk_kernel_shift(const WalkArgs a)
 {
-    __shared__ double lds[kEchoChunk][kMaxWaves];
+    __shared__ double lds[kEchoChunk][kEchoCols];
     const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int active = id < a.n;
     const uint64_t j = active ? id : 0;

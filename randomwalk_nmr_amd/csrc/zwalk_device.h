@@ -212,12 +212,19 @@ __device__ __forceinline__ void zchunk_fill(const char *words, uint32_t c, uint3
     }
 }
 
-// The 32-bit word of QV that holds address c's bit.
+// The 32-bit word of QV that holds address c's bit (descending order in
+// each 64-bit half: addresses 0-31 of the half in its high word).
 __device__ __forceinline__ uint32_t zchunk_word(uint32_t c, const uint4 &QV)
 {
     const bool up = (c & 64u) != 0u;
     const uint32_t lo = up ? QV.z : QV.x, hi = up ? QV.w : QV.y;
-    return (c & 32u) ? hi : lo;
+    return (c & 32u) ? lo : hi;
+}
+
+// Address c's bit from its word: bit 31 - (c & 31), brought to the sign bit.
+__device__ __forceinline__ uint32_t zchunk_bit(uint32_t c, const uint4 &QV)
+{
+    return (zchunk_word(c, QV) << (c & 31u)) >> 31;
 }
 
 // Chunk caches of CW bits (64: 4 x 4 x 4 voxels, 128: 8 x 4 x 4) read by
@@ -244,9 +251,22 @@ template <int CW>
 __device__ __forceinline__ uint32_t zchunk_wall(uint32_t c, const uint4 &QV)
 {
     uint64_t half = ((uint64_t)QV.y << 32) | QV.x;
-    if (CW == 128 && (c & 64u))
-        half = ((uint64_t)QV.w << 32) | QV.z;
-    return (uint32_t)(half >> (c & 63u)) & 1u;
+    if (CW == 128) {
+        // the half that bit 6 of c selects, by two bitfield inserts under the
+        // sign-extended bit (3 VALU; a compare and two selects take 4)
+        // (as inline VALU: the compiler turns the mask form back into selects)
+        const uint32_t sel = (uint32_t)__builtin_amdgcn_sbfe((int)c, 6u, 1u);
+        uint32_t lo, hi;
+        asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(lo) : "v"(sel), "v"(QV.z), "v"(QV.x));
+        asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(hi) : "v"(sel), "v"(QV.w), "v"(QV.y));
+        half = ((uint64_t)hi << 32) | lo;
+    }
+    // descending bit order: address c's bit is bit 63 - (c & 63); the empty
+    // asm keeps the sign test on the high word (the compiler would widen it
+    // to a 64-bit compare)
+    uint32_t top = (uint32_t)((half << (c & 63u)) >> 32);
+    asm("" : "+v"(top));
+    return top >> 31;
 }
 
 // Solid label (4 bits per voxel, same address order) of address c.
@@ -270,7 +290,7 @@ __device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
 // keys built for 32-byte entries); walkers must start inside the image.
 template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
-                                           double (*lds)[kMaxWaves], const uint4 *tab, const uint2 *fix,
+                                           double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
                                            const double *ldelta = nullptr, const uint8_t *keys = nullptr)
 {
     constexpr bool WIDE = TAB == 1;
@@ -321,8 +341,8 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     });
 
     auto echo_value = [&]() {
-        double v = 0.0;
-        static_for<K>([&](auto kc) { v += E[decltype(kc)::value]; });
+        double v = E[0];
+        static_for<K - 1>([&](auto kc) { v += E[decltype(kc)::value + 1]; });
         return v;
     };
     auto flush = [&]() {
@@ -370,7 +390,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                wallv[k] = __builtin_amdgcn_ubfe(zchunk_word(C[k], QV[k]), C[k], 1u);   // offset = C & 31
+                wallv[k] = zchunk_bit(C[k], QV[k]);
             });
         } else if (CACHE == 64 || CACHE == 128) {
             static_for<K>([&](auto kc) {
@@ -388,9 +408,10 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         } else {
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                // c < 2^bits, so the byte offset of its word is inside the volume
-                const uint32_t word = *reinterpret_cast<const uint32_t *>(words + ((C[k] >> 3) & ~3u));
-                wallv[k] = __builtin_amdgcn_ubfe(word, C[k], 1u);
+                // c < 2^bits, so the byte offset of its 64-bit pair of words is
+                // inside the volume (at least 4 words)
+                const uint64_t pair = *reinterpret_cast<const uint64_t *>(words + ((C[k] >> 3) & ~7u));
+                wallv[k] = (uint32_t)((pair << (C[k] & 63u)) >> 63);
             });
         }
         static_for<K>([&](auto kc) {
@@ -431,7 +452,14 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         uint32_t left = spe;
         while (left) {
             const uint32_t ns = left < kZFlush - since ? left : kZFlush - since;
-            for (uint32_t i = 0; i < ns; ++i)
+            // two steps per trip: the pipelined direction entry then lives in
+            // alternating registers instead of being copied every step
+            uint32_t i = 0;
+            for (; i + 2 <= ns; i += 2) {
+                step();
+                step();
+            }
+            if (i < ns)
                 step();
             since += ns;
             left -= ns;

@@ -14,7 +14,7 @@ namespace rwnmr {
 template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false>
 __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkArgs a)
 {
-    __shared__ double lds[kEchoChunk][kMaxWaves];
+    __shared__ double lds[kEchoChunk][kEchoCols];
     __shared__ uint4 tab[TAB == 1 ? kZWideEntries : (NOFLUX ? 12 : 6)];
     __shared__ uint4 small[6];
     __shared__ uint2 fix[6];
@@ -200,7 +200,9 @@ __global__ __launch_bounds__(256) void zimage_build_kernel(const uint64_t *block
     const uint64_t nbits = 1ull << zl.bits;
     uint32_t word = 0;
     for (uint32_t b = 0; b < 32; ++b) {
-        const uint64_t A64 = w * 32u + b;
+        // descending order in each 64-bit pair of words: bit p of the pair
+        // holds address 64 * pair + 63 - p
+        const uint64_t A64 = (w >> 1) * 64u + 63u - 32u * (uint32_t)(w & 1u) - b;
         uint32_t bit = 1;
         if (A64 < nbits) {
             const uint32_t A = (uint32_t)A64;
