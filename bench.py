@@ -58,6 +58,9 @@ def main():
                     help="boundary condition of the 512^3 CPMG workload (the headline is mirror)")
     ap.add_argument("--unfolded", type=int, default=-1,
                     help="unfolded-image kernel variant (rwnmr_ctx_set_unfolded); -1 keeps the library default")
+    ap.add_argument("--resort-every", type=int, default=0,
+                    help="echoes between locality re-sorts of the walkers (rwnmr_ctx_set_locality); 0 keeps the "
+                         "library default")
     ap.add_argument("--workload", default="cpmg512", choices=["cpmg512", "small", "pfgse", "batch", "fit"])
     ap.add_argument("--ga-time-limit", type=float, default=150.0, help="fit workload: GA wall-time cap (s)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -100,6 +103,8 @@ def main():
     ctx = R.Context(0)
     if args.unfolded >= 0:
         ctx.set_unfolded(args.unfolded)
+    if args.resort_every > 0:
+        ctx.set_locality(True, args.resort_every)
     img = R.Image.from_solid(ctx, solid)
     dev = torch.device("cuda:0")
     X0, Y0, Z0 = (torch.from_numpy(a).to(dev) for a in (x0, y0, z0))

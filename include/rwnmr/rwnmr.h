@@ -91,9 +91,13 @@ rwnmr_status rwnmr_ctx_set_locality(rwnmr_ctx *ctx, int32_t enable, uint32_t res
  * NMR_cpmg_cuda.cpp:1991-2015).  Built per image on first use: 2W x 2H x 2D
  * bits for mirror (128 MiB at 512^3), W x H x D for periodic; volumes over
  * 2^32 voxels use the folded kernels.  mode 0: folded kernels; 1: unfolded;
- * 2 (default): unfolded, each walker also keeping the 16-byte chunk
- * (8 x 4 x 4 voxels) of its last lookup in registers so most steps read no
- * memory.  Results are identical in every mode. */
+ * 2: unfolded, each walker also keeping the 16-byte chunk (8 x 4 x 4
+ * voxels) of its last lookup in registers so most steps read no memory;
+ * 3 (default) and 4: the chunk tested by one shift and the next direction
+ * drawn a step ahead, through the 4 KiB byte-key table (3) or the 64 KiB
+ * wide direction table (4).  Modes 3 and 4 also run noflux walks (image
+ * sides >= 2) on the periodic unfolded image.  Results are identical in
+ * every mode. */
 rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *ctx, int32_t mode);
 
 /* Upload a packed image in the reference bitBlock layout: block index
