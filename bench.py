@@ -467,6 +467,8 @@ def wl_fit(args):
     labels = R.voronoi_labels(solid, 64, 8, args.seed + 1)
     seeds, x, y, z = R.seed_and_place(solid, args.seed + 2, N)
     ctx = R.Context(0)
+    if args.launch:
+        ctx.set_launch(*(int(v) for v in args.launch.split(",")))
     img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
     model = CpmgModel(ctx, img, x, y, z, seeds, echoes=E, steps_per_echo=spe, voxel=voxel, D0=D0, ckpt_every=500)
     rho_true = rng.uniform(5.0, 40.0, 8)

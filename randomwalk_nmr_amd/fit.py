@@ -7,7 +7,7 @@ The reference fits relaxivity with an (unbuilt) MPI island GA
 (B/GA/ga_core.cpp:101-131 run, :133-166 reproduction, :231-357 roulette and
 parents, :706-770 blend crossover, individual.cpp:22-44 mutation) whose
 fitness is a full forward simulation; its operators are restated in
-`genetic_fit` below over this build's GPU forward.
+`genetic_fit` below over the GPU forward.
 
 Decay model (NMR_cpmg.cpp:346-385): S(t_n) = exp(-t_n / T2b) M(n) / M(0),
 M(n) = sum_w E_w(n), delta_l = 1 - (2/3) rho_l eps / (D0 1e3)
@@ -24,12 +24,36 @@ from . import core
 from ._lib import check, load
 
 
+def _spread3(v):
+    """Bits 0..20 of v moved to bits 0, 3, 6, ... (uint64)."""
+    v = v & np.uint64(0x1FFFFF)
+    for shift, mask in ((32, 0x1F00000000FFFF), (16, 0x1F0000FF0000FF), (8, 0x100F00F00F00F00F),
+                        (4, 0x10C30C30C30C30C3), (2, 0x1249249249249249)):
+        v = (v | (v << np.uint64(shift))) & np.uint64(mask)
+    return v
+
+
+def morton_order(x, y, z):
+    """Permutation sorting walkers by the Morton code of their position, so
+    that walkers walked by neighbouring lanes and blocks share image lines
+    in the caches."""
+    c = [np.asarray(a, np.int64) for a in (x, y, z)]
+    c = [(a - a.min()).astype(np.uint64) if a.size else a.astype(np.uint64) for a in c]
+    key = _spread3(c[0]) | (_spread3(c[1]) << np.uint64(1)) | (_spread3(c[2]) << np.uint64(2))
+    return np.argsort(key, kind="stable")
+
+
 class CpmgModel:
     """Fixed labelled geometry and walkers resident on the GPU; evaluates the
-    echo sums M(n) for label deltas and their gradient by checkpoint replay."""
+    echo sums M(n) for label deltas and their gradient by checkpoint replay.
+    With locality=True the walkers are held in the Morton order of their
+    start positions (`order`: the permutation applied to the inputs), which
+    keeps the image lines of neighbouring walkers in L2 for the forward walk
+    and the replay; the echo sums then differ from the input order's only by
+    summation order."""
 
     def __init__(self, ctx: core.Context, img: core.Image, x, y, z, seed, *, echoes, steps_per_echo, voxel, D0,
-                 bc="mirror", ckpt_every=500):
+                 bc="mirror", ckpt_every=500, locality=True):
         import torch
         if img.n_labels == 0:
             raise ValueError("the image needs solid labels")
@@ -42,6 +66,8 @@ class CpmgModel:
         self.n_labels = img.n_labels
         dev = torch.device(f"cuda:{ctx.device}")
         self.dev = dev
+        self.order = morton_order(x, y, z) if locality else np.arange(len(x))
+        x, y, z, seed = (np.asarray(a)[self.order] for a in (x, y, z, seed))
         self.x0, self.y0, self.z0 = (torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev) for a in (x, y, z))
         self.s0 = torch.from_numpy(np.ascontiguousarray(seed, np.uint64).view(np.int64)).to(dev)
         self.x, self.y, self.z, self.s = (t.clone() for t in (self.x0, self.y0, self.z0, self.s0))

@@ -19,7 +19,9 @@ def _model(ctx, n=3000, echoes=60, every=16, side=32, seed=5):
     seeds, x, y, z = R.seed_and_place(solid, seed, n)
     img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
     m = CpmgModel(ctx, img, x, y, z, seeds, echoes=echoes, steps_per_echo=3, voxel=VOXEL, D0=D0, ckpt_every=every)
-    return m, solid, labels, (x, y, z, seeds)
+    # the model walks the walkers in the Morton order of their start positions
+    assert sorted(m.order.tolist()) == list(range(n))
+    return m, solid, labels, tuple(a[m.order] for a in (x, y, z, seeds))
 
 
 def test_forward_ckpt_equals_plain_labelled_run(ctx):
