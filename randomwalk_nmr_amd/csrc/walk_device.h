@@ -24,7 +24,7 @@
 namespace rwnmr {
 
 static constexpr int kWave = 64;
-static constexpr int kEchoChunk = 32;   // echoes staged in LDS between block syncs
+static constexpr int kEchoChunk = 16;   // echoes staged in LDS between block syncs
 static constexpr int kMaxWaves = 16;    // up to 1024 threads per block
 // one staged echo row: a partial per half-wave, plus one pad column so the
 // rows of consecutive echoes start on different LDS banks

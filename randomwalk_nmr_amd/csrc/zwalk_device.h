@@ -288,14 +288,19 @@ __device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
 // the byte-key table).
 // NOFLUX: noflux walk on the periodic unfolded image (tab: the noflux table,
 // keys built for 32-byte entries); walkers must start inside the image.
+// EMODE 2 with K == 1 keeps the per-label hit counts in LDS (lh: kMaxLabels
+// x blockDim entries, label-major; one LDS add per hit instead of a compare
+// and add per label), else in registers.
 template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
-                                           const double *ldelta = nullptr, const uint8_t *keys = nullptr)
+                                           const double *ldelta = nullptr, const uint8_t *keys = nullptr,
+                                           uint32_t *lh = nullptr)
 {
     constexpr bool WIDE = TAB == 1;
     static_assert(!NOFLUX || CACHE >= 64, "noflux: pipelined chunk-cache walk only");
-    constexpr int NL = EMODE == 2 ? kMaxLabels : 1;
+    constexpr bool LH_LDS = EMODE == 2 && K == 1;
+    constexpr int NL = (EMODE == 2 && !LH_LDS) ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
     const uint64_t n = a.n;
@@ -338,6 +343,10 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             LH[k][l] = (EMODE == 2 && a.label_hits && act[k] && l < n_labels) ? a.label_hits[(uint64_t)l * n + j]
                                                                                : 0u;
         });
+        if (LH_LDS)
+            for (int l = 0; l < kMaxLabels; ++l)
+                lh[(uint32_t)l * blockDim.x + threadIdx.x] =
+                    (a.label_hits && act[k] && l < n_labels) ? a.label_hits[(uint64_t)l * n + j] : 0u;
     });
 
     auto echo_value = [&]() {
@@ -429,6 +438,26 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 }
                 return;
             }
+            if (CACHE >= 64 && EMODE == 2) {
+                if (__builtin_expect(wall != 0u, 0)) {
+                    Hc[k] += 1u;
+                    const uint32_t lab = zlabel(a.zlabels, C[k]);
+                    E[k] *= ldelta[lab];
+                    if (LH_LDS) {
+                        __hip_atomic_fetch_add(&lh[lab * blockDim.x + threadIdx.x], 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    } else {
+                        static_for<NL>([&](auto lc) {
+                            constexpr int l = decltype(lc)::value;
+                            LH[k][l] += lab == (uint32_t)l ? 1u : 0u;
+                        });
+                    }
+                } else {
+                    A[k] = C[k];
+                    P[k] += DP[k];
+                }
+                return;
+            }
             A[k] = wall ? A[k] : C[k];
             P[k] += wall ? 0 : DP[k];
             Hc[k] += wall;
@@ -437,10 +466,15 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             if (EMODE == 2 && wall) {
                 const uint32_t lab = zlabel(a.zlabels, C[k]);
                 E[k] *= ldelta[lab];
-                static_for<NL>([&](auto lc) {
-                    constexpr int l = decltype(lc)::value;
-                    LH[k][l] += lab == (uint32_t)l ? 1u : 0u;
-                });
+                if (LH_LDS) {
+                    __hip_atomic_fetch_add(&lh[lab * blockDim.x + threadIdx.x], 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    static_for<NL>([&](auto lc) {
+                        constexpr int l = decltype(lc)::value;
+                        LH[k][l] += lab == (uint32_t)l ? 1u : 0u;
+                    });
+                }
             }
         });
     };
@@ -487,12 +521,18 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 a.energy[id] = E[k];
             if (a.hits)
                 a.hits[id] = Hc[k];
-            if (EMODE == 2 && a.label_hits)
-                static_for<NL>([&](auto lc) {
-                    constexpr int l = decltype(lc)::value;
-                    if (l < n_labels)
-                        a.label_hits[(uint64_t)l * n + id] = LH[k][l];
-                });
+            if (EMODE == 2 && a.label_hits) {
+                if (LH_LDS) {
+                    for (int l = 0; l < n_labels; ++l)
+                        a.label_hits[(uint64_t)l * n + id] = lh[(uint32_t)l * blockDim.x + threadIdx.x];
+                } else {
+                    static_for<NL>([&](auto lc) {
+                        constexpr int l = decltype(lc)::value;
+                        if (l < n_labels)
+                            a.label_hits[(uint64_t)l * n + id] = LH[k][l];
+                    });
+                }
+            }
         }
     });
 }

@@ -20,6 +20,7 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
     __shared__ uint2 fix[6];
     __shared__ uint8_t keys[TAB == 2 ? kZKeyEntries : 4];
     __shared__ double ldelta[kMaxLabels];
+    extern __shared__ uint32_t zlh[];   // EMODE 2, K == 1: per-label hit counts (kMaxLabels x blockDim)
     const uint64_t tile = tile_of_block(a);
     if (a.tile_map && tile >= a.n_tiles)
         return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
@@ -37,7 +38,8 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
         __syncthreads();
     }
     zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX>(a, tile, a.partials ? a.partials + tile : nullptr,
-                                                      a.partial_stride, lds, tab, fix, ldelta, keys);
+                                                      a.partial_stride, lds, tab, fix, ldelta, keys,
+                                                      EMODE == 2 && K == 1 ? zlh : nullptr);
 }
 
 // PFGSE on the unfolded image: the walk between diffusion times counts hits
@@ -282,8 +284,10 @@ hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, ui
 template <int EMODE, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false>
 static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
+    // EMODE 2 with one walker per lane keeps the per-label hit counts in LDS
+    const size_t lh_bytes = EMODE == 2 ? (size_t)kMaxLabels * ls.threads * sizeof(uint32_t) : 0;
     switch (ls.k) {
-    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), lh_bytes, st, a); break;
     case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
     case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
     default: return hipErrorInvalidValue;
