@@ -219,7 +219,7 @@ rwnmr_status rwnmr_cpmg_batch_run(rwnmr_ctx *ctx, const rwnmr_cpmg_instance *ins
  * device walkers with an energy array, device label_delta and echo_sums of
  * echoes+1) that also stores, every `every` echoes, each walker's position,
  * RNG state, energy and per-label hits into ckpt (device memory of
- * rwnmr_ckpt_bytes).  Asynchronous.
+ * rwnmr_ckpt_bytes), and the per-label hits at the end.  Asynchronous.
  * rwnmr_cpmg_backward: given dL/dM(n) (device, echoes+1) returns
  * dL/d delta_l (host, n_labels) by replaying every segment from its
  * checkpoint in parallel (one thread per walker and segment) instead of

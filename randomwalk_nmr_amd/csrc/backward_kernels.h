@@ -38,6 +38,22 @@ __host__ __device__ inline CkptView ckpt_view(void *base, uint64_t n, uint32_t n
     return v;
 }
 
+// The per-label hits at the end of the run, after the n_segments segments.
+__host__ __device__ inline uint32_t *ckpt_final_hits(void *base, uint64_t n, uint32_t n_segments, uint32_t n_labels)
+{
+    return (uint32_t *)((uint8_t *)base + (uint64_t)n_segments * ckpt_segment_bytes(n, n_labels));
+}
+
+// Whole checkpoint buffer: the segments, then the final per-label hits.
+__host__ __device__ inline uint64_t ckpt_total_bytes(uint64_t n, uint32_t n_segments, uint32_t n_labels)
+{
+    return ckpt_segment_bytes(n, n_labels) * n_segments + ((4ull * n_labels * n + 7) / 8) * 8;
+}
+
+// Threads per block of the unfolded replay (its LDS holds kMaxLabels doubles
+// per thread).
+constexpr uint32_t kZBackThreads = 512;
+
 struct BackwardArgs {
     void *ckpt;
     uint64_t n;

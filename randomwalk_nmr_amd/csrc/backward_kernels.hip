@@ -94,17 +94,30 @@ __global__ __launch_bounds__(256) void backward_kernel(const WalkArgs a, const B
 }
 
 // The same replay on the unfolded image (periodic / mirror; a.zwords,
-// a.zl, a.zlabels), with the per-walker register chunk cache.
+// a.zl, a.zlabels) with the forward walk's step: 16-byte register chunk
+// cache tested by one shift, byte-key direction table, next direction drawn
+// while the chunk load is in flight.  The per-echo sum over labels is
+// rearranged so a step touches one label only: with Q(n) = sum of g E over
+// the segment's echoes up to n and h_wl(n) = h_wl(start) + hits so far,
+//   G_wl = h_wl(end) Q(end) - R_wl,  R_wl = sum over the segment's label-l
+// hits of Q at the hit (Q before the hit's echo),
+// so a hit adds Q to R[label] (one LDS add) and an echo adds g E to Q;
+// h_wl(end) is the next segment's checkpoint (or the final hits).
 template <bool WRAPFIX>
-__global__ __launch_bounds__(256) void zbackward_kernel(const WalkArgs a, const BackwardArgs b)
+__global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs a, const BackwardArgs b)
 {
     __shared__ double lds[kMaxLabels][kMaxWaves];
     __shared__ uint4 tab[6];
     __shared__ uint2 fix[6];
+    __shared__ uint8_t keys[kZKeyEntries];
     __shared__ double ldelta[kMaxLabels];
+    __shared__ double racc[kMaxLabels * kZBackThreads];   // R_wl, label-major
     zbuild_table(a.zl, tab, fix);
+    zbuild_key_table(keys, 16u);
     if (threadIdx.x < kMaxLabels)
         ldelta[threadIdx.x] = (int)threadIdx.x < b.n_labels ? a.label_delta[threadIdx.x] : 1.0;
+    for (int l = 0; l < kMaxLabels; ++l)
+        racc[(uint32_t)l * blockDim.x + threadIdx.x] = 0.0;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -115,49 +128,52 @@ __global__ __launch_bounds__(256) void zbackward_kernel(const WalkArgs a, const 
     const bool active = id < n;
     const uint64_t j = active ? id : 0;
     const CkptView cv = ckpt_view(b.ckpt, n, b.n_segments, b.n_labels, seg);
+    const uint32_t *h_end = seg + 1 < b.n_segments ? ckpt_view(b.ckpt, n, b.n_segments, b.n_labels, seg + 1).hits
+                                                   : ckpt_final_hits(b.ckpt, n, b.n_segments, b.n_labels);
     uint32_t A = zaddress(active ? cv.x[j] : 0, active ? cv.y[j] : 0, active ? cv.z[j] : 0, a.zl);
     uint64_t s = active ? cv.seed[j] : 1ull;
-    double e = active ? cv.energy[j] : 0.0;
+    double e = active ? cv.energy[j] : 0.0;   // idle lanes contribute 0
+    double Q = 0.0;
     uint32_t QB = 0xFFFFFFFFu;
     uint4 QV = make_uint4(0u, 0u, 0u, 0u);
-    uint32_t h[kMaxLabels];
-    double G[kMaxLabels];
-    const int n_labels = b.n_labels;
-    static_for<kMaxLabels>([&](auto lc) {
-        constexpr int l = decltype(lc)::value;
-        h[l] = (active && l < n_labels) ? cv.hits[(uint64_t)l * n + j] : 0u;
-        G[l] = 0.0;
-    });
     const char *words = reinterpret_cast<const char *>(a.zwords);
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
+    double *my_r = racc + threadIdx.x;
+    ZEntry T;
+    zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
+    auto step = [&]() {
+        const uint32_t c = zmove_entry<WRAPFIX>(A, T);
+        zchunk_fill_n<128>(words, c, QB, QV);
+        zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
+        if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
+            const uint32_t lab = zlabel(a.zlabels, c);
+            e *= ldelta[lab];
+            __hip_atomic_fetch_add(my_r + lab * blockDim.x, Q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            A = c;
+        }
+    };
     const uint32_t e0 = seg * b.every;
     const uint32_t e1 = (e0 + b.every < b.echoes) ? e0 + b.every : b.echoes;
+    const uint32_t spe = b.steps_per_echo;
     for (uint32_t echo = e0 + 1; echo <= e1; ++echo) {
-        for (uint32_t st = 0; st < b.steps_per_echo; ++st) {
-            int32_t dp;
-            const uint32_t c = zpropose<WRAPFIX>(s, A, tab_bytes, fix_bytes, dp);
-            zchunk_fill(words, c, QB, QV);
-            if (!zchunk_bit(c, QV)) {
-                A = c;
-            } else {
-                const uint32_t lab = zlabel(a.zlabels, c);
-                e *= ldelta[lab];
-                static_for<kMaxLabels>([&](auto lc) {
-                    constexpr int l = decltype(lc)::value;
-                    h[l] += (lab == (uint32_t)l) ? 1u : 0u;
-                });
-            }
+        uint32_t st = 0;
+        for (; st + 2 <= spe; st += 2) {
+            step();
+            step();
         }
-        const double ge = b.grad_echo[echo] * e;
-        static_for<kMaxLabels>([&](auto lc) {
-            constexpr int l = decltype(lc)::value;
-            G[l] += ge * (double)h[l];
-        });
+        if (st < spe)
+            step();
+        Q += b.grad_echo[echo] * e;
     }
+    const int n_labels = b.n_labels;
     static_for<kMaxLabels>([&](auto lc) {
         constexpr int l = decltype(lc)::value;
-        const double v = wave_sum(active ? G[l] : 0.0);
+        double G = 0.0;
+        if (active && l < n_labels)
+            G = (double)h_end[(uint64_t)l * n + j] * Q - my_r[l * blockDim.x];
+        const double v = wave_sum(G);
         if (lane == 0)
             lds[l][wave] = v;
     });
@@ -174,7 +190,7 @@ __global__ __launch_bounds__(256) void zbackward_kernel(const WalkArgs a, const 
 hipError_t launch_backward(const WalkArgs &a, const BackwardArgs &b, int bc, uint32_t threads, hipStream_t st)
 {
     if (a.zwords) {
-        if (!a.zlabels || threads != 256)
+        if (!a.zlabels || threads != kZBackThreads)
             return hipErrorInvalidValue;
         const dim3 grid((uint32_t)((b.n + threads - 1) / threads), b.n_segments);
         if (a.zl.wrapfix)

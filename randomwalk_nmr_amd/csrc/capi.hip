@@ -1398,7 +1398,7 @@ rwnmr_status pfgse_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w
 
 extern "C" uint64_t rwnmr_ckpt_bytes(uint64_t n, uint32_t n_segments, uint32_t n_labels)
 {
-    return rwnmr::ckpt_segment_bytes(n, n_labels) * (uint64_t)n_segments;
+    return rwnmr::ckpt_total_bytes(n, n_segments, n_labels);
 }
 
 extern "C" rwnmr_status rwnmr_cpmg_forward_ckpt(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
@@ -1440,6 +1440,9 @@ extern "C" rwnmr_status rwnmr_cpmg_forward_ckpt(rwnmr_ctx *c, const rwnmr_image 
         if (s != RWNMR_OK)
             return s;
     }
+    // the per-label hits at the end (the replay of the last segment reads them)
+    RW_HIP(c, hipMemcpyAsync(rwnmr::ckpt_final_hits(ckpt, n, nseg, nl), lh, 4ull * nl * n, hipMemcpyDeviceToDevice,
+                             c->stream));
     return RWNMR_OK;
 }
 
@@ -1460,7 +1463,21 @@ extern "C" rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *c, const rwnmr_image *img
         return fail(c, RWNMR_ERR_INVALID, "backward: unknown boundary condition");
     const uint32_t nl = (uint32_t)img->n_labels;
     const uint32_t nseg = (echoes + every - 1) / every;
-    const uint32_t threads = 256;
+    WalkArgs a;
+    memset(&a, 0, sizeof(a));
+    a.blocks = img->blocks;
+    a.labels_blocked = img->labels_blocked;
+    a.W = img->W;
+    a.H = img->H;
+    a.D = img->D;
+    a.bC = img->bC;
+    a.bCR = img->bC * img->bR;
+    a.layout = img->layout;
+    a.label_delta = label_delta;
+    a.n_labels = (int32_t)nl;
+    a.n = n;
+    attach_zimage(c, img, bc, 2, a);
+    const uint32_t threads = a.zwords ? rwnmr::kZBackThreads : 256u;
     const uint64_t bx = (n + threads - 1) / threads;
     if (bx > 0x7FFFFFFFull || nseg > 65535)
         return fail(c, RWNMR_ERR_INVALID, "backward: grid too large");
@@ -1476,20 +1493,6 @@ extern "C" rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *c, const rwnmr_image *img
     b.partial_stride = (uint64_t)nseg * bx;
     RW_HIP(c, c->partials.reserve(8ull * nl * b.partial_stride + 8));
     b.partials = c->partials.as<double>();
-    WalkArgs a;
-    memset(&a, 0, sizeof(a));
-    a.blocks = img->blocks;
-    a.labels_blocked = img->labels_blocked;
-    a.W = img->W;
-    a.H = img->H;
-    a.D = img->D;
-    a.bC = img->bC;
-    a.bCR = img->bC * img->bR;
-    a.layout = img->layout;
-    a.label_delta = label_delta;
-    a.n_labels = (int32_t)nl;
-    a.n = n;
-    attach_zimage(c, img, bc, 2, a);
     hipEvent_t t0 = timing_begin(c);
     RW_HIP(c, rwnmr::launch_backward(a, b, bc, threads, c->stream));
     timing_end(c, t0);
