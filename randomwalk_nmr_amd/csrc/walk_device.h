@@ -39,12 +39,36 @@ template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
+// v + v of lane ^ 16 (ROW16) or lane ^ 32, by the gfx950 permlane swaps:
+// with both operands v, the swap returns v with rows 1 and 3 (lanes 32-63)
+// replaced by rows 0 and 2 (lanes 0-31), and v with rows 0 and 2 (lanes
+// 0-31) replaced by rows 1 and 3 (lanes 32-63); their sum is the butterfly
+// sum, with the lower lane's value first in both lanes of a pair.
+template <bool ROW16>
+__device__ __forceinline__ double xor_rows_sum(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const auto l = ROW16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                         : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = ROW16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                         : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_f64(double v);
+
+// Sum of v over the 64 lanes, the same in every lane: lane ^ 1, lane ^ 2,
+// half-row mirror and row mirror by DPP, then lane ^ 16 and lane ^ 32 by the
+// permlane swaps (18 VALU, no LDS).  Fixed order, so deterministic.  The
+// whole wave must be active.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, kWave);
-    return v;
+    v += dpp_f64<0xB1, 0xF>(v);
+    v += dpp_f64<0x4E, 0xF>(v);
+    v += dpp_f64<0x141, 0xF>(v);
+    v += dpp_f64<0x140, 0xF>(v);
+    v = xor_rows_sum<true>(v);
+    return xor_rows_sum<false>(v);
 }
 
 // v from the lane given by the DPP control CTRL (row_mirror 0x140: lane
@@ -102,9 +126,8 @@ __device__ __forceinline__ double wave_reduce_scatter8(double (&v)[8])
     wave_halve<0x4E, 1>(v, (lane & 2) != 0);    // lane ^ 2: bit 1
     double r = v[0];
     r += dpp_f64<0xB1>(r);                      // lane ^ 1
-    r += __shfl_xor(r, 16, kWave);
-    r += __shfl_xor(r, 32, kWave);
-    return r;
+    r = xor_rows_sum<true>(r);                  // lane ^ 16
+    return xor_rows_sum<false>(r);              // lane ^ 32
 }
 
 // Image geometry as plain values (uniform across the grid).
