@@ -210,6 +210,30 @@ def test_cpmg_labels_unfolded_and_folded_match_oracle(ctx, bc, dims, unfolded):
 
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
+@pytest.mark.parametrize("shape", [(256, 1), (1024, 1), (512, 2), (256, 4)])
+def test_cpmg_labels_launch_shapes_match_oracle(ctx, bc, shape):
+    """Per-label CPMG on the unfolded image for every launch shape: one
+    walker per lane keeps the per-label counts in LDS, wider lanes in
+    registers; both bit-exact."""
+    print(f"[test] labels launch={shape} bc={bc}", flush=True)
+    solid, seeds, x, y, z, _ = _case(30, 21, 17, 3.0, 0.3, 2600, 17)
+    labels = R.voronoi_labels(solid, 16, 8, 5)
+    ld = np.linspace(0.95, 0.999, 8)
+    ctx.set_launch(*shape)
+    try:
+        img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
+        g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=19, steps_per_echo=6, bc=bc, label_delta=ld,
+                       label_hits=True)
+    finally:
+        ctx.set_launch(256, 1)
+    o = O.walk(O.bitblock_pack(solid), (30, 21, 17), x, y, z, seeds, echoes=19, steps_per_echo=6, bc=bc,
+               labels=labels, label_delta=ld, label_hits=True)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+    assert np.array_equal(g["label_hits"], o["label_hits"].T)
+
+
+@pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("shape", [(256, 1), (512, 2), (256, 4)])
 @pytest.mark.parametrize("spe", [4, 7])
 def test_internal_field_cpmg_matches_oracle(ctx, bc, shape, spe):
