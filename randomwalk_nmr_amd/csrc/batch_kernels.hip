@@ -41,7 +41,9 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         if (t >= b.n_tiles)
             break;
         const BatchTile bt = b.tiles[t];
-        const BatchInstance I = b.inst[__builtin_amdgcn_readfirstlane(bt.inst)];
+        // fields read through a reference (a copy of the whole descriptor
+        // went through scratch)
+        const BatchInstance &I = b.inst[__builtin_amdgcn_readfirstlane(bt.inst)];
         WalkArgs a;
         a.blocks = I.blocks;
         a.labels_blocked = nullptr;
