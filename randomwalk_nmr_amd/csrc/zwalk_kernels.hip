@@ -107,24 +107,35 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     for (uint32_t i = 0; i < p.n_delta; ++i) {
         uint32_t h = 0;
         uint32_t left = p.steps_inc[i];
+        auto step = [&]() {
+            int32_t dp;
+            uint32_t c;
+            if (NOFLUX) {
+                c = zmove_noflux(A, T, dp);
+            } else {
+                c = zmove_entry<WRAPFIX>(A, T);
+                dp = (int32_t)T.dp;
+            }
+            zchunk_fill_n<128>(words, c, QB, QV);
+            zdirection<WRAPFIX, 2, NOFLUX>(s, tab_bytes, fix_bytes, T, keys);
+            // two short divergent paths instead of selects
+            if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
+                h += 1u;
+            } else {
+                A = c;
+                P += dp;
+            }
+        };
         while (left) {
             const uint32_t ns = left < kZFlush - since ? left : kZFlush - since;
-            for (uint32_t st = 0; st < ns; ++st) {
-                int32_t dp;
-                uint32_t c;
-                if (NOFLUX) {
-                    c = zmove_noflux(A, T, dp);
-                } else {
-                    c = zmove_entry<WRAPFIX>(A, T);
-                    dp = (int32_t)T.dp;
-                }
-                zchunk_fill_n<128>(words, c, QB, QV);
-                zdirection<WRAPFIX, 2, NOFLUX>(s, tab_bytes, fix_bytes, T, keys);
-                const uint32_t wall = zchunk_wall<128>(c, QV);
-                A = wall ? A : c;
-                P += wall ? 0 : dp;
-                h += wall;
+            // two steps per trip (the pipelined entry alternates registers)
+            uint32_t st = 0;
+            for (; st + 2 <= ns; st += 2) {
+                step();
+                step();
             }
+            if (st < ns)
+                step();
             since += ns;
             left -= ns;
             if (since == kZFlush)
