@@ -8,7 +8,7 @@ echo), mirror boundaries, FP64 energies and echo sums.  One bench step = one
 full CPMG run (walkers reset to their start state, then 1.2e11 walker-steps)
 with all inputs resident in HBM.
 
-Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+Prints one JSON line.  --gpus above 1 is refused: the library runs on one GPU.
 
 --workload selects the other measured configurations (one JSON line each):
   small   128^3 CPMG, 10k walkers, 2000 echoes, plus the 128-bin Laplace
@@ -75,7 +75,7 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_calibration}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported; this build runs on one GPU "
+        print(f"bench.py: --gpus {args.gpus} is not supported; the library runs on one GPU "
               "(walkers would shard across GPUs with one all-reduce of the echo sums)", file=sys.stderr)
         sys.exit(2)
 

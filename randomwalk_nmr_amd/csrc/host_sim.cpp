@@ -15,7 +15,7 @@
 namespace {
 
 // splitmix64 (Steele, Lea, Flood 2014): the generator of the synthetic
-// images (image content is this build's own; the reference reads images).
+// images (image content is generated here; the reference reads images).
 struct SplitMix64 {
     uint64_t s;
     explicit SplitMix64(uint64_t seed) : s(seed) {}

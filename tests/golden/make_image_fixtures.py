@@ -5,7 +5,7 @@ B/NMR_Simulation/NMR_Simulation.cpp:577-617 binarises) and its bitBlock words
 packed by an independent numpy packer written here from the layout of
 B/BitBlock/bitBlock.h:113-134.
 
-Run in a checkout where /root/reference exists; the GPU box only reads the
+Run in a checkout where /root/reference exists; GPU test runs only read the
 committed .npz.
 """
 import os
