@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     double Q = 0.0;
     uint32_t QB = 0xFFFFFFFFu;
     uint4 QV = make_uint4(0u, 0u, 0u, 0u);
-    const char *words = reinterpret_cast<const char *>(a.zwords);
+    const zv4i zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
     double *my_r = racc + threadIdx.x;
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
     auto step = [&]() {
         const uint32_t c = zmove_entry<WRAPFIX>(A, T);
-        zchunk_fill_n<128>(words, c, QB, QV);
+        zchunk_fill_buf(zrsrc, c, QB, QV);
         zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
         if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
             const uint32_t lab = zlabel(a.zlabels, c);

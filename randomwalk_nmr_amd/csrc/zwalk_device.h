@@ -247,6 +247,36 @@ __device__ __forceinline__ void zchunk_fill_n(const char *words, uint32_t c, uin
     }
 }
 
+// Buffer descriptor over the unfolded image's 16-byte chunks (GFX9 V#:
+// base address, stride 16, num_records = chunk count, 32-bit data format)
+// for an indexed buffer load: the chunk load takes the chunk id itself as
+// the index (no address shift), and ids at or past the chunk count read
+// zeros instead of memory.
+typedef int32_t zv4i __attribute__((ext_vector_type(4)));
+__device__ zv4i zbuffer_load_v4(zv4i rsrc, int32_t vindex, int32_t voffset, int32_t soffset, int32_t aux)
+    __asm("llvm.amdgcn.struct.buffer.load.v4i32");
+
+__device__ __forceinline__ zv4i zchunk_rsrc(const uint32_t *words, const ZLayout &zl)
+{
+    const uint64_t p = (uint64_t)(uintptr_t)words;
+    zv4i r;
+    r.x = (int32_t)(uint32_t)p;
+    r.y = (int32_t)(((uint32_t)(p >> 32) & 0xFFFFu) | (16u << 16));
+    r.z = (int32_t)(zl.bits > 7 ? (1u << (zl.bits - 7)) : 1u);   // chunks: 128 addresses each
+    r.w = 0x00020000;
+    return r;
+}
+
+__device__ __forceinline__ void zchunk_fill_buf(const zv4i &rsrc, uint32_t c, uint32_t &QB, uint4 &QV)
+{
+    const uint32_t cid = c >> 7;
+    if (cid != QB) {
+        const zv4i v = zbuffer_load_v4(rsrc, (int32_t)cid, 0, 0, 0);
+        QV = make_uint4((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w);
+        QB = cid;
+    }
+}
+
 template <int CW>
 __device__ __forceinline__ uint32_t zchunk_wall(uint32_t c, const uint4 &QV)
 {
@@ -302,6 +332,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     constexpr bool LH_LDS = EMODE == 2 && K == 1;
     constexpr int NL = (EMODE == 2 && !LH_LDS) ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
+    const zv4i zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
     const uint64_t n = a.n;
     const uint32_t echoes = a.echoes, spe = a.steps_per_echo;
@@ -404,7 +435,10 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         } else if (CACHE == 64 || CACHE == 128) {
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                zchunk_fill_n<CACHE == 64 ? 64 : 128>(words, C[k], QB[k], QV[k]);
+                if (CACHE == 128)
+                    zchunk_fill_buf(zrsrc, C[k], QB[k], QV[k]);
+                else
+                    zchunk_fill_n<64>(words, C[k], QB[k], QV[k]);
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;

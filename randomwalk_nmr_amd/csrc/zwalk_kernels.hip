@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     uint4 QV = make_uint4(0u, 0u, 0u, 0u);
     int32_t P = 0;
     uint32_t since = 0;
-    const char *words = reinterpret_cast<const char *>(a.zwords);
+    const zv4i zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
     // the next direction is drawn a step ahead, during the chunk load (the
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
                 c = zmove_entry<WRAPFIX>(A, T);
                 dp = (int32_t)T.dp;
             }
-            zchunk_fill_n<128>(words, c, QB, QV);
+            zchunk_fill_buf(zrsrc, c, QB, QV);
             zdirection<WRAPFIX, 2, NOFLUX>(s, tab_bytes, fix_bytes, T, keys);
             // two short divergent paths instead of selects
             if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
