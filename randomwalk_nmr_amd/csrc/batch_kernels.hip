@@ -77,6 +77,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         a.zl = I.zl;
         a.zcache = 2;
         a.znoflux = 0;   // noflux instances run the folded walk here
+        a.zmflags = 0;   // mirror instances keep their unfolded start positions
         double *part = I.partials + bt.tile;
         if (I.zwords) {
             // the previous tile's table reads ended at its last block barrier

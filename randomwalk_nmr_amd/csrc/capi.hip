@@ -436,12 +436,18 @@ const uint32_t *zlabels_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
 // Use the unfolded-image kernel for this walk when one applies.  noflux_ok:
 // the caller's kernel also runs noflux walks on the periodic unfolded image
 // (the CPMG, map and PFGSE walks, with the pipelined forms 3 and 4 and every
-// image size >= 2).
-void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, WalkArgs &a, bool noflux_ok = false)
+// image size >= 2).  mflags_ok: the caller's kernel starts mirror walkers in
+// the first octant of the mirror image with reflection flags (the CPMG and
+// map walks, forms 3 and 4, both then on the byte-key table), so each launch
+// begins with the walkers' image lines inside one eighth of the mirror
+// image.
+void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, WalkArgs &a, bool noflux_ok = false,
+                   bool mflags_ok = false)
 {
     a.zwords = nullptr;
     a.zlabels = nullptr;
     a.znoflux = 0;
+    a.zmflags = 0;
     if (emode < 0 || emode > 2 || a.shift != 0)
         return;
     int zbc = bc;
@@ -459,6 +465,7 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
         a.zcache = c->unfolded >= 2 ? (uint32_t)(c->unfolded - 1) : 0u;
         a.zlabels = zlab;
         a.znoflux = bc == rwnmr::kNoflux ? 1u : 0u;
+        a.zmflags = (bc == rwnmr::kMirror && mflags_ok && c->unfolded >= 3) ? 1u : 0u;
     }
 }
 
@@ -552,7 +559,7 @@ rwnmr_status walk_local(rwnmr_ctx *c, const rwnmr_image *img, const WalkArgs &a,
         const rwnmr::SortedState dst = state_of(sets[which]);
         which ^= 1;
         if (a.zwords)
-            RW_HIP(c, rwnmr::launch_zkeys(cur.x, cur.y, cur.z, n, a.zl, keys, idx, c->stream));
+            RW_HIP(c, rwnmr::launch_zkeys(cur.x, cur.y, cur.z, n, a.zl, (int)a.zmflags, keys, idx, c->stream));
         else
             RW_HIP(c, rwnmr::launch_morton_keys(bc, cur.x, cur.y, cur.z, n, img->W, img->H, img->D, keys, idx,
                                                 c->stream));
@@ -676,7 +683,7 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
             }
         }
     }
-    attach_zimage(c, img, bc, emode, a, true);
+    attach_zimage(c, img, bc, emode, a, true, true);
     const uint32_t blocks = rwnmr::walk_grid_blocks(w->n, c->shape, (int)shift);
     a.partial_stride = blocks;
     double *sums = nullptr;
@@ -1064,7 +1071,7 @@ rwnmr_status rwnmr_map_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *
     WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
     a.energy = nullptr;  // the map walk never touches energies
     a.echoes = 1;
-    attach_zimage(c, img, bc, 0, a, true);
+    attach_zimage(c, img, bc, 0, a, true, true);
     uint64_t left = steps;
     while (left > 0) {
         const uint32_t now = (uint32_t)std::min<uint64_t>(left, 1ull << 30);

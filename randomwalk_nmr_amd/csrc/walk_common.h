@@ -315,16 +315,21 @@ RW_HD ZFlip zdir_noflux(const ZLayout &L, uint32_t idx)
     return f;
 }
 
-RW_HD ZDir zdir_make(const ZLayout &L, uint32_t idx)
+// flags: bit a reverses the address move along axis a (the mirror image of
+// the mirror tiling, u -> 2d - 1 - u, is the tiling itself, so a mirror walker
+// may sit at the reflected address with its moves reversed along that axis);
+// dp keeps the drawn direction's unbounded displacement.
+RW_HD ZDir zdir_make(const ZLayout &L, uint32_t idx, uint32_t flags = 0u)
 {
     const uint32_t r = zdir_residue(idx);
-    const int ax = move_axis(r), sg = move_sign(r);
+    const int ax = move_axis(r), sg0 = move_sign(r);
+    const int sg = ((flags >> ax) & 1u) ? -sg0 : sg0;
     const uint32_t m = L.m[ax];
     ZDir d;
     d.m = m;
     d.nm = ~m;
     d.add = sg > 0 ? (m & (~m + 1u)) : m;
-    d.dp = sg * (ax == 0 ? 1 : (ax == 1 ? 1 << 10 : 1 << 20));
+    d.dp = sg0 * (ax == 0 ? 1 : (ax == 1 ? 1 << 10 : 1 << 20));
     const uint32_t k = (uint32_t)__builtin_popcount(m);
     const uint32_t p = L.period[ax];
     if ((1ull << k) == (uint64_t)p) {

@@ -41,6 +41,7 @@ struct WalkArgs {
     ZLayout zl;                      // its dilated address layout
     uint32_t zcache;                 // chunk cache in registers: 0 none, 1 16-byte (word select), 2/3 16-byte, pipelined
     uint32_t znoflux;                // 1: noflux walk on the periodic unfolded image (zl of that image)
+    uint32_t zmflags;                // 1: mirror walk started in the first octant with reflection flags
     const uint32_t *zlabels;         // unfolded solid labels, 4 bits per voxel (per-label walks)
     // internal-field CPMG (emode 3): field[x + y W + z W H], phase step gammatau,
     // per-walker surviving fraction and phase (in/out)

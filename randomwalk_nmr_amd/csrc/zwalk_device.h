@@ -48,6 +48,25 @@ __device__ __forceinline__ void zbuild_table(const ZLayout &zl, uint4 *tab, uint
     }
 }
 
+// Mirror walks on the unfolded image start every walker in the first octant
+// (its folded position; flag bit a set when its unbounded coordinate is in an
+// odd mirror image along axis a) and read the direction table of its flags:
+// 8 copies of the 6-entry table (and of the wrap fix-up table) at
+// kZFlagStride-byte steps (fix-up: half that), so the walker's table offset
+// is flags x kZFlagStride, constant over the launch.  Threads 0..47 write;
+// the caller syncs.
+constexpr uint32_t kZFlagStride = 128u;
+
+__device__ __forceinline__ void zbuild_table_flags(const ZLayout &zl, uint4 *tab, uint2 *fix)
+{
+    if (threadIdx.x < 48) {
+        const uint32_t flags = threadIdx.x / 6u, idx = threadIdx.x % 6u;
+        const ZDir d = zdir_make(zl, idx, flags);
+        tab[flags * (kZFlagStride / 16u) + idx] = make_uint4(d.m, d.add, (uint32_t)d.dp, d.nm);
+        fix[flags * (kZFlagStride / 16u) + idx] = make_uint2(d.wrapv, d.fixv);
+    }
+}
+
 // Wide direction table: one 16-byte entry per (state & 1, byte sum of the
 // state), index (state & 1) * 2048 + sb, so the step reads its entry without
 // reducing sb mod 3 (sb <= 2040; 64 KiB of LDS).  Periods that are powers of
@@ -100,9 +119,10 @@ struct ZEntry {
     uint32_t tgt, add2, dp2, mp, mm;  // noflux: face test and the turned-around move (ZFlip)
 };
 
+// fo: the walker's table offset (mirror walks with reflection flags), else 0.
 template <bool WRAPFIX, int TAB = 0, bool NOFLUX = false>
 __device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, const char *fix_bytes, ZEntry &e,
-                                           const uint8_t *keys = nullptr)
+                                           const uint8_t *keys = nullptr, uint32_t fo = 0u)
 {
     static_assert(!NOFLUX || (TAB == 2 && !WRAPFIX), "noflux: byte-key table, no wrap fix-up");
     static_assert(!(TAB == 1 && WRAPFIX), "the wide table has no wrap fix-up");
@@ -125,6 +145,7 @@ __device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, c
         const uint32_t q = __umul24(sb, 2731u) >> 13;
         off = (uint32_t)__mul24((int)q, -96) + ((sb << 5) | ((lo & 1u) << 4));
     }
+    off += fo;
     const uint4 t = *reinterpret_cast<const uint4 *>(tab_bytes + off);
     e.m = t.x;
     e.add = t.y;
@@ -321,7 +342,8 @@ __device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
 // EMODE 2 with K == 1 keeps the per-label hit counts in LDS (lh: kMaxLabels
 // x blockDim entries, label-major; one LDS add per hit instead of a compare
 // and add per label), else in registers.
-template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false>
+// MFLAGS: mirror walk with reflection flags (tab / fix: zbuild_table_flags).
+template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false, bool MFLAGS = false>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
                                            const double *ldelta = nullptr, const uint8_t *keys = nullptr,
@@ -339,6 +361,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     const int n_labels = a.n_labels;
 
     uint32_t A[K], C[K], Hc[K];
+    uint32_t FO[K];   // MFLAGS: the walker's direction table offset (flags x kZFlagStride)
     uint32_t QB[K];   // chunk cache: chunk id (address >> 6 or >> 7) held in QV
     uint4 QV[K];
     int32_t P[K], DP[K], X[K], Y[K], Z[K];
@@ -365,7 +388,18 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         E[k] = act[k] ? (a.energy ? a.energy[j] : 1.0) : 0.0;   // idle lanes add 0 to echo sums
         Hc[k] = (act[k] && a.hits) ? a.hits[j] : 0u;
         DL[k] = EMODE == 1 ? ((act[k] && a.penalty) ? a.penalty[j] : a.delta_uniform) : 1.0;
-        A[k] = zaddress(X[k], Y[k], Z[k], a.zl);
+        if (MFLAGS) {
+            // start at the folded position (first octant of the mirror image)
+            // with the reflection flags of the unbounded one
+            const int32_t w = (int32_t)(a.zl.period[0] >> 1), h = (int32_t)(a.zl.period[1] >> 1),
+                          d = (int32_t)(a.zl.period[2] >> 1);
+            A[k] = zaddress(fold_mirror(X[k], w), fold_mirror(Y[k], h), fold_mirror(Z[k], d), a.zl);
+            FO[k] = kZFlagStride * ((uint32_t)mirror_flag(X[k], w) | ((uint32_t)mirror_flag(Y[k], h) << 1) |
+                                    ((uint32_t)mirror_flag(Z[k], d) << 2));
+        } else {
+            A[k] = zaddress(X[k], Y[k], Z[k], a.zl);
+            FO[k] = 0u;
+        }
         P[k] = 0;
         QB[k] = 0xFFFFFFFFu;   // no chunk: addresses >> 7 stay below 2^25
         QV[k] = make_uint4(0u, 0u, 0u, 0u);
@@ -406,7 +440,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     if (PIPE)
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys);
+            zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys, FO[k]);
         });
     auto step = [&]() {
         static_for<K>([&](auto kc) {
@@ -442,7 +476,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys);
+                zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys, FO[k]);
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;

@@ -31,7 +31,7 @@ hipError_t launch_zlabels_build(const uint8_t *labels_blocked, const BrickLayout
                                 const ZLayout &zl, uint32_t *out, hipStream_t st);
 
 // keys[i] = unfolded address of walker i >> 5 (its 32-bit word), iota[i] = i.
-hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n, const ZLayout &zl,
+hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n, const ZLayout &zl, int mflags,
                         uint32_t *keys, uint32_t *iota, hipStream_t st);
 
 // PFGSE walk + measurement on the unfolded image (a.zwords / a.zl).

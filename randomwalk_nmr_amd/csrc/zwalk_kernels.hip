@@ -11,13 +11,13 @@
 
 namespace rwnmr {
 
-template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false>
+template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false, bool MFLAGS = false>
 __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kEchoCols];
-    __shared__ uint4 tab[TAB == 1 ? kZWideEntries : (NOFLUX ? 12 : 6)];
+    __shared__ uint4 tab[TAB == 1 ? kZWideEntries : (MFLAGS ? 8 * kZFlagStride / 16 : (NOFLUX ? 12 : 6))];
     __shared__ uint4 small[6];
-    __shared__ uint2 fix[6];
+    __shared__ uint2 fix[MFLAGS ? 8 * kZFlagStride / 16 : 6];
     __shared__ uint8_t keys[TAB == 2 ? kZKeyEntries : 4];
     __shared__ double ldelta[kMaxLabels];
     extern __shared__ uint32_t zlh[];   // EMODE 2, K == 1: per-label hit counts (kMaxLabels x blockDim)
@@ -26,6 +26,8 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
         return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
     if (NOFLUX)
         zbuild_table_noflux(a.zl, tab);
+    else if (MFLAGS)
+        zbuild_table_flags(a.zl, tab, fix);
     else
         zbuild_table(a.zl, TAB == 1 ? small : tab, fix);
     if (TAB == 2)
@@ -37,9 +39,9 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
         zbuild_table_wide(small, tab);
         __syncthreads();
     }
-    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX>(a, tile, a.partials ? a.partials + tile : nullptr,
-                                                      a.partial_stride, lds, tab, fix, ldelta, keys,
-                                                      EMODE == 2 && K == 1 ? zlh : nullptr);
+    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS>(a, tile, a.partials ? a.partials + tile : nullptr,
+                                                              a.partial_stride, lds, tab, fix, ldelta, keys,
+                                                              EMODE == 2 && K == 1 ? zlh : nullptr);
 }
 
 // PFGSE on the unfolded image: the walk between diffusion times counts hits
@@ -255,13 +257,21 @@ __global__ __launch_bounds__(256) void zlabels_build_kernel(const uint8_t *label
     out[w] = word;
 }
 
+// mflags: the key of the folded position (where a mirror walk with
+// reflection flags starts the walker)
 __global__ __launch_bounds__(256) void zkeys_kernel(const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n,
-                                                   ZLayout zl, uint32_t *keys, uint32_t *iota)
+                                                   ZLayout zl, int mflags, uint32_t *keys, uint32_t *iota)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    keys[i] = zaddress(x[i], y[i], z[i], zl) >> 5;
+    int32_t xi = x[i], yi = y[i], zi = z[i];
+    if (mflags) {
+        xi = fold_mirror(xi, (int32_t)(zl.period[0] >> 1));
+        yi = fold_mirror(yi, (int32_t)(zl.period[1] >> 1));
+        zi = fold_mirror(zi, (int32_t)(zl.period[2] >> 1));
+    }
+    keys[i] = zaddress(xi, yi, zi, zl) >> 5;
     iota[i] = (uint32_t)i;
 }
 
@@ -284,23 +294,24 @@ hipError_t launch_zlabels_build(const uint8_t *labels_blocked, const BrickLayout
 }
 
 hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n, const ZLayout &zl,
-                        uint32_t *keys, uint32_t *iota, hipStream_t st)
+                        int mflags, uint32_t *keys, uint32_t *iota, hipStream_t st)
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(zkeys_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, x, y, z, n, zl, keys, iota);
+    hipLaunchKernelGGL(zkeys_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, x, y, z, n, zl, mflags, keys,
+                       iota);
     return hipGetLastError();
 }
 
-template <int EMODE, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false>
+template <int EMODE, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false, bool MFLAGS = false>
 static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
     // EMODE 2 with one walker per lane keeps the per-label hit counts in LDS
     const size_t lh_bytes = EMODE == 2 ? (size_t)kMaxLabels * ls.threads * sizeof(uint32_t) : 0;
     switch (ls.k) {
-    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), lh_bytes, st, a); break;
-    case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX, CACHE, TAB, NOFLUX>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS>), dim3(blocks), dim3(ls.threads), lh_bytes, st, a); break;
+    case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((zwalk_kernel<EMODE, 4, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -315,6 +326,9 @@ static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32
 {
     if (a.znoflux)   // noflux on the periodic image: pipelined byte-key walk only
         return a.zcache >= 2 ? launch_z<EMODE, false, 128, 2, true>(a, ls, blocks, st) : hipErrorInvalidValue;
+    if (a.zmflags && a.zcache >= 2)   // mirror, walkers folded to the first octant with reflection flags
+        return a.zl.wrapfix ? launch_z<EMODE, true, 128, 2, false, true>(a, ls, blocks, st)
+                            : launch_z<EMODE, false, 128, 2, false, true>(a, ls, blocks, st);
     if (a.zl.wrapfix) {
         switch (a.zcache) {
         case 0: return launch_z<EMODE, true, 0, 0>(a, ls, blocks, st);
