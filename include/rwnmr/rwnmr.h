@@ -96,8 +96,9 @@ rwnmr_status rwnmr_ctx_set_locality(rwnmr_ctx *ctx, int32_t enable, uint32_t res
  * 3 (default) and 4: the chunk tested by one shift and the next direction
  * drawn a step ahead, through the 4 KiB byte-key table (3) or the 64 KiB
  * wide direction table (4).  Modes 3 and 4 also run noflux walks (image
- * sides >= 2) on the periodic unfolded image.  Results are identical in
- * every mode. */
+ * sides >= 2) on the periodic unfolded image, and start mirror CPMG / map
+ * walkers at their folded positions with reflection flags (mirrored
+ * direction tables, byte-key form).  Results are identical in every mode. */
 rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *ctx, int32_t mode);
 
 /* Upload a packed image in the reference bitBlock layout: block index
