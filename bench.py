@@ -269,6 +269,12 @@ def wl_small(args):
     X, Y, Z, SD = X0.clone(), Y0.clone(), Z0.clone(), S0.clone()
     EN = torch.ones(N, dtype=torch.float64, device=dev)
     SUMS = torch.zeros(E + 1, dtype=torch.float64, device=dev)
+    if args.launch:
+        ctx.set_launch(*(int(v) for v in args.launch.split(",")))
+    elif not args.no_autotune:
+        tpb, k, ms = ctx.autotune(img, X0, Y0, Z0, S0, steps_per_echo=spe, bc="mirror", trial_echoes=E, delta=delta)
+        log(f"autotune: threads {tpb} x walkers/lane {k} ({ms:.3f} ms per {E} echoes)")
+    tpb, k = ctx.get_launch()
 
     def run():
         X.copy_(X0), Y.copy_(Y0), Z.copy_(Z0), SD.copy_(S0), EN.fill_(1.0)
@@ -297,7 +303,8 @@ def wl_small(args):
     _line({"metric": "walker-steps/s, CPMG 128^3 10k walkers", "value": steps / sec, "unit": "walker-steps/s",
            "ms_per_run": sec * 1e3, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"CPMG {S}^3 sphere pack (r 8, porosity {phi:.3f}), {N} walkers, {E} echoes x "
-                                  f"{spe} steps, mirror, voxel {voxel} um, rho {rho} um/s, T2b {t2b} ms"},
+                                  f"{spe} steps, mirror, voxel {voxel} um, rho {rho} um/s, T2b {t2b} ms",
+                      "launch": f"{tpb}x{k}"},
            "laplace": {"bins": 128, "lambda": lam, "ms": lap_ms, "t2_peak_ms": t2_peak},
            "cpu_baseline": {"value": steps / secsN, "unit": "walker-steps/s", "cores": threads, "kind": "port",
                             "sample": "all 10,000 walkers x 2000 echoes (reference loop order, digit-sum mod6)",
