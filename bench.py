@@ -404,6 +404,8 @@ def wl_batch(args):
     import randomwalk_nmr_amd as R
     rng = np.random.default_rng(args.seed)
     ctx = R.Context(0)
+    if args.launch:   # the batched kernel takes walkers per lane from it
+        ctx.set_launch(*(int(v) for v in args.launch.split(",")))
     dev = torch.device("cuda:0")
     voxel, D0, E = 1.0, 2.5, 4000
     spe = int(round(0.2 / R.time_step(voxel, D0)))
