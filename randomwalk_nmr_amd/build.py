@@ -8,6 +8,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -40,28 +41,30 @@ def _run(cmd):
     return out
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source (always) and link the shared library."""
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every source (always, `jobs` compilers at once; 0: up to 8) and
+    link the shared library."""
     os.makedirs(OUT_DIR, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(OUT_DIR, src + ".o")
-        out = _run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
-        if verbose and out.strip():
-            print(out)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
         objs.append(obj)
     for src in CXX_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(OUT_DIR, src + ".o")
-        out = _run(["g++", *CXX_FLAGS, "-I", os.path.join(HERE, "..", "include"), "-c", path, "-o", obj])
-        if verbose and out.strip():
-            print(out)
+        cmds.append(["g++", *CXX_FLAGS, "-I", os.path.join(HERE, "..", "include"), "-c", path, "-o", obj])
         objs.append(obj)
+    workers = jobs or max(1, min(8, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for out in pool.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-fopenmp", "-lz"])
     os.replace(tmp, LIB)
