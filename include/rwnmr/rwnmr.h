@@ -162,6 +162,25 @@ rwnmr_status rwnmr_map_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers
 rwnmr_status rwnmr_cpmg_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
                             uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums);
 
+/* CPMG that allocates nothing: rwnmr_cpmg_run for device-resident walkers
+ * (on_device must be 1) and a device echo_sums (echoes+1, nullable) that
+ * calls no hipMalloc / hipFree and no allocating library routine.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Its echo partials and error word live
+ * in a fixed 16 MiB device array of the library, one per GPU, held until the
+ * process ends: a run needs at most 16 MiB / 16 B = 1,048,560 blocks of
+ * walkers (threads per block x walkers per lane each) and is split into
+ * launches of as many echoes as the array holds, in which case w->energy
+ * must be non-null.  It never builds an unfolded image (it uses one an
+ * earlier call built, else the folded kernels) and never re-sorts walkers;
+ * walker results equal rwnmr_cpmg_run's bit for bit, echo sums to rounding.
+ * Asynchronous; rwnmr_ctx_synchronize reports a noflux walker that started
+ * outside the image.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (two contexts, two threads) race. */
+rwnmr_status rwnmr_cpmg_run_noalloc(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                    uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums);
+
 /* Internal-field CPMG (NMR_cpmg::image_simulation_cuda with an internal
  * field, kernels CPMG_walk_{noflux,periodic,mirror}_field,
  * B/NMR_Simulation/NMR_cpmg_cuda.cpp:146-293, 409-567, 683-841, driver

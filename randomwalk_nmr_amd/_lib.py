@@ -109,6 +109,7 @@ SIGNATURES = {
     "rwnmr_bitblock_pack": (I32, [P, I32, I32, I32, P]),
     "rwnmr_map_run": (I32, [P, P, ct.POINTER(Walkers), U64, I32, U32]),
     "rwnmr_cpmg_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
+    "rwnmr_cpmg_run_noalloc": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
     "rwnmr_cpmg_run_labels": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P, P]),
     "rwnmr_pfgse_run_sampled": (I32, [P, P, ct.POINTER(Walkers), P, P, P, P, U32, P, P, P, U32, I32, D, I32, U32, P,
                                       U32, P, P, P]),

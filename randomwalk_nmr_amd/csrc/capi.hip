@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -68,6 +69,39 @@ __global__ void fill_f64_kernel(double *p, double v, uint64_t n)
         p[i] = v;
 }
 
+// Device scratch of the no-allocation CPMG entry point
+// (rwnmr_cpmg_run_noalloc): the error word in the first 256 bytes, the echo
+// partials after it.  One copy per GPU, held until the process ends.  Host
+// code never passes the array's name to a kernel or a copy: each GPU's copy
+// has its own device address, read once per device by hipGetSymbolAddress
+// (scratch_address) and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = 16u << 20;
+constexpr size_t kScratchFlagBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices];
+
+// Device address of `scratch` on `device`, which must be the current device;
+// resolved on the first call per device.  Empty string on success.
+std::string scratch_address(int device, void **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices)
+        return "device index beyond the scratch table";
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return std::string();
+}
+
 }  // namespace
 
 struct rwnmr_ctx {
@@ -85,7 +119,12 @@ struct rwnmr_ctx {
     // batched CPMG: descriptors, tile list, reduction rows, queue head
     DevBuf batch_inst, batch_tiles, batch_rows, batch_counter;
     DevBuf flag;  // uint32 error word
-    uint32_t *host_flag = nullptr;  // pinned readback of flag
+    uint32_t *host_flag = nullptr;  // pinned readback: [0] of flag, [1] of the scratch error word
+    // this device's copy of `scratch` (no-allocation entry point); null with
+    // scratch_err set when its address could not be read
+    void *scratch = nullptr;
+    std::string scratch_err;
+    bool scratch_flag_pending = false;  // a no-allocation run's error word is still to be checked
     // optional HIP-event timing of the walk / PFGSE kernels on this stream
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -213,15 +252,17 @@ rwnmr_status clear_flag(rwnmr_ctx *c)
 // Synchronise and turn a raised kernel flag into RWNMR_ERR_WALKER.
 rwnmr_status sync_and_check(rwnmr_ctx *c)
 {
-    if (!c->flag.p) {
-        RW_HIP(c, hipStreamSynchronize(c->stream));
-        timing_collect(c);
-        return RWNMR_OK;
-    }
-    RW_HIP(c, hipMemcpyAsync(c->host_flag, c->flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    c->host_flag[0] = 0;
+    c->host_flag[1] = 0;
+    if (c->flag.p)
+        RW_HIP(c, hipMemcpyAsync(c->host_flag, c->flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    const bool scratch_flag = c->scratch_flag_pending;
+    if (scratch_flag)
+        RW_HIP(c, hipMemcpyAsync(c->host_flag + 1, c->scratch, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     RW_HIP(c, hipStreamSynchronize(c->stream));
+    c->scratch_flag_pending = false;
     timing_collect(c);
-    if (*c->host_flag)
+    if (c->host_flag[0] || c->host_flag[1])
         return fail(c, RWNMR_ERR_WALKER, "a walker started outside the image (noflux boundary)");
     return RWNMR_OK;
 }
@@ -359,12 +400,12 @@ WalkArgs base_args(const rwnmr_image *img, const WalkerView &v, const rwnmr_walk
 // Unfolded image for a periodic or mirror walk, built on the context stream
 // on first use; null when the walk runs on the folded kernels (option off,
 // noflux, more than 2^32 voxels in the unfolded volume, or no memory for it).
-const ZImage *zimage_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
+const ZImage *zimage_for(rwnmr_ctx *c, const rwnmr_image *img, int bc, bool build = true)
 {
     if (!c->unfolded || (bc != rwnmr::kPeriodic && bc != rwnmr::kMirror))
         return nullptr;
     ZImage &z = img->z[bc == rwnmr::kMirror ? 1 : 0];
-    if (z.state == 0) {
+    if (z.state == 0 && build) {
         const uint32_t f = bc == rwnmr::kMirror ? 2u : 1u;
         if (!rwnmr::zlayout_make(f * (uint32_t)img->W, f * (uint32_t)img->H, f * (uint32_t)img->D, z.zl)) {
             z.state = -1;
@@ -441,8 +482,10 @@ const uint32_t *zlabels_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
 // map walks, forms 3 and 4, both then on the byte-key table), so each launch
 // begins with the walkers' image lines inside one eighth of the mirror
 // image.
+// build: false uses an unfolded image only when an earlier call built it
+// (callers that must not allocate device memory).
 void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, WalkArgs &a, bool noflux_ok = false,
-                   bool mflags_ok = false)
+                   bool mflags_ok = false, bool build = true)
 {
     a.zwords = nullptr;
     a.zlabels = nullptr;
@@ -457,9 +500,9 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
         zbc = rwnmr::kPeriodic;
     }
     const uint32_t *zlab = nullptr;
-    if (emode == 2 && !(zlab = zlabels_for(c, img, zbc)))
+    if (emode == 2 && (!build || !(zlab = zlabels_for(c, img, zbc))))
         return;
-    if (const ZImage *z = zimage_for(c, img, zbc)) {
+    if (const ZImage *z = zimage_for(c, img, zbc, build)) {
         a.zwords = z->words;
         a.zl = z->zl;
         a.zcache = c->unfolded >= 2 ? (uint32_t)(c->unfolded - 1) : 0u;
@@ -785,12 +828,15 @@ rwnmr_status rwnmr_ctx_create(rwnmr_ctx **out, int32_t device)
         delete c;
         return RWNMR_ERR_HIP;
     }
-    if (hipHostMalloc((void **)&c->host_flag, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&c->host_flag, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return RWNMR_ERR_ALLOC;
     }
-    *c->host_flag = 0;
+    c->host_flag[0] = 0;
+    c->host_flag[1] = 0;
+    // the no-allocation entry point's scratch on this device (the device is current)
+    c->scratch_err = scratch_address(device, &c->scratch);
     *out = c;
     return RWNMR_OK;
 }
@@ -1201,6 +1247,65 @@ rwnmr_status rwnmr_cpmg_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers 
                             uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums)
 {
     return cpmg_common(c, img, w, echoes, steps_per_echo, bc, shift, 1, nullptr, nullptr, echo_sums);
+}
+
+rwnmr_status rwnmr_cpmg_run_noalloc(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                    uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums)
+{
+    rwnmr_status s;
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if ((s = check_walkers(c, img, w, bc, shift)) != RWNMR_OK)
+        return s;
+    if (!w->on_device)
+        return fail(c, RWNMR_ERR_INVALID, "cpmg_run_noalloc: walker arrays must be device memory (on_device = 1)");
+    if (!c->scratch)
+        return fail(c, RWNMR_ERR_HIP, "cpmg_run_noalloc: no device scratch: " + c->scratch_err);
+    uint32_t *flag = static_cast<uint32_t *>(c->scratch);
+    double *part = reinterpret_cast<double *>(static_cast<unsigned char *>(c->scratch) + kScratchFlagBytes);
+    const uint64_t part_doubles = (kScratchBytes - kScratchFlagBytes) / sizeof(double);
+    const uint32_t blocks = rwnmr::walk_grid_blocks(w->n, c->shape, (int)shift);
+    // echoes per launch: rows 0..chunk of [echo][block] partials must fit the scratch
+    uint64_t chunk = echoes ? echoes : 1;
+    if (echo_sums && blocks) {
+        const uint64_t rows = part_doubles / blocks;
+        if (rows < 2)
+            return fail(c, RWNMR_ERR_INVALID,
+                        "cpmg_run_noalloc: " + std::to_string(w->n) + " walkers need " +
+                            std::to_string(16ull * blocks) + " bytes of echo partials per launch; the scratch holds " +
+                            std::to_string(part_doubles * sizeof(double)) + " (at most " +
+                            std::to_string(part_doubles / 2) + " blocks of walkers)");
+        chunk = std::min<uint64_t>(chunk, rows - 1);
+    }
+    if (chunk < echoes && !w->energy && w->n)
+        return fail(c, RWNMR_ERR_INVALID,
+                    "cpmg_run_noalloc: a run of " + std::to_string(echoes) + " echoes takes several launches (" +
+                        std::to_string(chunk) + " echoes each) and needs a device energy array to carry the "
+                        "energies between them");
+    // the error word starts at zero on every call (a fill kernel, no memset node)
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c->stream, flag, 0u, 1ull);
+    RW_HIP(c, hipGetLastError());
+    c->scratch_flag_pending = true;
+    const WalkerView v{w->x, w->y, w->z, w->seed, w->energy, w->hits, w->penalty};
+    WalkArgs a = base_args(img, v, w, shift, flag);
+    a.n_labels = img->n_labels;
+    // an unfolded image is used only when an earlier call built it
+    attach_zimage(c, img, bc, 1, a, true, true, false);
+    a.partial_stride = blocks;
+    a.steps_per_echo = steps_per_echo;
+    uint32_t done = 0;
+    do {
+        const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
+        a.echoes = ne;
+        a.partials = echo_sums ? part : nullptr;
+        RW_HIP(c, launch_any_walk(a, bc, 1, c->shape, c->stream));
+        if (echo_sums)
+            RW_HIP(c, rwnmr::launch_reduce_rows(part, (uint64_t)ne + 1, blocks, echo_sums + done, c->stream));
+        done += ne;
+    } while (done < echoes);
+    return RWNMR_OK;
 }
 
 rwnmr_status rwnmr_cpmg_field_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
