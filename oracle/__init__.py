@@ -19,7 +19,11 @@ Parity status of each oracle routine (how it is pinned, see the tests):
     mt19937 / Lemire placement     pinned (C++ standard value, numpy, libstdc++)
     setup_walkers (point / cubic,  pinned (brute-force reachable sets on tiny
       voxel division)                images, draw counts, seed_and_place)
-    laplace                        pinned (scipy.optimize.nnls, closed forms)
+    laplace                        pinned (scipy.optimize.nnls, closed forms;
+                                     the L-curve curvature against a finite-
+                                     difference geometric curvature, with the
+                                     reference's one denominator substitution,
+                                     TikhonovSolver.cpp:111, checked as such)
 """
 from __future__ import annotations
 
@@ -76,7 +80,7 @@ def lib():
                                      _I64, _I64, _P, _I]
         L.orc_cpmg_reference_loop.restype = _D
         L.orc_cpmg_reference_loop.argtypes = [_P, _I, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P,
-                                              _I64, _I64, _P, _P, _P, _P, _I]
+                                              _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _I]
         L.orc_pfgse.restype = _I
         L.orc_pfgse.argtypes = [_P, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                 _I, _P, _I, _P, _I, _I, _P, _P, _P, _D, _P, _P, _I]
@@ -197,24 +201,26 @@ def walk_field(blocks, dims, x, y, z, seed, *, echoes, steps_per_echo, field, ga
 
 def cpmg_reference_loop(blocks, dims, x, y, z, seed, penalty, *, echoes, steps_per_echo,
                         bc="mirror", shift=0, use_digit_sum=True, threads=1):
-    """Reference-order CPU CPMG (timing baseline).  Returns (seconds, echo_sums, x, y, z)."""
+    """Reference-order CPU CPMG (timing baseline).  Returns a dict: seconds
+    (the step loop only), echo_sums and the final x, y, z, seed, energy, hits."""
     W, H, D = dims
     n = len(x)
     xs, ys, zs = (_c(a, _np.int32) for a in (x, y, z))
     sd = _c(seed, _np.uint64)
     pen = _c(_np.broadcast_to(penalty, (n,)), _np.float64)
-    es = _np.zeros(echoes + 1)
-    xo = _np.zeros(n, _np.int32)
-    yo = _np.zeros(n, _np.int32)
-    zo = _np.zeros(n, _np.int32)
+    out = dict(echo_sums=_np.zeros(echoes + 1), x=_np.zeros(n, _np.int32), y=_np.zeros(n, _np.int32),
+               z=_np.zeros(n, _np.int32), seed=_np.zeros(n, _np.uint64), energy=_np.zeros(n),
+               hits=_np.zeros(n, _np.uint32))
     blocks = _c(blocks, _np.uint64)
     secs = lib().orc_cpmg_reference_loop(_ptr(blocks), W, H, D, BC_CODES.get(bc, bc), shift,
                                          int(use_digit_sum), n, _ptr(xs), _ptr(ys), _ptr(zs),
-                                         _ptr(sd), _ptr(pen), echoes, steps_per_echo, _ptr(es),
-                                         _ptr(xo), _ptr(yo), _ptr(zo), threads)
+                                         _ptr(sd), _ptr(pen), echoes, steps_per_echo, _ptr(out["echo_sums"]),
+                                         *(_ptr(out[k]) for k in ("x", "y", "z", "seed", "energy", "hits")),
+                                         threads)
     if secs < 0:
         raise MemoryError("orc_cpmg_reference_loop")
-    return secs, es, xo, yo, zo
+    out["seconds"] = secs
+    return out
 
 
 def pfgse(blocks, dims, x, y, z, seed, *, steps_inc, kx, ky, kz, voxel, bc="mirror", shift=0,

@@ -95,7 +95,12 @@ def prune(data, num_pruning: int, start: int, slices, use_log: bool):
 def lcurve(K, b, min_lambda=-4.0, max_lambda=2.0, num_lambdas=512):
     """Hansen's L-curve curvature over log-spaced lambdas
     (B/Laplace/src/TikhonovSolver.cpp:14-116).  Returns dict with lambdas,
-    solution_norms, residual_norms, curvature, best_lambda, best_index."""
+    solution_norms, residual_norms, curvature, best_lambda, best_index.
+
+    The curvature keeps the reference's denominator (lam^2 eta^2 + rho^2)^1.5
+    (TikhonovSolver.cpp:111), where Hansen's closed form has lam^4 eta^2, so
+    that the same lambda is chosen; with lam^4 the expression is exactly twice
+    the geometric curvature of (ln rho, ln eta)."""
     U, s, _ = np.linalg.svd(np.asarray(K, float), full_matrices=False)
     keep = 0
     while keep < len(s) and s[keep] != 0.0:

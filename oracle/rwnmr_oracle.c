@@ -500,7 +500,8 @@ double orc_cpmg_reference_loop(const uint64_t *blocks, int W, int H, int D, int 
                                int use_digit_sum, int64_t n, const int32_t *x, const int32_t *y,
                                const int32_t *z, const uint64_t *seed, const double *penalty,
                                int64_t echoes, int64_t steps_per_echo, double *echo_sums,
-                               int32_t *x_out, int32_t *y_out, int32_t *z_out, int threads)
+                               int32_t *x_out, int32_t *y_out, int32_t *z_out, uint64_t *seed_out,
+                               double *energy_out, uint32_t *hits_out, int threads)
 {
     orc_walker_rec *wk = (orc_walker_rec *)malloc(sizeof(orc_walker_rec) * (size_t)(n > 0 ? n : 1));
     if (!wk)
@@ -559,6 +560,14 @@ double orc_cpmg_reference_loop(const uint64_t *blocks, int W, int H, int D, int 
             y_out[w] = wk[w].y;
             z_out[w] = wk[w].z;
         }
+    for (int64_t w = 0; w < n; ++w) {
+        if (seed_out)
+            seed_out[w] = wk[w].seed;
+        if (energy_out)
+            energy_out[w] = wk[w].energy;
+        if (hits_out)
+            hits_out[w] = wk[w].hits;
+    }
     free(wk);
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }

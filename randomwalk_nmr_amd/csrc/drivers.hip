@@ -360,8 +360,9 @@ rwnmr_status rwnmr_pfgse_simulate(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_
                 lhs[k] = ratio > 1.0e-06 ? log(ratio) : log(1.0e-06);
             }
             uint32_t used = n_k;
+            // the window is first clamped to the gradient points (applyThreshold, NMR_pfgse.cpp:139)
             if (p->threshold_lhs > 0.0 && p->threshold_lhs < 1.0)
-                used = rwnmr_pfgse_threshold_lhs(lhs, n_k, p->threshold_lhs, p->threshold_window);
+                used = rwnmr_pfgse_threshold_lhs(lhs, n_k, p->threshold_lhs, std::min(p->threshold_window, n_k));
             else if (p->threshold_samples > 1 && p->threshold_samples <= n_k)
                 used = p->threshold_samples;
             out->points[(size_t)q * n_times + i] = used;

@@ -278,6 +278,12 @@ bool nnls(const Mat &A, const Vec &b, Vec &x, double tol, int max_iter)
     return false;
 }
 
+// L-curve curvature at lambda from the SVD (B/Laplace/src/TikhonovSolver.cpp:82-116):
+// eta = |x|^2, rho = |r|^2 restricted to range(U), deta = d eta / d lambda.  The
+// denominator is the reference's (lambda^2 eta^2 + rho^2)^1.5 (TikhonovSolver.cpp:111);
+// Hansen's closed form has lambda^4 eta^2 there and is then twice the geometric
+// curvature of (ln rho, ln eta).  The reference's form is kept so that the same
+// lambda is chosen.
 double hansen_curvature(const Vec &s, const Vec &uatb, const Vec &q, const Vec &q2, double lambda, double &sol,
                         double &res)
 {

@@ -92,6 +92,43 @@ def test_pfgse_simulate_matches_oracle(ctx, threshold):
         assert abs(g["D_msd"][i] - dmsd) <= 1e-14 * dmsd
 
 
+def _lhs_threshold_points(lhs, value, window):
+    """applyThreshold with type "lhs" (NMR_pfgse.cpp:133-143): the window is
+    clamped to the gradient points (:139), then setThresholdFromLHSValue
+    (:381-402) takes the first point from window-1 on whose LHS is below
+    log(value), or the last point."""
+    window = min(window, len(lhs))
+    idx, greater, lv = window - 1, True, math.log(value)
+    while idx < len(lhs) and greater:
+        if lhs[idx] < lv:
+            greater = False
+        else:
+            idx += 1
+    return idx - 1 if greater else idx
+
+
+@pytest.mark.parametrize("value,window", [(0.5, 3), (0.5, 50), (1e-9, 50)])
+def test_pfgse_simulate_lhs_threshold_clamps_window(ctx, value, window):
+    print(f"[test] pfgse_simulate lhs threshold value={value} window={window}", flush=True)
+    solid, seeds, x, y, z = _case(9, n=2000)
+    img = R.Image.from_solid(ctx, solid)
+    deltas = np.array([0.5, 2.0])
+    K = 12
+    kx = np.stack([np.linspace(0.0, 2.5 / math.sqrt(D0 * d), K) for d in deltas])
+    zero = np.zeros_like(kx)
+    g = R.pfgse_simulate(ctx, img, x, y, z, seeds, exposure_times=deltas, kx=kx, ky=zero, kz=zero, voxel_size=VOX,
+                         D0=D0, bulk_time=T2B, apply_bulk=True, bc="mirror", rho=8.0, threshold_lhs=value,
+                         threshold_window=window)
+    for i, d in enumerate(deltas):
+        n_used = _lhs_threshold_points(list(g["lhs"][0, i]), value, window)
+        assert g["points"][0, i] == n_used
+        rhs, lhs = g["rhs"][0, i, :n_used], g["lhs"][0, i, :n_used]
+        b = float(np.dot(rhs, lhs) / np.dot(rhs, rhs))
+        assert abs(g["D_sat"][0, i] - b) <= 1e-9 * abs(b)
+    if window > K:
+        assert np.all(g["points"] <= K - 1)
+
+
 @pytest.mark.parametrize("scale", ["log", "manual"])
 def test_multitau_matches_oracle(ctx, scale):
     print(f"[test] multitau scale={scale}", flush=True)

@@ -152,7 +152,7 @@ def test_inversion_fixed_lambda_matches_oracle():
     assert abs(mean_log - (0.35 * np.log10(40) + 0.65 * np.log10(700))) < 0.15
 
 
-def test_lcurve_matches_oracle_and_finite_difference_curvature():
+def test_lcurve_matches_oracle_and_explicit_tikhonov_norms():
     times = np.arange(1, 1001) * 0.5
     d = _decay(times) + 1e-4 * np.sin(times)
     inv = RL.Inverter(times, num_lambdas=256)
@@ -180,6 +180,54 @@ def test_lcurve_matches_oracle_and_finite_difference_curvature():
         assert np.isclose(np.linalg.norm(xl), lc["solution_norms"][il], rtol=1e-6)
         r2 = np.sum((K @ xl - bb) ** 2) - b_perp2
         assert np.isclose(np.sqrt(max(r2, 0.0)), lc["residual_norms"][il], rtol=1e-4)
+
+
+def _tikhonov_norms(K, b, lam, b_perp2):
+    """eta = |x|^2 and rho = |K x - b|^2 - |b outside range(K)|^2 of the explicit
+    Tikhonov solution x = argmin |K x - b|^2 + lam^2 |x|^2 (augmented QR solve)."""
+    A = np.vstack([K, lam * np.eye(K.shape[1])])
+    x, *_ = np.linalg.lstsq(A, np.concatenate([b, np.zeros(K.shape[1])]), rcond=None)
+    return float(x @ x), float(np.sum((K @ x - b) ** 2) - b_perp2)
+
+
+def test_lcurve_curvature_pinned_by_geometric_curvature():
+    """TikhonovSolver.cpp:82-116 writes Hansen's closed-form L-curve curvature
+    with lambda^2 eta^2 in the denominator where Hansen has lambda^4 eta^2.
+    Pin: (1) Hansen's form, -2 eta rho / eta' (lam^2 eta' rho + 2 lam eta rho +
+    lam^4 eta eta') / (lam^4 eta^2 + rho^2)^1.5, is exactly twice the geometric
+    curvature of the curve (ln rho, ln eta) (the curve (ln |r|, ln |x|) has
+    half the coordinates), computed here by central differences in ln lambda
+    from explicit Tikhonov solves; (2) the oracle's curvature is Hansen's form
+    with that one substitution and nothing else, so its mult * numerator is
+    pinned by (1)."""
+    times = np.linspace(0.5, 300.0, 300)
+    t2 = np.logspace(-1, 4, 64)
+    K = np.exp(-times[:, None] / t2[None, :])
+    rng = np.random.default_rng(5)
+    b = 0.6 * np.exp(-times / 30.0) + 0.4 * np.exp(-times / 400.0) + 1e-3 * rng.standard_normal(times.size)
+    U, s, _ = np.linalg.svd(K, full_matrices=False)
+    b_perp2 = float(b @ b - np.sum((U.T @ b) ** 2))
+    lc = OL.lcurve(K, b, num_lambdas=512)
+    h = 2e-3
+    checked = 0
+    for il in range(40, 512, 37):
+        lam = lc["lambdas"][il]
+        pts = [_tikhonov_norms(K, b, lam * np.exp(j * h), b_perp2) for j in (-1, 0, 1)]
+        if min(r for _, r in pts) <= 1e-10 * float(b @ b):
+            continue   # residual at rounding level: the curve is numerically flat there
+        a = np.log([r for _, r in pts])
+        c = np.log([e for e, _ in pts])
+        at, ct_ = (a[2] - a[0]) / (2 * h), (c[2] - c[0]) / (2 * h)
+        att, ctt = (a[2] - 2 * a[1] + a[0]) / h ** 2, (c[2] - 2 * c[1] + c[0]) / h ** 2
+        k_geo = (at * ctt - att * ct_) / (at * at + ct_ * ct_) ** 1.5
+        eta, rho = lc["solution_norms"][il] ** 2, lc["residual_norms"][il] ** 2
+        assert np.isclose(eta, pts[1][0], rtol=1e-6) and np.isclose(rho, pts[1][1], rtol=1e-4)
+        den_ref = (lam ** 2 * eta ** 2 + rho ** 2) ** 1.5
+        den_hansen = (lam ** 4 * eta ** 2 + rho ** 2) ** 1.5
+        k_hansen = lc["curvature"][il] * den_ref / den_hansen
+        assert abs(k_hansen - 2.0 * k_geo) <= 2e-4 * abs(2.0 * k_geo), (il, lam, k_hansen, k_geo)
+        checked += 1
+    assert checked >= 8
 
 
 def test_stejskal_tanner_hand_vectors():
