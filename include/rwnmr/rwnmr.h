@@ -206,6 +206,22 @@ rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr
                                    uint32_t echoes, uint32_t steps_per_echo, int32_t bc, uint32_t shift,
                                    const double *label_delta, uint32_t *label_hits, double *echo_sums);
 
+/* CPMG with n_sets sets of per-label relaxivity evaluated on one trajectory
+ * (relaxation never changes a trajectory): set_label_delta holds
+ * [n_sets][n_labels] deltas and echo_sums receives [n_sets][echoes+1] sums,
+ * each set's equal to an rwnmr_cpmg_run_labels call with that set's deltas
+ * from the same start state (per-walker energies bit for bit, sums to
+ * rounding).  Sets are walked 8 at a time, each walker carrying 8 energies;
+ * every group of 8 replays the trajectories from the start state.  The
+ * walkers end as after one run; w->energy (nullable) returns the last
+ * group's first set's energies; w->hits is not updated.  Host or device
+ * placement of every array follows w->on_device.  No reference counterpart:
+ * it scores a whole population of the relaxivity fit's genetic algorithm
+ * (B/GA/ga_core.cpp:463-490 evaluates one individual per full simulation). */
+rwnmr_status rwnmr_cpmg_run_label_sets(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                       uint32_t steps_per_echo, int32_t bc, uint32_t n_sets,
+                                       const double *set_label_delta, double *echo_sums);
+
 /* One CPMG instance of a batch: its image, device-resident walkers
  * (on_device must be 1; per-walker penalty or delta_uniform), echo count,
  * steps per echo, boundary, and a device echo_sums array of echoes+1. */

@@ -111,6 +111,7 @@ SIGNATURES = {
     "rwnmr_cpmg_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
     "rwnmr_cpmg_run_noalloc": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
     "rwnmr_cpmg_run_labels": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P, P]),
+    "rwnmr_cpmg_run_label_sets": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P]),
     "rwnmr_pfgse_run_sampled": (I32, [P, P, ct.POINTER(Walkers), P, P, P, P, U32, P, P, P, U32, I32, D, I32, U32, P,
                                       U32, P, P, P]),
     "rwnmr_pfgse_sample_stats": (I32, [P, U32, U32, P, U32, P, P, P, P, P, ct.POINTER(D), ct.POINTER(D),
@@ -144,6 +145,7 @@ SIGNATURES = {
     "rwnmr_cpmg_forward_ckpt": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, P, U32, P, P]),
     "rwnmr_cpmg_backward": (I32, [P, P, P, U64, U32, U32, I32, P, U32, P, P]),
     "rwnmr_ckpt_bytes": (U64, [U64, U32, U32]),
+    "rwnmr_probe_gather": (I32, [P, U64, U32, ct.POINTER(D), ct.POINTER(D)]),
     # sequence drivers (include/rwnmr/rwnmr_drivers.h)
     "rwnmr_cpmg_simulate": (I32, [P, P, ct.POINTER(Walkers), ct.POINTER(CpmgParams), U32, P, P,
                                   ct.POINTER(U32)]),

@@ -112,6 +112,15 @@ class Context:
                                         int(trial_echoes), ct.byref(t), ct.byref(k), ct.byref(ms)), self.handle)
         return t.value, k.value, ms.value
 
+    def probe_gather(self, table_bytes: int, loads_per_thread: int = 256):
+        """Random 16-byte loads per second from a device table of
+        table_bytes (rwnmr_probe_gather).  Returns (loads_per_s, kernel_ms)."""
+        rate, ms = ct.c_double(), ct.c_double()
+        code = load().rwnmr_probe_gather(self.handle, int(table_bytes), int(loads_per_thread), ct.byref(rate),
+                                         ct.byref(ms))
+        check(code, self.handle)
+        return rate.value, ms.value
+
     @property
     def stream(self) -> int:
         return load().rwnmr_ctx_stream(self.handle) or 0
@@ -326,6 +335,37 @@ def cpmg_run(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo,
                                 _ptr(es))
     check(code, ctx.handle)
     return dict(x=x, y=y, z=z, seed=seed, energy=energy, hits=hits, echo_sums=es, label_hits=lh)
+
+
+def cpmg_run_label_sets(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo, set_label_delta,
+                        bc="mirror", energy=None, echo_sums=None, sync=True):
+    """Several per-label relaxivity sets on one trajectory
+    (rwnmr_cpmg_run_label_sets): set_label_delta is [n_sets, n_labels].
+    Host inputs are copied and a dict (x, y, z, seed, energy, echo_sums
+    [n_sets, echoes+1]) returned; device inputs are updated in place and
+    echo_sums must be a device tensor of n_sets * (echoes+1) doubles."""
+    dev = _is_dev(x)
+    n = len(x)
+    L = load()
+    if dev:
+        _torch_sync()
+        n_sets = set_label_delta.shape[0]
+        w = _walker_struct(x, y, z, seed, energy, None, None, 1.0, n, 1)
+        check(L.rwnmr_cpmg_run_label_sets(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo, bc_code(bc),
+                                          n_sets, _ptr(set_label_delta), _ptr(echo_sums)), ctx.handle)
+        if sync:
+            ctx.synchronize()
+        return dict(x=x, y=y, z=z, seed=seed, energy=energy, echo_sums=echo_sums)
+    sld = np.ascontiguousarray(set_label_delta, np.float64)
+    n_sets = sld.shape[0]
+    x, y, z = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+    seed = _host(seed, np.uint64)
+    energy = np.ones(n) if energy is None else _host(energy, np.float64, n)
+    es = np.zeros((n_sets, echoes + 1))
+    w = _walker_struct(x, y, z, seed, energy, None, None, 1.0, n, 0)
+    check(L.rwnmr_cpmg_run_label_sets(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo, bc_code(bc),
+                                      n_sets, _ptr(sld), _ptr(es)), ctx.handle)
+    return dict(x=x, y=y, z=z, seed=seed, energy=energy, echo_sums=es)
 
 
 def map_run(ctx: Context, img: Image, x, y, z, seed, *, steps, bc="mirror", shift=0, hits=None, sync=True):

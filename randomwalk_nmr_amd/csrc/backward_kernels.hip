@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
         zchunk_fill_buf(zrsrc, c, QB, QV);
         zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
         if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
-            const uint32_t lab = zlabel(a.zlabels, c);
+            const uint32_t lab = zlabel_at(a, c);
             e *= ldelta[lab];
             __hip_atomic_fetch_add(my_r + lab * blockDim.x, Q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {

@@ -116,6 +116,9 @@ struct rwnmr_ctx {
     // internal-field CPMG staging (host-pointer calls)
     DevBuf field, p_alive, phase;
     DevBuf msamples;   // PFGSE per-sample sums (host-pointer calls)
+    // label-set CPMG: the walkers' start state (replayed per group of 8 sets)
+    // and the 8 set energies carried between launches
+    DevBuf lset_x, lset_y, lset_z, lset_seed, lset_energy, lset_energies;
     // batched CPMG: descriptors, tile list, reduction rows, queue head
     DevBuf batch_inst, batch_tiles, batch_rows, batch_counter;
     DevBuf flag;  // uint32 error word
@@ -489,6 +492,7 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
 {
     a.zwords = nullptr;
     a.zlabels = nullptr;
+    a.zlfold = 0;
     a.znoflux = 0;
     a.zmflags = 0;
     if (emode < 0 || emode > 2 || a.shift != 0)
@@ -507,6 +511,8 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
         a.zl = z->zl;
         a.zcache = c->unfolded >= 2 ? (uint32_t)(c->unfolded - 1) : 0u;
         a.zlabels = zlab;
+        // mirror image of power-of-two periods: labels read at the first-octant address
+        a.zlfold = (zlab && bc == rwnmr::kMirror && !z->zl.wrapfix) ? 1u : 0u;
         a.znoflux = bc == rwnmr::kNoflux ? 1u : 0u;
         a.zmflags = (bc == rwnmr::kMirror && mflags_ok && c->unfolded >= 3) ? 1u : 0u;
     }
@@ -861,7 +867,8 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
     for (DevBuf *b : {&c->x, &c->y, &c->z, &c->seed, &c->energy, &c->hits, &c->penalty, &c->x0, &c->y0, &c->z0,
                       &c->partials, &c->sums, &c->label_delta, &c->label_hits, &c->steps_inc, &c->kvec,
                       &c->set_delta, &c->flag, &c->batch_inst, &c->batch_tiles, &c->batch_rows,
-                      &c->batch_counter, &c->lkeys, &c->lidx, &c->lsort_tmp, &c->field, &c->p_alive, &c->phase, &c->msamples})
+                      &c->batch_counter, &c->lkeys, &c->lidx, &c->lsort_tmp, &c->field, &c->p_alive, &c->phase, &c->msamples,
+                      &c->lset_x, &c->lset_y, &c->lset_z, &c->lset_seed, &c->lset_energy, &c->lset_energies})
         b->release();
     for (int i = 0; i < 8; ++i) {
         c->sortA[i].release();
@@ -1328,6 +1335,157 @@ rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_w
                                    uint32_t *label_hits, double *echo_sums)
 {
     return cpmg_common(c, img, w, echoes, steps_per_echo, bc, shift, 2, label_delta, label_hits, echo_sums);
+}
+
+rwnmr_status rwnmr_cpmg_run_label_sets(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                       uint32_t steps_per_echo, int32_t bc, uint32_t n_sets,
+                                       const double *set_label_delta, double *echo_sums)
+{
+    rwnmr_status s;
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if ((s = check_walkers(c, img, w, bc, 0)) != RWNMR_OK)
+        return s;
+    if (!img->labels_blocked || img->n_labels <= 0)
+        return fail(c, RWNMR_ERR_INVALID, "label sets: image has no solid labels");
+    if (n_sets == 0 || !set_label_delta || !echo_sums)
+        return fail(c, RWNMR_ERR_INVALID, "label sets: need n_sets > 0, set_label_delta and echo_sums");
+    const uint64_t n = w->n;
+    const size_t nl = (size_t)img->n_labels, rows = (size_t)echoes + 1;
+    if (n == 0) {
+        std::vector<double> zero(rows * n_sets, 0.0);
+        if (w->on_device)
+            RW_HIP(c, hipMemcpyAsync(echo_sums, zero.data(), 8 * zero.size(), hipMemcpyHostToDevice, c->stream));
+        else
+            memcpy(echo_sums, zero.data(), 8 * zero.size());
+        return w->on_device ? RWNMR_OK : sync_and_check(c);
+    }
+    WalkerView v;
+    if ((s = clear_flag(c)) != RWNMR_OK)
+        return s;
+    if ((s = stage_in(c, w, v)) != RWNMR_OK)
+        return s;
+    const hipMemcpyKind in_kind = w->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    RW_HIP(c, c->label_delta.reserve(8 * nl * n_sets));
+    RW_HIP(c, hipMemcpyAsync(c->label_delta.p, set_label_delta, 8 * nl * n_sets, in_kind, c->stream));
+    double *sums = echo_sums;
+    if (!w->on_device) {
+        RW_HIP(c, c->sums.reserve(8 * rows * n_sets));
+        sums = c->sums.as<double>();
+    }
+    WalkArgs a = base_args(img, v, w, 0, c->flag.as<uint32_t>());
+    a.n_labels = img->n_labels;
+    attach_zimage(c, img, bc, 2, a, true, true);
+    // the start state, replayed for every set (fallback) or group of 8 sets
+    auto save_start = [&]() -> rwnmr_status {
+        RW_HIP(c, c->lset_x.reserve(4 * n));
+        RW_HIP(c, c->lset_y.reserve(4 * n));
+        RW_HIP(c, c->lset_z.reserve(4 * n));
+        RW_HIP(c, c->lset_seed.reserve(8 * n));
+        RW_HIP(c, hipMemcpyAsync(c->lset_x.p, v.x, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(c->lset_y.p, v.y, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(c->lset_z.p, v.z, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(c->lset_seed.p, v.seed, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+        if (v.energy) {
+            RW_HIP(c, c->lset_energy.reserve(8 * n));
+            RW_HIP(c, hipMemcpyAsync(c->lset_energy.p, v.energy, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+        }
+        return RWNMR_OK;
+    };
+    auto restore_start = [&]() -> rwnmr_status {
+        RW_HIP(c, hipMemcpyAsync(v.x, c->lset_x.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(v.y, c->lset_y.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(v.z, c->lset_z.p, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+        RW_HIP(c, hipMemcpyAsync(v.seed, c->lset_seed.p, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+        if (v.energy)
+            RW_HIP(c, hipMemcpyAsync(v.energy, c->lset_energy.p, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+        return RWNMR_OK;
+    };
+    if (!a.zwords || !a.zlabels) {
+        // no unfolded image: one labelled walk per set from the same start state
+        if (n_sets > 1 && (s = save_start()) != RWNMR_OK)
+            return s;
+        // the set whose energies are returned (the first of the last group of 8) walks last
+        const uint32_t last = 8u * ((n_sets - 1u) / 8u);
+        for (uint32_t i = 0; i < n_sets; ++i) {
+            const uint32_t q = i + 1 < n_sets ? (i < last ? i : i + 1) : last;
+            if (i > 0 && (s = restore_start()) != RWNMR_OK)
+                return s;
+            rwnmr_walkers ws = *w;
+            ws.on_device = 1;
+            ws.x = v.x;
+            ws.y = v.y;
+            ws.z = v.z;
+            ws.seed = v.seed;
+            ws.energy = v.energy;
+            ws.hits = nullptr;
+            ws.penalty = nullptr;
+            if ((s = cpmg_common(c, img, &ws, echoes, steps_per_echo, bc, 0, 2,
+                                 c->label_delta.as<double>() + (size_t)q * nl, nullptr, sums + (size_t)q * rows)) !=
+                RWNMR_OK)
+                return s;
+        }
+    } else {
+        const uint32_t threads = c->shape.threads;
+        const uint32_t blocks = (uint32_t)((n + threads - 1) / threads);
+        // echoes per launch: 8 sets x (chunk + 1) rows of partials within the bound
+        const uint64_t max_rows = std::max<uint64_t>(2, kMaxPartialBytes / (8ull * 8ull * blocks));
+        const uint64_t chunk = std::min<uint64_t>(echoes ? echoes : 1, max_rows - 1);
+        const uint32_t groups = (n_sets + 7) / 8;
+        RW_HIP(c, c->partials.reserve(8ull * 8ull * (chunk + 1) * blocks + 8));
+        const bool carry = chunk < echoes;
+        if (carry)
+            RW_HIP(c, c->lset_energies.reserve(8ull * 8ull * n));
+        if (groups > 1 && (s = save_start()) != RWNMR_OK)
+            return s;
+        for (uint32_t g = 0; g < groups; ++g) {
+            // every group walks the same trajectories from the start state
+            if (g > 0 && (s = restore_start()) != RWNMR_OK)
+                return s;
+            if (carry) {
+                // the 8 carried set energies start at the walkers' start energies
+                for (int q = 0; q < 8; ++q) {
+                    double *row = c->lset_energies.as<double>() + (size_t)q * n;
+                    if (v.energy)
+                        RW_HIP(c, hipMemcpyAsync(row, v.energy, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+                    else
+                        hipLaunchKernelGGL(fill_f64_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
+                                           c->stream, row, 1.0, n);
+                }
+                RW_HIP(c, hipGetLastError());
+            }
+            const uint32_t gsets = std::min<uint32_t>(8u, n_sets - 8u * g);
+            WalkArgs b = a;
+            b.label_delta = c->label_delta.as<double>() + (size_t)8 * g * nl;
+            b.n_label_sets = gsets;
+            b.partial_stride = blocks;
+            b.steps_per_echo = steps_per_echo;
+            b.set_energy = carry ? c->lset_energies.as<double>() : nullptr;
+            double *part = c->partials.as<double>();
+            uint32_t done = 0;
+            do {
+                const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
+                b.echoes = ne;
+                b.partials = part;
+                hipEvent_t t0 = timing_begin(c);
+                RW_HIP(c, rwnmr::launch_zwalk_sets(b, threads, c->stream));
+                timing_end(c, t0);
+                for (uint32_t q = 0; q < gsets; ++q)
+                    RW_HIP(c, rwnmr::launch_reduce_rows(part + (size_t)q * (ne + 1) * blocks, (uint64_t)ne + 1,
+                                                        blocks, sums + (size_t)(8 * g + q) * rows + done, c->stream));
+                done += ne;
+            } while (done < echoes);
+        }
+    }
+    if (!w->on_device) {
+        RW_HIP(c, hipMemcpyAsync(echo_sums, sums, 8 * rows * n_sets, hipMemcpyDeviceToHost, c->stream));
+        if ((s = stage_out(c, w, v)) != RWNMR_OK)
+            return s;
+        return sync_and_check(c);
+    }
+    return RWNMR_OK;
 }
 
 rwnmr_status rwnmr_pfgse_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, const int32_t *x0,

@@ -300,6 +300,42 @@ __device__ __forceinline__ void stage_echo(double (*lds)[kEchoCols], uint32_t ec
     }
 }
 
+// Echo sums of 8 energy sets per walker: one reduce-scatter gives every lane
+// the wave total of set (lane >> 1) & 7, lanes 0, 2, ..., 14 stage their
+// set's wave total in LDS for kEchoChunk echoes, and the block then writes
+// one partial per (set, echo): out[set * set_stride + row * row_stride].
+// About 7 VALU per set and echo instead of 15 for separate half-wave sums.
+// Fixed order, so deterministic.  All threads of the block call it, with
+// the whole wave active.
+__device__ __forceinline__ void stage_echo_sets8(double (*lds8)[8][kMaxWaves], uint32_t echo, uint32_t last_echo,
+                                                 const double (&v)[8], double *out, uint64_t row_stride,
+                                                 uint64_t set_stride)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    double t[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        t[q] = v[q];
+    const double r = wave_reduce_scatter8(t);
+    const int slot = echo % kEchoChunk;
+    if (lane < 16 && (lane & 1) == 0)
+        lds8[slot][lane >> 1][wave] = r;
+    if (slot == kEchoChunk - 1 || echo == last_echo) {
+        __syncthreads();
+        const int cnt = slot + 1;
+        for (int q = (int)threadIdx.x; q < 8 * cnt; q += (int)blockDim.x) {
+            const int row = q >> 3, set = q & 7;
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w)
+                s += lds8[row][set][w];
+            out[(uint64_t)set * set_stride + ((uint64_t)(echo - slot) + (uint64_t)row) * row_stride] = s;
+        }
+        __syncthreads();
+    }
+}
+
 // PFGSE measurement at diffusion time i for one walker with displacement
 // (dX, dY, dZ) voxels and energies e[0..nsets): block sums of
 // e[q] cos((k . dr) voxel) for every k of the time's k row and every set q,

@@ -43,6 +43,9 @@ struct WalkArgs {
     uint32_t znoflux;                // 1: noflux walk on the periodic unfolded image (zl of that image)
     uint32_t zmflags;                // 1: mirror walk started in the first octant with reflection flags
     const uint32_t *zlabels;         // unfolded solid labels, 4 bits per voxel (per-label walks)
+    uint32_t zlfold;                 // 1: labels read at the first-octant address (mirror, power-of-two periods)
+    uint32_t n_label_sets;           // label-set walks: label_delta holds [n_label_sets][n_labels]
+    double *set_energy;              // label-set walks: nullable [8][n] energies carried between launches
     // internal-field CPMG (emode 3): field[x + y W + z W H], phase step gammatau,
     // per-walker surviving fraction and phase (in/out)
     const double *field;

@@ -326,6 +326,32 @@ __device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
     return (zlabels[c >> 3] >> ((c & 7u) * 4u)) & 0xFu;
 }
 
+// First-octant address of c in the mirror image of power-of-two periods:
+// the image is its own reflection u -> period - 1 - u along every axis, and
+// along an axis of 2^k addresses that reflection complements the axis' bits,
+// so each axis whose top bit is set has all its bits flipped (its top bit
+// then clears).  The voxel there carries the same label, and the label reads
+// of a walk stay in one eighth of the label volume.
+__device__ __forceinline__ uint32_t zfold_first_octant(uint32_t c, const ZLayout &zl)
+{
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const uint32_t m = zl.m[ax];
+        if (m) {
+            const uint32_t top = 31u - (uint32_t)__builtin_clz(m);
+            c ^= (uint32_t)__builtin_amdgcn_sbfe((int)c, top, 1u) & m;
+        }
+    }
+    return c;
+}
+
+// Label of the voxel at address c of a labelled walk (a.zlfold: read at the
+// first-octant address).
+__device__ __forceinline__ uint32_t zlabel_at(const WalkArgs &a, uint32_t c)
+{
+    return zlabel(a.zlabels, a.zlfold ? zfold_first_octant(c, a.zl) : c);
+}
+
 // One tile (blockDim.x * K consecutive walkers from tile * blockDim.x * K)
 // walked for a.echoes x a.steps_per_echo steps on the unfolded image
 // a.zwords / a.zl, with the direction table already in LDS.  EMODE 0 = hit
@@ -343,16 +369,27 @@ __device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
 // x blockDim entries, label-major; one LDS add per hit instead of a compare
 // and add per label), else in registers.
 // MFLAGS: mirror walk with reflection flags (tab / fix: zbuild_table_flags).
-template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false, bool MFLAGS = false>
+// NS == 8 (EMODE 2, K == 1, pipelined chunk cache): label-set walk, each
+// walker carrying 8 energies, one per set of label deltas (ldelta: 8 rows of
+// kMaxLabels, set-major), all starting from the walker's energy; a hit
+// multiplies each by its set's delta of the hit label, in the order of the
+// single-set walk, so each set's energies equal that walk's bit for bit.
+// Echo partials of set q go to part[q * (a.echoes + 1) * part_stride + row
+// * part_stride] (lds8: the staging of stage_echo_sets8); set 0's energy is
+// stored.
+template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false, bool MFLAGS = false,
+          int NS = 1>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
                                            const double *ldelta = nullptr, const uint8_t *keys = nullptr,
-                                           uint32_t *lh = nullptr)
+                                           uint32_t *lh = nullptr, double (*lds8)[8][kMaxWaves] = nullptr)
 {
     constexpr bool WIDE = TAB == 1;
     static_assert(!NOFLUX || CACHE >= 64, "noflux: pipelined chunk-cache walk only");
-    constexpr bool LH_LDS = EMODE == 2 && K == 1;
-    constexpr int NL = (EMODE == 2 && !LH_LDS) ? kMaxLabels : 1;
+    static_assert(NS == 1 || (NS == 8 && EMODE == 2 && K == 1 && CACHE >= 64), "label sets: 8 sets, K == 1");
+    // label-set walks count no per-label hits (a.label_hits is null)
+    constexpr bool LH_LDS = EMODE == 2 && K == 1 && NS == 1;
+    constexpr int NL = (EMODE == 2 && !LH_LDS && NS == 1) ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
     const zv4i zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
@@ -367,6 +404,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     int32_t P[K], DP[K], X[K], Y[K], Z[K];
     uint64_t S[K];
     double E[K], DL[K];
+    double ES[NS];   // label-set walks: one energy per set
     uint32_t LH[K][NL];
     bool act[K];
     static_for<K>([&](auto kc) {
@@ -401,6 +439,11 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             FO[k] = 0u;
         }
         P[k] = 0;
+        if (NS > 1)
+            static_for<NS>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                ES[q] = (act[k] && a.set_energy) ? a.set_energy[(uint64_t)q * n + j] : E[k];
+            });
         QB[k] = 0xFFFFFFFFu;   // no chunk: addresses >> 7 stay below 2^25
         QV[k] = make_uint4(0u, 0u, 0u, 0u);
         static_for<NL>([&](auto lc) {
@@ -509,8 +552,14 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             if (CACHE >= 64 && EMODE == 2) {
                 if (__builtin_expect(wall != 0u, 0)) {
                     Hc[k] += 1u;
-                    const uint32_t lab = zlabel(a.zlabels, C[k]);
-                    E[k] *= ldelta[lab];
+                    const uint32_t lab = zlabel_at(a, C[k]);
+                    if (NS > 1)
+                        static_for<NS>([&](auto qc) {
+                            constexpr int q = decltype(qc)::value;
+                            ES[q] *= ldelta[q * kMaxLabels + lab];
+                        });
+                    else
+                        E[k] *= ldelta[lab];
                     if (LH_LDS) {
                         __hip_atomic_fetch_add(&lh[lab * blockDim.x + threadIdx.x], 1u, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -532,7 +581,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             if (EMODE == 1 && wall)
                 E[k] *= DL[k];
             if (EMODE == 2 && wall) {
-                const uint32_t lab = zlabel(a.zlabels, C[k]);
+                const uint32_t lab = zlabel_at(a, C[k]);
                 E[k] *= ldelta[lab];
                 if (LH_LDS) {
                     __hip_atomic_fetch_add(&lh[lab * blockDim.x + threadIdx.x], 1u, __ATOMIC_RELAXED,
@@ -547,8 +596,15 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         });
     };
 
+    const uint64_t set_stride = (uint64_t)(echoes + 1) * part_stride;
+    auto stage = [&](uint32_t echo) {
+        if constexpr (NS > 1)
+            stage_echo_sets8(lds8, echo, echoes, ES, part, part_stride, set_stride);
+        else
+            stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
+    };
     if (part)
-        stage_echo(lds, 0, echoes, echo_value(), part, part_stride);
+        stage(0);
     uint32_t since = 0;
     for (uint32_t echo = 1; echo <= echoes; ++echo) {
         uint32_t left = spe;
@@ -571,9 +627,11 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             }
         }
         if (part)
-            stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
+            stage(echo);
     }
     flush();
+    if (NS > 1)
+        E[0] = ES[0];
     if (PIPE)
         static_for<K>([&](auto kc) { S[decltype(kc)::value] = xorshift64_inverse(S[decltype(kc)::value]); });
 
@@ -587,6 +645,11 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             a.seed[id] = S[k];
             if (a.energy)
                 a.energy[id] = E[k];
+            if (NS > 1 && a.set_energy)
+                static_for<NS>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    a.set_energy[(uint64_t)q * n + id] = ES[q];
+                });
             if (a.hits)
                 a.hits[id] = Hc[k];
             if (EMODE == 2 && a.label_hits) {

@@ -41,4 +41,9 @@ hipError_t launch_zpfgse(const WalkArgs &a, const PfgseArgs &p, hipStream_t st);
 // and a.label_delta) walk on a.zwords / a.zl; a.shift must be 0.
 hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hipStream_t st);
 
+// Label-set CPMG walk (a.n_label_sets <= 8 rows of a.n_labels deltas in
+// a.label_delta; a.zlabels needed; one walker per lane, `threads` per
+// block); partials [set][echo rows][a.partial_stride] of 8 sets.
+hipError_t launch_zwalk_sets(const WalkArgs &a, uint32_t threads, hipStream_t st);
+
 }  // namespace rwnmr

@@ -44,6 +44,37 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
                                                               EMODE == 2 && K == 1 ? zlh : nullptr);
 }
 
+// Label-set CPMG: 8 sets of label deltas evaluated on one trajectory (one
+// walker per lane; zwalk_tile with NS = 8).  Unused sets (a.n_label_sets <
+// 8) carry deltas of 1.
+template <bool WRAPFIX, bool NOFLUX, bool MFLAGS>
+__global__ __launch_bounds__(1024) void zwalk_sets_kernel(const WalkArgs a)
+{
+    __shared__ double lds[kEchoChunk][kEchoCols];
+    __shared__ double lds8[kEchoChunk][8][kMaxWaves];
+    __shared__ uint4 tab[MFLAGS ? 8 * kZFlagStride / 16 : (NOFLUX ? 12 : 6)];
+    __shared__ uint2 fix[MFLAGS ? 8 * kZFlagStride / 16 : 6];
+    __shared__ uint8_t keys[kZKeyEntries];
+    __shared__ double ldelta[8 * kMaxLabels];
+    const uint64_t tile = tile_of_block(a);
+    if (a.tile_map && tile >= a.n_tiles)
+        return;
+    if (NOFLUX)
+        zbuild_table_noflux(a.zl, tab);
+    else if (MFLAGS)
+        zbuild_table_flags(a.zl, tab, fix);
+    else
+        zbuild_table(a.zl, tab, fix);
+    zbuild_key_table(keys, NOFLUX ? 32u : 16u);
+    for (uint32_t t = threadIdx.x; t < 8u * kMaxLabels; t += blockDim.x) {
+        const uint32_t q = t / kMaxLabels, l = t % kMaxLabels;
+        ldelta[t] = (q < a.n_label_sets && (int)l < a.n_labels) ? a.label_delta[q * (uint32_t)a.n_labels + l] : 1.0;
+    }
+    __syncthreads();
+    zwalk_tile<2, 1, WRAPFIX, 128, 2, NOFLUX, MFLAGS, 8>(a, tile, a.partials ? a.partials + tile : nullptr,
+                                                         a.partial_stride, lds, tab, fix, ldelta, keys, nullptr, lds8);
+}
+
 // PFGSE on the unfolded image: the walk between diffusion times counts hits
 // (register chunk cache, packed displacement flushed every kZFlush steps and
 // at every diffusion time), then pow(delta, hits) per energy set and the
@@ -345,6 +376,32 @@ static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32
     case 3: return launch_z<EMODE, false, 128, 1>(a, ls, blocks, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_zwalk_sets(const WalkArgs &a, uint32_t threads, hipStream_t st)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    if (a.shift != 0 || !a.zwords || !a.zlabels || !a.label_delta || a.n_labels <= 0 || a.n_labels > kMaxLabels ||
+        a.n_label_sets == 0 || a.n_label_sets > 8 || threads == 0 || threads > 1024 || threads % 64 != 0)
+        return hipErrorInvalidValue;
+    const uint64_t tiles = (a.n + threads - 1) / threads;
+    if (tiles > 0x7FFFFFF0ull || (a.tile_map && a.n_tiles != tiles))
+        return hipErrorInvalidValue;
+    const uint32_t blocks = a.tile_map ? (uint32_t)(((tiles + kXcds - 1) / kXcds) * kXcds) : (uint32_t)tiles;
+    if (a.znoflux) {
+        hipLaunchKernelGGL((zwalk_sets_kernel<false, true, false>), dim3(blocks), dim3(threads), 0, st, a);
+    } else if (a.zmflags) {
+        if (a.zl.wrapfix)
+            hipLaunchKernelGGL((zwalk_sets_kernel<true, false, true>), dim3(blocks), dim3(threads), 0, st, a);
+        else
+            hipLaunchKernelGGL((zwalk_sets_kernel<false, false, true>), dim3(blocks), dim3(threads), 0, st, a);
+    } else if (a.zl.wrapfix) {
+        hipLaunchKernelGGL((zwalk_sets_kernel<true, false, false>), dim3(blocks), dim3(threads), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((zwalk_sets_kernel<false, false, false>), dim3(blocks), dim3(threads), 0, st, a);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hipStream_t st)
