@@ -142,14 +142,29 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     double *my_r = racc + threadIdx.x;
     ZEntry T;
     zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
+    // a hit's label word is loaded at the hit and applied one step later (or
+    // at the echo's end, before Q moves on), so its latency overlaps the next
+    // step's chunk load
+    uint32_t pw = 0u, psh = 0u;
+    bool pend = false;
+    auto settle = [&]() {
+        if (pend) {
+            const uint32_t lab = (pw >> psh) & 0xFu;
+            e *= ldelta[lab];
+            __hip_atomic_fetch_add(my_r + lab * blockDim.x, Q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            pend = false;
+        }
+    };
     auto step = [&]() {
         const uint32_t c = zmove_entry<WRAPFIX>(A, T);
         zchunk_fill_buf(zrsrc, c, QB, QV);
         zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
+        settle();
         if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
-            const uint32_t lab = zlabel_at(a, c);
-            e *= ldelta[lab];
-            __hip_atomic_fetch_add(my_r + lab * blockDim.x, Q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t cl = a.zlfold ? zfold_first_octant(c, a.zl) : c;
+            pw = a.zlabels[cl >> 3];
+            psh = (cl & 7u) * 4u;
+            pend = true;
         } else {
             A = c;
         }
@@ -165,6 +180,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
         }
         if (st < spe)
             step();
+        settle();
         Q += b.grad_echo[echo] * e;
     }
     const int n_labels = b.n_labels;

@@ -406,6 +406,11 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     double E[K], DL[K];
     double ES[NS];   // label-set walks: one energy per set
     uint32_t LH[K][NL];
+    // labelled walks: a hit's label word is loaded at the hit and applied one
+    // step later (or at the echo's end), so its latency overlaps the next
+    // step's chunk load instead of following the hit's own
+    uint32_t PW[K], PSH[K];
+    bool PEND[K];
     bool act[K];
     static_for<K>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
@@ -439,6 +444,9 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             FO[k] = 0u;
         }
         P[k] = 0;
+        PW[k] = 0u;
+        PSH[k] = 0u;
+        PEND[k] = false;
         if (NS > 1)
             static_for<NS>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -485,6 +493,40 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             constexpr int k = decltype(kc)::value;
             zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys, FO[k]);
         });
+    // apply walker k's pending hit: energy *= its label's delta (per set) and
+    // the per-label hit count, in hit order
+    auto apply_label = [&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const uint32_t lab = (PW[k] >> PSH[k]) & 0xFu;
+        if constexpr (NS > 1) {
+            static_for<NS>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                ES[q] *= ldelta[q * kMaxLabels + lab];
+            });
+        } else {
+            E[k] *= ldelta[lab];
+        }
+        if (LH_LDS) {
+            __hip_atomic_fetch_add(&lh[lab * blockDim.x + threadIdx.x], 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            static_for<NL>([&](auto lc) {
+                constexpr int l = decltype(lc)::value;
+                LH[k][l] += lab == (uint32_t)l ? 1u : 0u;
+            });
+        }
+    };
+    constexpr bool DEFER = CACHE >= 64 && EMODE == 2;
+    auto settle_labels = [&]() {
+        if constexpr (DEFER)
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                if (PEND[k]) {
+                    apply_label(kc);
+                    PEND[k] = false;
+                }
+            });
+    };
     auto step = [&]() {
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
@@ -521,6 +563,8 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 constexpr int k = decltype(kc)::value;
                 zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys, FO[k]);
             });
+            // the previous step's hit (its label load overlapped this step's chunk load)
+            settle_labels();
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
                 wallv[k] = zchunk_wall<CACHE == 64 ? 64 : 128>(C[k], QV[k]);
@@ -552,23 +596,10 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             if (CACHE >= 64 && EMODE == 2) {
                 if (__builtin_expect(wall != 0u, 0)) {
                     Hc[k] += 1u;
-                    const uint32_t lab = zlabel_at(a, C[k]);
-                    if (NS > 1)
-                        static_for<NS>([&](auto qc) {
-                            constexpr int q = decltype(qc)::value;
-                            ES[q] *= ldelta[q * kMaxLabels + lab];
-                        });
-                    else
-                        E[k] *= ldelta[lab];
-                    if (LH_LDS) {
-                        __hip_atomic_fetch_add(&lh[lab * blockDim.x + threadIdx.x], 1u, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                    } else {
-                        static_for<NL>([&](auto lc) {
-                            constexpr int l = decltype(lc)::value;
-                            LH[k][l] += lab == (uint32_t)l ? 1u : 0u;
-                        });
-                    }
+                    const uint32_t c = a.zlfold ? zfold_first_octant(C[k], a.zl) : C[k];
+                    PW[k] = a.zlabels[c >> 3];
+                    PSH[k] = (c & 7u) * 4u;
+                    PEND[k] = true;
                 } else {
                     A[k] = C[k];
                     P[k] += DP[k];
@@ -626,6 +657,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 since = 0;
             }
         }
+        settle_labels();
         if (part)
             stage(echo);
     }
