@@ -341,10 +341,13 @@ __device__ __forceinline__ void stage_echo_sets8(double (*lds8)[8][kMaxWaves], u
 // e[q] cos((k . dr) voxel) for every k of the time's k row and every set q,
 // staged in LDS (lds rows q * kPfgseKChunk + k) and written as partials
 // [(q * n_delta + i) * n_k + k][blockIdx.x].  cos is evaluated once per k and
-// shared by the sets; with ROT (k row k_0 + j d) the phases advance by a
-// complex rotation, e^{i(ph0 + j phd)} = e^{i ph0} (e^{i phd})^j, 4 FMAs per k
-// instead of a cos (rounding grows ~2 ulp per k, far inside the 1e-11 M(0)
-// tolerance).  All threads of the block call it.
+// shared by the sets; with ROT (k row k_0 + j d) the cosines follow the
+// three-term recurrence cos(ph0 + (j+1) phd) = 2 cos(phd) cos(ph0 + j phd) -
+// cos(ph0 + (j-1) phd), one FMA per k instead of a cos (over 64 k the error
+// stays below 5e-13 of a term, far inside the 1e-11 M(0) tolerance); the
+// recurrence runs through whole groups of 8 k, the values past the last k
+// unused.  Idle lanes carry zero energies, so their terms are zero without
+// a select.  All threads of the block call it.
 template <bool ROT, int MAXSETS = kMaxSets>
 __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, double dX, double dY, double dZ,
                                               const double *e, int nsets, bool active,
@@ -353,14 +356,18 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const uint64_t krow = (uint64_t)i * p.k_delta_stride;
-    double c = 0.0, s = 0.0, c1 = 1.0, s1 = 0.0;
+    double c = 0.0, cprev = 0.0, twoc1 = 2.0;
     if (ROT) {
         const double ph0 = (p.kx[krow] * dX + p.ky[krow] * dY + p.kz[krow] * dZ) * p.voxel;
         const double phd = ((p.kx[krow + 1] - p.kx[krow]) * dX + (p.ky[krow + 1] - p.ky[krow]) * dY +
                             (p.kz[krow + 1] - p.kz[krow]) * dZ) * p.voxel;
+        double s = 0.0, s1 = 0.0, c1 = 1.0;
         sincos(ph0, &s, &c);
         sincos(phd, &s1, &c1);
+        cprev = c * c1 + s * s1;   // cos(ph0 - phd)
+        twoc1 = 2.0 * c1;
     }
+    (void)active;
     for (uint32_t k0 = 0; k0 < p.n_k; k0 += kPfgseKChunk) {
         const uint32_t kc = p.n_k - k0 < (uint32_t)kPfgseKChunk ? p.n_k - k0 : (uint32_t)kPfgseKChunk;
         // groups of 8 k: the cosines of the group, then one reduce-scatter
@@ -369,15 +376,15 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
             double cs[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const uint32_t kk = g0 + (uint32_t)j;
-                cs[j] = 0.0;
-                if (kk < kc) {   // uniform: the rotation advances once per k
-                    if (ROT) {
-                        cs[j] = c;
-                        const double cn = c * c1 - s * s1;
-                        s = s * c1 + c * s1;
-                        c = cn;
-                    } else {
+                if (ROT) {
+                    cs[j] = c;
+                    const double cn = fma(twoc1, c, -cprev);
+                    cprev = c;
+                    c = cn;
+                } else {
+                    const uint32_t kk = g0 + (uint32_t)j;
+                    cs[j] = 0.0;
+                    if (kk < kc) {   // uniform
                         const uint64_t kk_i = krow + k0 + kk;
                         cs[j] = cos((p.kx[kk_i] * dX + p.ky[kk_i] * dY + p.kz[kk_i] * dZ) * p.voxel);
                     }
@@ -389,7 +396,7 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
                     double t[8];
 #pragma unroll
                     for (int j = 0; j < 8; ++j)
-                        t[j] = active ? cs[j] * e[q] : 0.0;
+                        t[j] = cs[j] * e[q];
                     const double v = wave_reduce_scatter8(t);
                     const uint32_t kk = g0 + (uint32_t)(lane >> 1);
                     if ((lane & 0x31) == 0 && kk < kc)
