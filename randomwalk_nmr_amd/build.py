@@ -19,7 +19,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["walk_kernels.hip", "zwalk_kernels.hip", "batch_kernels.hip", "backward_kernels.hip",
-               "locality_kernels.hip", "capi.hip", "drivers.hip"]
+               "locality_kernels.hip", "capi.hip", "drivers.hip", "probe_kernels.hip"]
 CXX_SOURCES = ["laplace.cpp", "host_sim.cpp", "image_io.cpp", "histogram.cpp", "config.cpp", "output.cpp"]
 # the command-line program, linked against the library
 APP_SOURCE = "rwnmr_app.cpp"
@@ -41,17 +41,46 @@ def _run(cmd):
     return out
 
 
+# The walk kernels reload image chunks with the indexed buffer load, bound by
+# its LLVM intrinsic name (clang has a builtin only for the raw form).  The
+# probe compiles that binding; a toolchain without it builds the raw form.
+_PROBE_SRC = """#include <hip/hip_runtime.h>
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+__device__ v4u load4(__amdgpu_buffer_rsrc_t r, int vindex, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+__global__ void probe(unsigned int *w, v4u *out, int i)
+{
+    __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(w, (short)16, 64, 0x00020000);
+    out[threadIdx.x] = load4(r, i, 0, 0, 0);
+}
+"""
+
+
+def _indexed_buffer_load_flags():
+    """[] when the indexed buffer load compiles, else the raw-load define."""
+    src = os.path.join(OUT_DIR, "probe_indexed_load.hip")
+    with open(src, "w") as f:
+        f.write(_PROBE_SRC)
+    p = subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-O3", "--cuda-device-only", "-c", src, "-o",
+                        os.path.join(OUT_DIR, "probe_indexed_load.o")], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    return [] if p.returncode == 0 else ["-DRWNMR_RAW_BUFFER_LOAD"]
+
+
 def build(verbose: bool = False, jobs: int = 0) -> str:
     """Compile every source (always, `jobs` compilers at once; 0: up to 8) and
     link the shared library."""
     os.makedirs(OUT_DIR, exist_ok=True)
+    load_flags = _indexed_buffer_load_flags()
+    if verbose and load_flags:
+        print("indexed buffer load unavailable: building the raw-load form")
     cmds, objs = [], []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(OUT_DIR, src + ".o")
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *load_flags, "-c", path, "-o", obj])
         objs.append(obj)
     for src in CXX_SOURCES:
         path = os.path.join(CSRC, src)

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     double Q = 0.0;
     uint32_t QB = 0xFFFFFFFFu;
     uint4 QV = make_uint4(0u, 0u, 0u, 0u);
-    const zv4i zrsrc = zchunk_rsrc(a.zwords, a.zl);
+    const zrsrc_t zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
     double *my_r = racc + threadIdx.x;

@@ -268,32 +268,51 @@ __device__ __forceinline__ void zchunk_fill_n(const char *words, uint32_t c, uin
     }
 }
 
-// Buffer descriptor over the unfolded image's 16-byte chunks (GFX9 V#:
-// base address, stride 16, num_records = chunk count, 32-bit data format)
-// for an indexed buffer load: the chunk load takes the chunk id itself as
-// the index (no address shift), and ids at or past the chunk count read
-// zeros instead of memory.
-typedef int32_t zv4i __attribute__((ext_vector_type(4)));
-__device__ zv4i zbuffer_load_v4(zv4i rsrc, int32_t vindex, int32_t voffset, int32_t soffset, int32_t aux)
-    __asm("llvm.amdgcn.struct.buffer.load.v4i32");
+// Buffer descriptor over the unfolded image's 16-byte chunks (base address,
+// stride 16, num_records = chunk count, built by the compiler's
+// make_buffer_rsrc builtin) for the chunk reloads; ids at or past the chunk
+// count read zeros instead of memory.  The last descriptor word (32-bit data
+// format, no swizzle) is the GFX9 layout (gfx90a / gfx942 / gfx950): other
+// families lay it out differently, so they are refused at compile time.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(__GFX9__)
+#error "the chunk buffer descriptor is laid out for GFX9 targets (gfx90a, gfx942, gfx950)"
+#endif
+constexpr int32_t kZChunkRsrcWord3 = 0x00020000;
+typedef __amdgpu_buffer_rsrc_t zrsrc_t;
+typedef uint32_t zv4u __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ zv4i zchunk_rsrc(const uint32_t *words, const ZLayout &zl)
+#if defined(RWNMR_RAW_BUFFER_LOAD)
+// Raw buffer load (a documented clang builtin) at byte offset cid * 16: one
+// shift more per reload than the indexed form below.
+__device__ __forceinline__ zv4u zchunk_load(zrsrc_t rsrc, uint32_t cid)
 {
-    const uint64_t p = (uint64_t)(uintptr_t)words;
-    zv4i r;
-    r.x = (int32_t)(uint32_t)p;
-    r.y = (int32_t)(((uint32_t)(p >> 32) & 0xFFFFu) | (16u << 16));
-    r.z = (int32_t)(zl.bits > 7 ? (1u << (zl.bits - 7)) : 1u);   // chunks: 128 addresses each
-    r.w = 0x00020000;
-    return r;
+    return __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int32_t)(cid << 4), 0, 0);
+}
+#else
+// Indexed (struct) buffer load: the chunk id itself is the index, so no
+// address shift.  Clang has no builtin for the indexed form, so the LLVM
+// intrinsic is bound by name; the build probes that this compiles and
+// otherwise builds the raw form above (RWNMR_RAW_BUFFER_LOAD).
+__device__ zv4u zbuffer_load_v4(zrsrc_t rsrc, int32_t vindex, int32_t voffset, int32_t soffset, int32_t aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+__device__ __forceinline__ zv4u zchunk_load(zrsrc_t rsrc, uint32_t cid)
+{
+    return zbuffer_load_v4(rsrc, (int32_t)cid, 0, 0, 0);
+}
+#endif
+
+__device__ __forceinline__ zrsrc_t zchunk_rsrc(const uint32_t *words, const ZLayout &zl)
+{
+    const int32_t chunks = (int32_t)(zl.bits > 7 ? (1u << (zl.bits - 7)) : 1u);   // 128 addresses each
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(words), (short)16, chunks, kZChunkRsrcWord3);
 }
 
-__device__ __forceinline__ void zchunk_fill_buf(const zv4i &rsrc, uint32_t c, uint32_t &QB, uint4 &QV)
+__device__ __forceinline__ void zchunk_fill_buf(const zrsrc_t &rsrc, uint32_t c, uint32_t &QB, uint4 &QV)
 {
     const uint32_t cid = c >> 7;
     if (cid != QB) {
-        const zv4i v = zbuffer_load_v4(rsrc, (int32_t)cid, 0, 0, 0);
-        QV = make_uint4((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z, (uint32_t)v.w);
+        const zv4u v = zchunk_load(rsrc, cid);
+        QV = make_uint4(v.x, v.y, v.z, v.w);
         QB = cid;
     }
 }
@@ -391,7 +410,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     constexpr bool LH_LDS = EMODE == 2 && K == 1 && NS == 1;
     constexpr int NL = (EMODE == 2 && !LH_LDS && NS == 1) ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
-    const zv4i zrsrc = zchunk_rsrc(a.zwords, a.zl);
+    const zrsrc_t zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
     const uint64_t n = a.n;
     const uint32_t echoes = a.echoes, spe = a.steps_per_echo;

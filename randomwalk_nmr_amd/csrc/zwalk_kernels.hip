@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
     uint4 QV = make_uint4(0u, 0u, 0u, 0u);
     int32_t P = 0;
     uint32_t since = 0;
-    const zv4i zrsrc = zchunk_rsrc(a.zwords, a.zl);
+    const zrsrc_t zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
     // the next direction is drawn a step ahead, during the chunk load (the
