@@ -65,7 +65,32 @@ struct BackwardArgs {
     const double *grad_echo;   // [echoes + 1] dL/dM(n)
     double *partials;          // [n_labels][partial_stride]
     uint64_t partial_stride;   // = segments * blocks
+    uint32_t blocks;           // walker tiles (blocks) per segment
 };
+
+// Segment and walker tile of this workgroup of a replay launch (a 1-D grid
+// of 8 * ceil(blocks * segments / 8) workgroups).  Workgroup g runs on XCD
+// g % 8, and XCD j replays the j-th contiguous eighth of the tile-major,
+// segment-minor list, so with position-ordered walkers each XCD's L2 holds
+// one region of the image in every segment.  False for a spare workgroup.
+__device__ __forceinline__ bool backward_block(const BackwardArgs &b, uint32_t &seg, uint64_t &tile)
+{
+    const uint64_t total = (uint64_t)b.blocks * b.n_segments;
+    const uint64_t per_xcd = (total + kXcds - 1) / kXcds;
+    const uint64_t t = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    if (t >= total)
+        return false;
+    tile = t / b.n_segments;
+    seg = (uint32_t)(t % b.n_segments);
+    return true;
+}
+
+// Workgroups of a replay launch of `blocks` tiles per segment.
+inline uint64_t backward_grid(uint64_t blocks, uint32_t n_segments)
+{
+    const uint64_t total = blocks * n_segments;
+    return ((total + kXcds - 1) / kXcds) * kXcds;
+}
 
 hipError_t launch_backward(const WalkArgs &a, const BackwardArgs &b, int bc, uint32_t threads, hipStream_t st);
 

@@ -29,8 +29,11 @@ __global__ __launch_bounds__(256) void backward_kernel(const WalkArgs a, const B
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
-    const uint32_t seg = blockIdx.y;
-    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t seg;
+    uint64_t tile;
+    if (!backward_block(b, seg, tile))
+        return;   // spare workgroup of the XCD-striped grid (the whole block leaves)
+    const uint64_t id = tile * blockDim.x + threadIdx.x;
     const uint64_t n = b.n;
     const bool active = id < n;
     const uint64_t j = active ? id : 0;
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(256) void backward_kernel(const WalkArgs a, const B
         double s = 0.0;
         for (int q = 0; q < nw; ++q)
             s += lds[threadIdx.x][q];
-        const uint64_t col = (uint64_t)seg * gridDim.x + blockIdx.x;
+        const uint64_t col = (uint64_t)seg * b.blocks + tile;
         b.partials[(uint64_t)threadIdx.x * b.partial_stride + col] = s;
     }
 }
@@ -112,6 +115,10 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     __shared__ uint8_t keys[kZKeyEntries];
     __shared__ double ldelta[kMaxLabels];
     __shared__ double racc[kMaxLabels * kZBackThreads];   // R_wl, label-major
+    uint32_t seg;
+    uint64_t tile;
+    if (!backward_block(b, seg, tile))
+        return;   // spare workgroup of the XCD-striped grid (the whole block leaves)
     zbuild_table(a.zl, tab, fix);
     zbuild_key_table(keys, 16u);
     if (threadIdx.x < kMaxLabels)
@@ -122,8 +129,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int nw = blockDim.x / kWave;
-    const uint32_t seg = blockIdx.y;
-    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t id = tile * blockDim.x + threadIdx.x;
     const uint64_t n = b.n;
     const bool active = id < n;
     const uint64_t j = active ? id : 0;
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
         double sum = 0.0;
         for (int q = 0; q < nw; ++q)
             sum += lds[threadIdx.x][q];
-        const uint64_t col = (uint64_t)seg * gridDim.x + blockIdx.x;
+        const uint64_t col = (uint64_t)seg * b.blocks + tile;
         b.partials[(uint64_t)threadIdx.x * b.partial_stride + col] = sum;
     }
 }
@@ -208,7 +214,9 @@ hipError_t launch_backward(const WalkArgs &a, const BackwardArgs &b, int bc, uin
     if (a.zwords) {
         if (!a.zlabels || threads != kZBackThreads)
             return hipErrorInvalidValue;
-        const dim3 grid((uint32_t)((b.n + threads - 1) / threads), b.n_segments);
+        if ((uint64_t)b.blocks != (b.n + threads - 1) / threads)
+            return hipErrorInvalidValue;
+        const dim3 grid((uint32_t)backward_grid(b.blocks, b.n_segments));
         if (a.zl.wrapfix)
             hipLaunchKernelGGL(zbackward_kernel<true>, grid, dim3(threads), 0, st, a, b);
         else
@@ -216,7 +224,9 @@ hipError_t launch_backward(const WalkArgs &a, const BackwardArgs &b, int bc, uin
         return hipGetLastError();
     }
     const uint32_t bx = (uint32_t)((b.n + threads - 1) / threads);
-    const dim3 grid(bx, b.n_segments);
+    if ((uint64_t)b.blocks != bx)
+        return hipErrorInvalidValue;
+    const dim3 grid((uint32_t)backward_grid(bx, b.n_segments));
     switch (bc) {
     case kNoflux: hipLaunchKernelGGL(backward_kernel<kNoflux>, grid, dim3(threads), 0, st, a, b); break;
     case kPeriodic: hipLaunchKernelGGL(backward_kernel<kPeriodic>, grid, dim3(threads), 0, st, a, b); break;

@@ -768,6 +768,13 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         RW_HIP(c, hipGetLastError());
     }
     a.steps_per_echo = steps_per_echo;
+    if (shift == 0) {
+        // XCD-striped tiles: XCD j walks the j-th contiguous eighth of the
+        // walkers, so position-ordered walkers keep one region of the image
+        // per XCD L2
+        a.tile_map = 1;
+        a.n_tiles = blocks;
+    }
     uint32_t done = 0;
     do {
         const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
@@ -1461,6 +1468,8 @@ rwnmr_status rwnmr_cpmg_run_label_sets(rwnmr_ctx *c, const rwnmr_image *img, rwn
             b.label_delta = c->label_delta.as<double>() + (size_t)8 * g * nl;
             b.n_label_sets = gsets;
             b.partial_stride = blocks;
+            b.tile_map = 1;   // XCD-striped tiles (position-ordered walkers keep an image region per L2)
+            b.n_tiles = blocks;
             b.steps_per_echo = steps_per_echo;
             b.set_energy = carry ? c->lset_energies.as<double>() : nullptr;
             double *part = c->partials.as<double>();
@@ -1749,7 +1758,7 @@ extern "C" rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *c, const rwnmr_image *img
     attach_zimage(c, img, bc, 2, a);
     const uint32_t threads = a.zwords ? rwnmr::kZBackThreads : 256u;
     const uint64_t bx = (n + threads - 1) / threads;
-    if (bx > 0x7FFFFFFFull || nseg > 65535)
+    if (rwnmr::backward_grid(bx, nseg) > 0x7FFFFFFFull)
         return fail(c, RWNMR_ERR_INVALID, "backward: grid too large");
     rwnmr::BackwardArgs b;
     b.ckpt = const_cast<void *>(ckpt);
@@ -1761,6 +1770,7 @@ extern "C" rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *c, const rwnmr_image *img
     b.steps_per_echo = steps_per_echo;
     b.grad_echo = grad_echo;
     b.partial_stride = (uint64_t)nseg * bx;
+    b.blocks = (uint32_t)bx;
     RW_HIP(c, c->partials.reserve(8ull * nl * b.partial_stride + 8));
     b.partials = c->partials.as<double>();
     hipEvent_t t0 = timing_begin(c);
