@@ -9,6 +9,11 @@ full CPMG run (walkers reset to their start state, then 1.2e11 walker-steps)
 with all inputs resident in HBM.
 
 Prints one JSON line.  --gpus above 1 is refused: the library runs on one GPU.
+Beside the timed runs the line carries the walk kernel's roofline (VALU
+issue, the binding roof, with HBM, L2-request and random-gather-probe
+figures), the CPU reference loop timed on the host cores, and under "fit"
+the metric's second half: the relaxivity fit's GPU wall time to 1e-3 RMS
+decay error, gradient fit against the genetic algorithm (--no-fit skips it).
 
 --workload selects the other measured configurations (one JSON line each):
   small   128^3 CPMG, 10k walkers, 2000 echoes, plus the 128-bin Laplace
@@ -35,7 +40,14 @@ if ROOT not in sys.path:
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E bytes/s (published)
 BYTES_PER_STEP = 8         # one uint64 bitBlock word read per walker-step
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_kernel_cpmg512_traffic.json")
+# VALU issue ceiling: 256 CUs x 4 SIMDs, each issuing one wave64 VALU
+# instruction per 4 cycles at the published 2.4 GHz peak engine clock
+SIMDS, CLOCK_HZ, CYCLES_PER_VALU = 1024, 2.4e9, 4
+VALU_WAVE_INSTR_PER_S = SIMDS * CLOCK_HZ / CYCLES_PER_VALU
+# committed rocprofv3 counters of the headline walk kernel (VALU per wave-step,
+# L2 requests and HBM bytes per launch), applied when the launch shape matches
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "walk_kernel_cpmg512_counters.json")
+PROBE_TABLES = {"2MiB_L2": 2 << 20, "128MiB_MALL": 128 << 20, "4GiB_HBM": 4 << 30}
 
 
 def log(msg):
@@ -62,7 +74,9 @@ def main():
                     help="echoes between locality re-sorts of the walkers (rwnmr_ctx_set_locality); 0 keeps the "
                          "library default")
     ap.add_argument("--workload", default="cpmg512", choices=["cpmg512", "small", "pfgse", "batch", "fit"])
-    ap.add_argument("--ga-time-limit", type=float, default=150.0, help="fit workload: GA wall-time cap (s)")
+    ap.add_argument("--ga-time-limit", type=float, default=60.0, help="fit workload: GA wall-time cap (s)")
+    ap.add_argument("--no-fit", action="store_true", help="skip the relaxivity fit part of the headline line")
+    ap.add_argument("--no-probe", action="store_true", help="skip the random-gather probe")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (HBM counter scale)")
     args = ap.parse_args()
@@ -155,15 +169,23 @@ def main():
     value = walker_steps / (ms_per_step * 1e-3)
     avg_kernel_s = kern_ms * 1e-3 / max(launches, 1)
     steps_per_launch = walker_steps * args.steps / max(launches, 1)
-    achieved = steps_per_launch * BYTES_PER_STEP / avg_kernel_s
-    # HBM bytes per launch come from the committed FETCH_SIZE / WRITE_SIZE profile of this
-    # same command; they are only reported when that profile was taken at this launch shape.
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
+    kernel_steps_per_s = steps_per_launch / avg_kernel_s
+    # VALU per wave-step, L2 requests and HBM bytes per launch come from the committed
+    # rocprofv3 counters of this same command; they apply when the profile was taken with the
+    # same boundary and walkers per lane (the kernel's code; threads per block do not change it)
+    prof = None
+    if os.path.exists(COUNTERS_FILE):
+        with open(COUNTERS_FILE) as f:
             prof = json.load(f)
-        if prof.get("launch") == f"{tpb},{k}" and args.bc == "mirror":
-            traffic = prof.get("hbm_bytes_per_launch")
+        if prof.get("bc") != args.bc or int(prof.get("walkers_per_lane", 0)) != k or args.unfolded not in (-1, 3):
+            prof = None
+    # gather probe: random 16-byte loads from tables held in L2, in the MALL and in HBM
+    probe = {}
+    if not args.no_probe:
+        for name, nbytes in PROBE_TABLES.items():
+            rate, _ = ctx.probe_gather(nbytes)
+            probe[name] = rate
+            log(f"gather probe {name}: {rate:.3e} random 16-B loads/s")
 
     sums = SUMS.cpu().numpy()
     decay_ok = bool(np.all(np.isfinite(sums)) and sums[0] == N and np.all(np.diff(sums) <= 0))
@@ -220,21 +242,69 @@ def main():
                                f"{spe} steps, {args.bc} BC, voxel {voxel} um, rho {rho} um/s",
                    "walker_steps_per_step": walker_steps, "launch": {"threads_per_block": tpb,
                                                                     "walkers_per_lane": k}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": ("walk_kernel<noflux, CPMG> on the brick-ordered bitBlock image" if args.bc == "noflux" else
-                                "zwalk_kernel<CPMG, 16-byte chunk cache, byte-key direction table, direction drawn a step "
-                                f"ahead> on the unfolded {args.bc} image"),
-                     "avg_kernel_ms": avg_kernel_s * 1e3, "launches": launches,
-                     "algorithmic_bytes_per_launch": steps_per_launch * BYTES_PER_STEP},
+        "roofline": roofline(args, prof, probe, kernel_steps_per_s, steps_per_launch, avg_kernel_s, launches, k),
         "cpu_baseline": cpu,
         "checks": {"decay_monotone": decay_ok, "subsample_bit_exact_vs_cpu": parity},
     }
-    out["roofline"]["achieved"] = achieved / 1e9
-    out["roofline"]["peak"] = HBM_PEAK / 1e9
-    print(json.dumps(out), flush=True)
     img.close()
     ctx.close()
+    if not args.no_fit:
+        # the second half of the metric: relaxivity fit time, gradient vs genetic algorithm
+        out["fit"] = run_fit(args)
+    print(json.dumps(out), flush=True)
+
+
+def walk_kernel_label(bc, k):
+    form = {"mirror": "unfolded mirror image, walkers folded into its first octant with reflection-flag direction "
+                      "tables at every launch",
+            "periodic": "unfolded periodic image",
+            "noflux": "periodic unfolded image with the noflux face test and turn-around"}[bc]
+    return (f"zwalk_kernel<CPMG, {k} walker(s) per lane, 16-byte register chunk cache by indexed buffer loads, "
+            f"byte-key direction table, direction drawn a step ahead> on the {form}")
+
+
+def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, launches, k):
+    """Roofline of the dominant kernel.  The binding roof is VALU issue: the
+    walk keeps each walker's 16-byte image chunk in registers and the L2
+    serves nearly every reload, so HBM is far from busy; its ceiling in
+    walker-steps/s is the VALU issue rate x 64 lanes / VALU instructions per
+    wave-step (committed SQ counters).  Secondary figures: the algorithmic
+    8 B per step priced at HBM peak (above 1 because the L2 serves the
+    gathers), the measured HBM bytes per launch, and the walk's L2 request
+    rate against the random-gather probe's."""
+    out = {"bound": "valu", "achieved": steps_per_s, "peak": None, "unit": "walker-steps/s", "frac": None,
+           "traffic": None, "kernel": walk_kernel_label(args.bc, k), "avg_kernel_ms": avg_kernel_s * 1e3,
+           "launches": launches, "walker_steps_per_launch": steps_per_launch,
+           "peak_basis": f"{SIMDS} SIMDs x {CLOCK_HZ / 1e9} GHz / {CYCLES_PER_VALU} cycles per wave64 VALU "
+                         "instruction x 64 lanes / VALU instructions per wave-step"}
+    sec = {"hbm_algorithmic": {"bytes_per_step": BYTES_PER_STEP,
+                               "achieved_GBps": steps_per_s * BYTES_PER_STEP / 1e9, "peak_GBps": HBM_PEAK / 1e9,
+                               "frac": steps_per_s * BYTES_PER_STEP / HBM_PEAK}}
+    if prof:
+        vps = float(prof["valu_per_wave_step"])
+        peak = VALU_WAVE_INSTR_PER_S * 64.0 / vps
+        out.update(peak=peak, frac=steps_per_s / peak, valu_per_wave_step=vps,
+                   valu_busy_measured=prof.get("valu_busy"), counters=os.path.relpath(COUNTERS_FILE, ROOT))
+        # HBM bytes per launch scale with the launch's walker-steps
+        scale = steps_per_launch / float(prof["walker_steps_per_launch"])
+        hbm = float(prof["hbm_bytes_per_launch"]) * scale
+        out["traffic"] = hbm
+        sec["hbm_measured"] = {"bytes_per_launch": hbm, "achieved_GBps": hbm / avg_kernel_s / 1e9,
+                               "frac": hbm / avg_kernel_s / HBM_PEAK}
+        req = float(prof["l2_requests_per_launch"]) * scale
+        l2 = {"requests_per_launch": req, "achieved_per_s": req / avg_kernel_s,
+              "hit_rate": prof.get("l2_hit_rate")}
+        if probe:
+            l2.update(probe_L2_per_s=probe.get("2MiB_L2"), frac_of_L2_probe=req / avg_kernel_s / probe["2MiB_L2"],
+                      probe_MALL_per_s=probe.get("128MiB_MALL"),
+                      frac_of_MALL_probe=req / avg_kernel_s / probe["128MiB_MALL"])
+        sec["l2_requests"] = l2
+    if probe:
+        sec["gather_probe"] = {"unit": "random 16-B loads/s (4096 x 256 lanes, 8 loads in flight per lane)",
+                               **probe,
+                               **{f"{n}_GBps_of_128B_lines": r * 128 / 1e9 for n, r in probe.items()}}
+    out["secondary"] = sec
+    return out
 
 
 # ------------------------------------------------------------------ other workloads
@@ -468,16 +538,18 @@ def wl_batch(args):
     ctx.close()
 
 
-def wl_fit(args):
+def run_fit(args):
     """256^3 sphere pack (r 6-12, porosity 0.25), solid labelled 0-7 by a
     64-site Voronoi partition, rho_l U(5, 40), T2b 2800 ms, 1M walkers, 4000
     echoes x 3 steps (voxel 1 um); target = forward(truth) + 0.5% Gaussian
-    noise; gradient fit (checkpoint every 500 echoes) vs GA (population 64),
-    GPU wall time until the RMS error against the noise-free decay <= 1e-3."""
+    noise; gradient fit (checkpoint every 500 echoes) vs the genetic
+    algorithm (population 64, each batch of individuals scored on one
+    trajectory), GPU wall time until the fitted decay is within 1e-3 RMS of
+    the noise-free decay.  Returns the result dict."""
     import numpy as np
     import torch
     import randomwalk_nmr_amd as R
-    from randomwalk_nmr_amd.fit import CpmgModel, genetic_fit, gradient_fit, rms
+    from randomwalk_nmr_amd.fit import CpmgModel, genetic_fit, gradient_fit
     rng = np.random.default_rng(args.seed)
     voxel, D0, N, E, S = 1.0, 2.5, 1_000_000, 4000, 256
     spe = int(round(0.2 / R.time_step(voxel, D0)))
@@ -500,24 +572,48 @@ def wl_fit(args):
         model.decay(torch.tensor(rho_true), torch.tensor(t2b_true))
     torch.cuda.synchronize()
     fwd_s = time.perf_counter() - t0
+    # one batch of 64 individuals on one trajectory (8 label-set walks)
+    pop = np.concatenate([rng.uniform(1.0, 60.0, (64, 8)), rng.uniform(500.0, 5000.0, (64, 1))], axis=1)
+    model.decays(pop[:8, :-1], pop[:8, -1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model.decays(pop[:, :-1], pop[:, -1])
+    torch.cuda.synchronize()
+    batch64_s = time.perf_counter() - t0
     g = gradient_fit(model, target, rho0=np.full(8, 20.0), t2b0=2000.0, truth=truth, tol=1e-3, max_evals=60,
                      log=log)
     ga = genetic_fit(model, target, bounds_lo=np.concatenate([np.full(8, 1.0), [500.0]]),
                      bounds_hi=np.concatenate([np.full(8, 60.0), [5000.0]]), truth=truth, tol=1e-3, population=64,
                      time_limit=args.ga_time_limit, log=log)
-    _line({"metric": "rho-fit GPU wall time to 1e-3 RMS decay error (gradient vs GA)",
-           "value": g["seconds"], "unit": "s", "higher_is_better": False, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"{S}^3 sphere pack (r 6-12, porosity {phi:.3f}), 64-site Voronoi labels 0-7, "
-                                  f"{N} walkers, {E} echoes x {spe} steps, mirror, noise 0.5%, checkpoint every 500",
-                      "forward_s": fwd_s},
-           "gradient": {"converged": g["converged"], "seconds": g["seconds"], "evals": g["evals"],
-                        "rms_truth": g["rms_truth"], "rho": np.asarray(g["rho"]).tolist(), "t2b": g["t2b"]},
-           "ga": {"converged": ga["converged"], "seconds": ga["seconds"], "generations": ga["generations"],
-                  "evals": ga["evals"], "rms_truth": ga["rms_truth"], "time_limit_s": args.ga_time_limit},
-           "truth": {"rho": rho_true.tolist(), "t2b": t2b_true},
-           "ga_over_gradient": (ga["seconds"] / g["seconds"]) if g["converged"] else None})
     img.close()
     ctx.close()
+
+    def recovery(rho, t2b):
+        rel = np.abs(np.asarray(rho) - rho_true) / rho_true
+        return {"rho_rel_err": rel.tolist(), "rho_rel_err_max": float(rel.max()),
+                "rho_rel_err_mean": float(rel.mean()), "t2b_rel_err": abs(float(t2b) - t2b_true) / t2b_true}
+
+    return {"metric": "rho-fit GPU wall time to 1e-3 RMS decay error (gradient vs GA)",
+            "value": g["seconds"], "unit": "s", "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{S}^3 sphere pack (r 6-12, porosity {phi:.3f}), 64-site Voronoi labels 0-7, "
+                                   f"{N} walkers, {E} echoes x {spe} steps, mirror, noise 0.5%, checkpoint every 500",
+                       "forward_s": fwd_s, "population64_scoring_s": batch64_s},
+            "gradient": {"converged": g["converged"], "seconds": g["seconds"], "evals": g["evals"],
+                         "rms_truth": g["rms_truth"], "rho": np.asarray(g["rho"]).tolist(), "t2b": g["t2b"],
+                         "recovery": recovery(g["rho"], g["t2b"])},
+            "ga": {"converged": ga["converged"], "seconds": ga["seconds"], "generations": ga["generations"],
+                   "evals": ga["evals"], "resets": ga["resets"], "rms_truth": ga["rms_truth"],
+                   "time_limit_s": args.ga_time_limit, "genotype": np.asarray(ga["genotype"]).tolist(),
+                   "recovery": recovery(ga["genotype"][:-1], ga["genotype"][-1])},
+            "truth": {"rho": rho_true.tolist(), "t2b": t2b_true},
+            "identifiability": "a decay within 1e-3 RMS does not determine 8 per-label relaxivities: different "
+                               "(rho_l, T2b) give decays that agree to the noise level, so recovery errors are "
+                               "reported beside the decay error",
+            "ga_over_gradient": (ga["seconds"] / g["seconds"]) if (g["converged"] and ga["converged"]) else None}
+
+
+def wl_fit(args):
+    _line(run_fit(args))
 
 
 if __name__ == "__main__":

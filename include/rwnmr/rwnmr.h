@@ -306,6 +306,17 @@ rwnmr_status rwnmr_pfgse_run_sampled(rwnmr_ctx *ctx, const rwnmr_image *img, rwn
                                      const double *set_delta, uint32_t n_samples, double *M_samples, double *M,
                                      double *msd);
 
+/* Random-gather probe (measurement; no reference counterpart): times
+ * loads_per_thread independent random 16-byte loads per lane (the walk
+ * kernels' chunk reloads), over 4096 x 256 lanes, from a device table of
+ * table_bytes (a power of two, 16 B .. 64 GiB), which the probe allocates,
+ * fills, reads once untimed and frees.  The table size decides what serves
+ * the loads: L2 (a few MiB), the Infinity Cache / MALL (up to 256 MiB) or
+ * HBM.  Returns random loads per second (each a separate cache line) and
+ * the timed kernel's milliseconds.  Synchronous on the context stream. */
+rwnmr_status rwnmr_probe_gather(rwnmr_ctx *ctx, uint64_t table_bytes, uint32_t loads_per_thread, double *loads_per_s,
+                                double *kernel_ms);
+
 /* Laplace inversion, same names and semantics as the reference's C API
  * (B/Laplace/include/nmrinv.h:13-39, B/Laplace/src/nmrinv.cpp): one global
  * inverter guarded by a mutex; FP64 throughout. */

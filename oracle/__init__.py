@@ -19,6 +19,9 @@ Parity status of each oracle routine (how it is pinned, see the tests):
     mt19937 / Lemire placement     pinned (C++ standard value, numpy, libstdc++)
     setup_walkers (point / cubic,  pinned (brute-force reachable sets on tiny
       voxel division)                images, draw counts, seed_and_place)
+    xorshift64, accumulated        pinned against the reference's own code
+      linspace/logspace,             (oracle/_ref/libref.so, oracle/build_ref.py;
+      Stejskal-Tanner fit            tests/test_ref.py)
     laplace                        pinned (scipy.optimize.nnls, closed forms;
                                      the L-curve curvature against a finite-
                                      difference geometric curvature, with the

@@ -12,6 +12,7 @@ singular vectors' sign convention does not matter.
 """
 from __future__ import annotations
 
+import math
 import numpy as np
 from scipy.optimize import nnls as _scipy_nnls
 
@@ -28,8 +29,10 @@ def linspace_accumulated(start: float, end: float, size: int, include_final: boo
 
 
 def logspace_accumulated(start: float, end: float, size: int, include_final: bool = True):
-    """10**linspace_accumulated (B/Laplace/src/vector_funcs.cpp:102-107)."""
-    return np.power(10.0, linspace_accumulated(start, end, size, include_final))
+    """10**linspace_accumulated (B/Laplace/src/vector_funcs.cpp:102-107), by
+    the C library's pow as the reference (numpy's vectorised power differs
+    from it by an ulp on some values)."""
+    return np.array([math.pow(10.0, v) for v in linspace_accumulated(start, end, size, include_final)])
 
 
 def t2_bins(min_t2: float, max_t2: float, num: int, use_log: bool = True):
@@ -180,6 +183,14 @@ def stejskal_tanner(rhs, lhs, n_points=None):
     if n_points is not None:
         x, y = x[:n_points], y[:n_points]
     n = len(x)
-    B = np.sum(x * y) / np.sum(x * x)
-    mse = np.sum((y - B * x) ** 2) / n
-    return float(B), float(np.sqrt(mse * n / (n - 1.0)))
+    # sequential sums in index order, as the reference's loops
+    sxy = sxx = 0.0
+    for xi, yi in zip(x.tolist(), y.tolist()):
+        sxy += xi * yi
+        sxx += xi * xi
+    B = sxy / sxx
+    res = 0.0
+    for xi, yi in zip(x.tolist(), y.tolist()):
+        res += (yi - (B * xi + 0.0)) ** 2
+    mse = res / n
+    return float(B), float(math.sqrt(mse * (n / (n - 1.0))))
