@@ -359,6 +359,14 @@ def wl_small(args):
     run()
     sec = _timed(run, 10)
     steps = float(N) * spe * E
+    eager_sums = SUMS.cpu().numpy().copy()
+    # the same run captured as a HIP graph (walker resets, walk, echo reduction) and replayed
+    graph = R.CpmgGraph(ctx, img, X, Y, Z, SD, echoes=E, steps_per_echo=spe, bc="mirror", delta=delta, energy=EN,
+                        echo_sums=SUMS, start=(X0, Y0, Z0, S0))
+    graph.launch(sync=True)
+    sec_graph = _timed(lambda: graph.launch(sync=True), 10)
+    graph_rel = float(np.max(np.abs(SUMS.cpu().numpy() - eager_sums) / np.abs(eager_sums)))
+    graph.close()
     M = SUMS.cpu().numpy()
     t = np.arange(E + 1) * spe * dt
     decay = np.exp(-t / t2b) * M / M[0]
@@ -380,7 +388,10 @@ def wl_small(args):
                  np.array_equal(EN.cpu().numpy().view(np.uint64), ref["energy"].view(np.uint64)))
     echo_rel = float(np.max(np.abs(M - es1) / np.abs(es1)))
     _line({"metric": "walker-steps/s, CPMG 128^3 10k walkers", "value": steps / sec, "unit": "walker-steps/s",
-           "ms_per_run": sec * 1e3, "dtype": "f64", "data": "synthetic",
+           "ms_per_run": sec * 1e3,
+           "graph": {"ms_per_run": sec_graph * 1e3, "walker_steps_per_s": steps / sec_graph,
+                     "echo_sums_max_rel_vs_eager": graph_rel},
+           "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"CPMG {S}^3 sphere pack (r 8, porosity {phi:.3f}), {N} walkers, {E} echoes x "
                                   f"{spe} steps, mirror, voxel {voxel} um, rho {rho} um/s, T2b {t2b} ms",
                       "launch": f"{tpb}x{k}"},

@@ -162,6 +162,29 @@ rwnmr_status rwnmr_map_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers
 rwnmr_status rwnmr_cpmg_run(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
                             uint32_t steps_per_echo, int32_t bc, uint32_t shift, double *echo_sums);
 
+/* A CPMG run captured once as a HIP graph and replayed with one graph launch
+ * (no reference counterpart: the reference launches and synchronises every
+ * 16 echoes, B/NMR_Simulation/NMR_cpmg_cuda.cpp:1174-1334).  The graph holds,
+ * in order: when `start` is given, copies of its device arrays into w's
+ * (x, y, z, seed; energy from start->energy or 1.0 when w->energy is set;
+ * hits from start->hits or 0 when w->hits is set); the error-word clear; the
+ * walk launches (no locality re-sort) and the echo reductions into the device
+ * echo_sums (echoes+1, nullable).  w and start must be device memory
+ * (on_device = 1).  Every replay (rwnmr_graph_launch, asynchronous on the
+ * context stream; rwnmr_ctx_synchronize waits and reports a noflux walker
+ * started outside the image) repeats that work on the same arrays: with
+ * `start` each replay is one full run from the start state, without it each
+ * replay continues the walk.  Results equal rwnmr_cpmg_run's (walkers bit for
+ * bit, echo sums to rounding).  The graph owns its echo partials and keeps
+ * the pointers it was built with: destroy it before the image, the arrays or
+ * the context. */
+typedef struct rwnmr_graph rwnmr_graph;
+rwnmr_status rwnmr_cpmg_graph_create(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w,
+                                     const rwnmr_walkers *start, uint32_t echoes, uint32_t steps_per_echo, int32_t bc,
+                                     double *echo_sums, rwnmr_graph **out);
+rwnmr_status rwnmr_graph_launch(rwnmr_graph *graph);
+void rwnmr_graph_destroy(rwnmr_graph *graph);
+
 /* CPMG that allocates nothing: rwnmr_cpmg_run for device-resident walkers
  * (on_device must be 1) and a device echo_sums (echoes+1, nullable) that
  * calls no hipMalloc / hipFree and no allocating library routine.  No

@@ -110,6 +110,10 @@ SIGNATURES = {
     "rwnmr_map_run": (I32, [P, P, ct.POINTER(Walkers), U64, I32, U32]),
     "rwnmr_cpmg_run": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
     "rwnmr_cpmg_run_noalloc": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P]),
+    "rwnmr_cpmg_graph_create": (I32, [P, P, ct.POINTER(Walkers), ct.POINTER(Walkers), U32, U32, I32, P,
+                                      ct.POINTER(P)]),
+    "rwnmr_graph_launch": (I32, [P]),
+    "rwnmr_graph_destroy": (None, [P]),
     "rwnmr_cpmg_run_labels": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P, P]),
     "rwnmr_cpmg_run_label_sets": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, U32, P, P]),
     "rwnmr_pfgse_run_sampled": (I32, [P, P, ct.POINTER(Walkers), P, P, P, P, U32, P, P, P, U32, I32, D, I32, U32, P,

@@ -368,6 +368,42 @@ def cpmg_run_label_sets(ctx: Context, img: Image, x, y, z, seed, *, echoes, step
     return dict(x=x, y=y, z=z, seed=seed, energy=energy, echo_sums=es)
 
 
+class CpmgGraph:
+    """A CPMG run captured as a HIP graph (rwnmr_cpmg_graph_create) on
+    device walkers x, y, z, seed (torch CUDA tensors, walked in place; energy
+    and hits optional).  With start=(x0, y0, z0, seed0[, energy0]) every
+    launch() is one full run from that start state; without it launches
+    continue the walk.  Keep the tensors alive while the graph exists."""
+
+    def __init__(self, ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo, bc="mirror",
+                 penalty=None, delta=1.0, energy=None, hits=None, echo_sums=None, start=None):
+        self.ctx = ctx
+        self._keep = (img, x, y, z, seed, penalty, energy, hits, echo_sums, start)
+        n = len(x)
+        self._w = _walker_struct(x, y, z, seed, energy, hits, penalty, delta, n, 1)
+        st = None
+        if start is not None:
+            e0 = start[4] if len(start) > 4 else None
+            self._s = _walker_struct(start[0], start[1], start[2], start[3], e0, None, None, 1.0, n, 1)
+            st = ct.byref(self._s)
+        h = ct.c_void_p()
+        _torch_sync()
+        check(load().rwnmr_cpmg_graph_create(ctx.handle, img.handle, ct.byref(self._w), st, int(echoes),
+                                             int(steps_per_echo), bc_code(bc), _ptr(echo_sums), ct.byref(h)),
+              ctx.handle)
+        self.handle = h
+
+    def launch(self, sync=False):
+        check(load().rwnmr_graph_launch(self.handle), self.ctx.handle)
+        if sync:
+            self.ctx.synchronize()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().rwnmr_graph_destroy(self.handle)
+            self.handle = None
+
+
 def map_run(ctx: Context, img: Image, x, y, z, seed, *, steps, bc="mirror", shift=0, hits=None, sync=True):
     """Hit-count walk (rwnmr_map_run)."""
     dev = _is_dev(x)

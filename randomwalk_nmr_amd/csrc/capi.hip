@@ -1322,6 +1322,156 @@ rwnmr_status rwnmr_cpmg_run_noalloc(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_
     return RWNMR_OK;
 }
 
+}  // extern "C"
+
+// A CPMG run captured as a HIP graph: resets of the walkers from start
+// arrays (optional), the error-word clear, the walk launches and the echo
+// reductions, replayed by one hipGraphLaunch.  The graph owns its echo
+// partials (allocated before the capture), so nothing allocates in the
+// graph and no context buffer can move under it; it records fill kernels
+// and copies only (no memset nodes).
+struct rwnmr_graph {
+    rwnmr_ctx *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double *partials = nullptr;
+};
+
+namespace {
+
+void graph_release(rwnmr_graph *g)
+{
+    if (g->exec)
+        (void)hipGraphExecDestroy(g->exec);
+    if (g->graph)
+        (void)hipGraphDestroy(g->graph);
+    if (g->partials)
+        (void)hipFree(g->partials);
+    delete g;
+}
+
+}  // namespace
+
+extern "C" {
+
+rwnmr_status rwnmr_cpmg_graph_create(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
+                                     const rwnmr_walkers *start, uint32_t echoes, uint32_t steps_per_echo, int32_t bc,
+                                     double *echo_sums, rwnmr_graph **out)
+{
+    rwnmr_status s;
+    if (!c || !out)
+        return RWNMR_ERR_INVALID;
+    *out = nullptr;
+    if ((s = check_image(c, img)) != RWNMR_OK)
+        return s;
+    if ((s = check_walkers(c, img, w, bc, 0)) != RWNMR_OK)
+        return s;
+    if (!w->on_device || (start && (!start->on_device || start->n != w->n || !start->x || !start->y || !start->z ||
+                                    !start->seed)))
+        return fail(c, RWNMR_ERR_INVALID,
+                    "cpmg_graph_create: device walkers, and start arrays (if given) of the same count on the device");
+    const uint64_t n = w->n;
+    const uint32_t blocks = rwnmr::walk_grid_blocks(n, c->shape, 0);
+    uint64_t chunk = echoes ? echoes : 1;
+    if (echo_sums && blocks)
+        chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(2, kMaxPartialBytes / (8ull * blocks)) - 1);
+    if (chunk < echoes && !w->energy && n)
+        return fail(c, RWNMR_ERR_INVALID, "cpmg_graph_create: a run of several launches needs a device energy array");
+    // everything the graph touches exists before the capture: the error word,
+    // the unfolded image and the graph's own partials
+    RW_HIP(c, c->flag.reserve(sizeof(uint32_t)));
+    const WalkerView v{w->x, w->y, w->z, w->seed, w->energy, w->hits, w->penalty};
+    WalkArgs a = base_args(img, v, w, 0, c->flag.as<uint32_t>());
+    a.n_labels = img->n_labels;
+    attach_zimage(c, img, bc, 1, a, true, true, true);
+    a.partial_stride = blocks;
+    a.steps_per_echo = steps_per_echo;
+    a.tile_map = 1;
+    a.n_tiles = blocks;
+    RW_HIP(c, hipStreamSynchronize(c->stream));
+    rwnmr_graph *g = new (std::nothrow) rwnmr_graph();
+    if (!g)
+        return fail(c, RWNMR_ERR_ALLOC, "cpmg_graph_create: no host memory");
+    g->ctx = c;
+    if (echo_sums && blocks &&
+        hipMalloc((void **)&g->partials, 8ull * blocks * (chunk + 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        graph_release(g);
+        return fail(c, RWNMR_ERR_ALLOC, "cpmg_graph_create: no device memory for the echo partials");
+    }
+    auto capture = [&]() -> rwnmr_status {
+        if (start) {
+            RW_HIP(c, hipMemcpyAsync(w->x, start->x, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+            RW_HIP(c, hipMemcpyAsync(w->y, start->y, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+            RW_HIP(c, hipMemcpyAsync(w->z, start->z, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+            RW_HIP(c, hipMemcpyAsync(w->seed, start->seed, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+            const uint32_t nb = (uint32_t)((n + 255) / 256);
+            if (w->energy) {
+                if (start->energy)
+                    RW_HIP(c, hipMemcpyAsync(w->energy, start->energy, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+                else if (n)
+                    hipLaunchKernelGGL(fill_f64_kernel, dim3(nb), dim3(256), 0, c->stream, w->energy, 1.0, n);
+            }
+            if (w->hits) {
+                if (start->hits)
+                    RW_HIP(c, hipMemcpyAsync(w->hits, start->hits, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+                else if (n)
+                    hipLaunchKernelGGL(fill_u32_kernel, dim3(nb), dim3(256), 0, c->stream, w->hits, 0u, n);
+            }
+            RW_HIP(c, hipGetLastError());
+        }
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, c->stream, c->flag.as<uint32_t>(), 0u, 1ull);
+        RW_HIP(c, hipGetLastError());
+        uint32_t done = 0;
+        do {
+            const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
+            a.echoes = ne;
+            a.partials = echo_sums ? g->partials : nullptr;
+            RW_HIP(c, launch_any_walk(a, bc, 1, c->shape, c->stream));
+            if (echo_sums)
+                RW_HIP(c, rwnmr::launch_reduce_rows(g->partials, (uint64_t)ne + 1, blocks, echo_sums + done,
+                                                    c->stream));
+            done += ne;
+        } while (done < echoes);
+        return RWNMR_OK;
+    };
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        graph_release(g);
+        return fail(c, RWNMR_ERR_HIP, "cpmg_graph_create: hipStreamBeginCapture failed");
+    }
+    s = capture();
+    const hipError_t e = hipStreamEndCapture(c->stream, &g->graph);
+    if (s != RWNMR_OK || e != hipSuccess) {
+        (void)hipGetLastError();
+        graph_release(g);
+        return s != RWNMR_OK ? s : fail(c, RWNMR_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        graph_release(g);
+        return fail(c, RWNMR_ERR_HIP, "cpmg_graph_create: hipGraphInstantiate failed");
+    }
+    *out = g;
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_graph_launch(rwnmr_graph *g)
+{
+    if (!g || !g->exec)
+        return RWNMR_ERR_INVALID;
+    RW_HIP(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream));
+    return RWNMR_OK;
+}
+
+void rwnmr_graph_destroy(rwnmr_graph *g)
+{
+    if (!g)
+        return;
+    (void)hipStreamSynchronize(g->ctx->stream);
+    graph_release(g);
+}
+
 rwnmr_status rwnmr_cpmg_field_run(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
                                   uint32_t steps_per_echo, int32_t bc, const double *field,
                                   double gyromagnetic_ratio, double time_interval_ms, double *p_alive, double *phase,
