@@ -136,7 +136,8 @@ def main():
         tpb, k, ms = ctx.autotune(img, X0, Y0, Z0, SEED0, steps_per_echo=spe, bc=args.bc, trial_echoes=1000,
                                   delta=delta)
         log(f"autotune: threads {tpb} x walkers/lane {k} ({ms:.2f} ms per 1000 echoes) in {time.time() - t:.1f}s")
-    tpb, k = ctx.get_launch()
+    tpb, k, form, resort = ctx.get_tuning()
+    log(f"tuning: {tpb} x {k}, image form {form}, re-sort every {resort} echoes")
 
     def one_step():
         X.copy_(X0)
@@ -177,7 +178,7 @@ def main():
     if os.path.exists(COUNTERS_FILE):
         with open(COUNTERS_FILE) as f:
             prof = json.load(f)
-        if prof.get("bc") != args.bc or int(prof.get("walkers_per_lane", 0)) != k or args.unfolded not in (-1, 3):
+        if prof.get("bc") != args.bc or int(prof.get("walkers_per_lane", 0)) != k or form != 3:
             prof = None
     # gather probe: random 16-byte loads from tables held in L2, in the MALL and in HBM
     probe = {}
@@ -240,8 +241,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"CPMG {S}^3 sphere pack (r 6-16, porosity {phi:.3f}), {N} walkers, {E} echoes x "
                                f"{spe} steps, {args.bc} BC, voxel {voxel} um, rho {rho} um/s",
-                   "walker_steps_per_step": walker_steps, "launch": {"threads_per_block": tpb,
-                                                                    "walkers_per_lane": k}},
+                   "walker_steps_per_step": walker_steps,
+                   "launch": {"threads_per_block": tpb, "walkers_per_lane": k,
+                              "image_form": {3: "unfolded + register chunk cache", 0: "folded bitBlock bricks"}.get(
+                                  form, f"mode {form}"),
+                              "echoes_between_resorts": resort}},
         "roofline": roofline(args, prof, probe, kernel_steps_per_s, steps_per_launch, avg_kernel_s, launches, k),
         "cpu_baseline": cpu,
         "checks": {"decay_monotone": decay_ok, "subsample_bit_exact_vs_cpu": parity},

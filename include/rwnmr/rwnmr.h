@@ -137,14 +137,25 @@ typedef struct {
     int32_t on_device;
 } rwnmr_walkers;
 
-/* Autotuner: time a trial_echoes-long CPMG run of copies of the walkers for
- * every launch shape (threads per block 64..1024 x walkers per lane 1/2/4)
- * and keep the fastest in the context.  The reference fixes its launch shape
- * in its config (BLOCKS/THREADSPERBLOCK/ECHOESPERKERNEL,
+/* Autotuner: times trial_echoes-long CPMG runs of copies of the walkers and
+ * keeps the fastest settings in the context, in three steps: (1) the launch
+ * shape, threads per block 64..1024 x walkers per lane 1/2/4 (walk kernel
+ * time); (2) the image form the walk reads, the unfolded Z-order image with
+ * the per-walker register chunk cache against the folded brick-ordered
+ * bitBlock words (rwnmr_ctx_set_unfolded 3 / 0; walk kernel time); (3) for
+ * runs that use the locality ordering, the echoes per launch between walker
+ * re-sorts, trial_echoes / 4, / 2 or trial_echoes (whole-run time, re-sorts
+ * included).  best_* report step 1's shape and the fastest walk time; the
+ * rest is read back with rwnmr_ctx_get_tuning.  The reference fixes all of
+ * this in its config (BLOCKS/THREADSPERBLOCK/ECHOESPERKERNEL,
  * config/default/rwnmr.config:43-47). */
 rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *ctx, const rwnmr_image *img, const rwnmr_walkers *w,
                                 uint32_t steps_per_echo, int32_t bc, uint32_t trial_echoes,
                                 uint32_t *best_threads, uint32_t *best_k, double *best_ms);
+/* The context's current launch shape, image form (rwnmr_ctx_set_unfolded
+ * mode) and echoes between locality re-sorts. */
+rwnmr_status rwnmr_ctx_get_tuning(const rwnmr_ctx *ctx, uint32_t *threads_per_block, uint32_t *walkers_per_lane,
+                                  int32_t *unfolded, uint32_t *resort_every);
 
 /* Map simulation: advance every walker `steps` steps counting wall hits
  * (map_3D_{noflux,periodic,mirror}, B/NMR_Simulation/NMR_Simulation_cuda.cpp:418-721,

@@ -104,6 +104,7 @@ SIGNATURES = {
     "rwnmr_ctx_get_timing": (I32, [P, ct.POINTER(D), ct.POINTER(U64), I32]),
     "rwnmr_ctx_autotune": (I32, [P, P, ct.POINTER(Walkers), U32, I32, U32, ct.POINTER(U32), ct.POINTER(U32),
                                  ct.POINTER(D)]),
+    "rwnmr_ctx_get_tuning": (I32, [P, ct.POINTER(U32), ct.POINTER(U32), ct.POINTER(I32), ct.POINTER(U32)]),
     "rwnmr_image_upload": (I32, [P, P, I32, I32, I32, P, I32, ct.POINTER(P)]),
     "rwnmr_image_destroy": (None, [P]),
     "rwnmr_bitblock_pack": (I32, [P, I32, I32, I32, P]),

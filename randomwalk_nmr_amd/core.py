@@ -112,6 +112,15 @@ class Context:
                                         int(trial_echoes), ct.byref(t), ct.byref(k), ct.byref(ms)), self.handle)
         return t.value, k.value, ms.value
 
+    def get_tuning(self):
+        """(threads_per_block, walkers_per_lane, unfolded_mode, resort_every) now in use
+        (rwnmr_ctx_get_tuning)."""
+        t, k, r = ct.c_uint32(), ct.c_uint32(), ct.c_uint32()
+        u = ct.c_int32()
+        check(load().rwnmr_ctx_get_tuning(self.handle, ct.byref(t), ct.byref(k), ct.byref(u), ct.byref(r)),
+              self.handle)
+        return t.value, k.value, u.value, r.value
+
     def probe_gather(self, table_bytes: int, loads_per_thread: int = 256):
         """Random 16-byte loads per second from a device table of
         table_bytes (rwnmr_probe_gather).  Returns (loads_per_s, kernel_ms)."""

@@ -1184,69 +1184,135 @@ rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *c, const rwnmr_image *img, const rwnm
     const double saved_ms = c->kernel_ms;
     const uint64_t saved_launches = c->kernel_launches;
     const LaunchShape saved = c->shape;
+    const int saved_unfolded = c->unfolded;
+    const uint32_t saved_resort = c->resort_every;
+    auto restore = [&]() {
+        c->timing = was_timing;
+        c->kernel_ms = saved_ms;
+        c->kernel_launches = saved_launches;
+        c->shape = saved;
+        c->unfolded = saved_unfolded;
+        c->resort_every = saved_resort;
+        cleanup();
+    };
+    c->timing = true;
+    (void)hipStreamSynchronize(c->stream);
+    timing_collect(c);
+    // One trial run from the caller's start state; returns the walk kernels'
+    // milliseconds (whole == false) or the whole run's, re-sorts included.
+    auto trial = [&](double &ms, bool whole) -> rwnmr_status {
+        rwnmr_walkers tw = *w;
+        tw.x = tx.as<int32_t>();
+        tw.y = ty.as<int32_t>();
+        tw.z = tz.as<int32_t>();
+        tw.seed = ts.as<uint64_t>();
+        tw.energy = te.as<double>();
+        tw.hits = nullptr;
+        tw.on_device = 1;
+        if (tw.penalty && !w->on_device)
+            tw.penalty = nullptr;  // host penalties: time with the uniform delta instead
+        hipError_t e = hipMemcpyAsync(tw.x, w->x, n * 4, kind, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(tw.y, w->y, n * 4, kind, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(tw.z, w->z, n * 4, kind, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(tw.seed, w->seed, n * 8, kind, c->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(fill_f64_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, c->stream,
+                               tw.energy, 1.0, n);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess)
+            return fail(c, RWNMR_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        c->kernel_ms = 0.0;
+        c->kernel_launches = 0;
+        hipEvent_t t0 = whole ? take_event(c) : nullptr, t1 = whole ? take_event(c) : nullptr;
+        if (whole && (!t0 || !t1))
+            return fail(c, RWNMR_ERR_HIP, "autotune: no events");
+        if (whole)
+            (void)hipEventRecord(t0, c->stream);
+        rwnmr_status st = cpmg_common(c, img, &tw, trial_echoes, steps_per_echo, bc, 0, 1, nullptr, nullptr,
+                                      tsum.as<double>());
+        if (whole)
+            (void)hipEventRecord(t1, c->stream);
+        if (st == RWNMR_OK)
+            st = sync_and_check(c);
+        ms = c->kernel_ms;
+        if (whole) {
+            float f = 0.f;
+            if (st == RWNMR_OK && hipEventElapsedTime(&f, t0, t1) == hipSuccess)
+                ms = f;
+            c->event_pool.push_back(t0);
+            c->event_pool.push_back(t1);
+        }
+        return st;
+    };
+    // 1. launch shape: threads per block x walkers per lane
     LaunchShape best = saved;
     double best_t = 1e300;
     const uint32_t threads[] = {64, 128, 256, 512, 1024};
     const uint32_t ks[] = {1, 2, 4};
-    c->timing = true;
-    (void)hipStreamSynchronize(c->stream);
-    timing_collect(c);
     for (uint32_t t : threads)
         for (uint32_t k : ks) {
             if (t > rwnmr::max_threads_for_k((int)k))
                 continue;
-            rwnmr_walkers tw = *w;
-            tw.x = tx.as<int32_t>();
-            tw.y = ty.as<int32_t>();
-            tw.z = tz.as<int32_t>();
-            tw.seed = ts.as<uint64_t>();
-            tw.energy = te.as<double>();
-            tw.hits = nullptr;
-            tw.on_device = 1;
-            hipError_t e = hipMemcpyAsync(tw.x, w->x, n * 4, kind, c->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(tw.y, w->y, n * 4, kind, c->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(tw.z, w->z, n * 4, kind, c->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(tw.seed, w->seed, n * 8, kind, c->stream);
-            if (e == hipSuccess) {
-                hipLaunchKernelGGL(fill_f64_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, c->stream,
-                                   tw.energy, 1.0, n);
-                e = hipGetLastError();
-            }
-            if (tw.penalty && !w->on_device)
-                tw.penalty = nullptr;  // host penalties: time with the uniform delta instead
-            if (e == hipSuccess)
-                e = hipStreamSynchronize(c->stream);
-            if (e != hipSuccess) {
-                c->timing = was_timing;
-                c->shape = saved;
-                cleanup();
-                return fail(c, RWNMR_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-            }
-            c->kernel_ms = 0.0;
-            c->kernel_launches = 0;
             c->shape = LaunchShape{t, k};
-            s = cpmg_common(c, img, &tw, trial_echoes, steps_per_echo, bc, 0, 1, nullptr, nullptr,
-                            tsum.as<double>());
-            if (s == RWNMR_OK)
-                s = sync_and_check(c);
-            if (s != RWNMR_OK) {
-                c->timing = was_timing;
-                c->shape = saved;
-                cleanup();
+            double ms = 0.0;
+            if ((s = trial(ms, false)) != RWNMR_OK) {
+                restore();
                 return s;
             }
             // more walkers per lane must win by more than 3%: short trials on a
             // cool device favour them, sustained runs do not
             const double margin = k > best.k ? 0.97 : 1.0;
-            if (c->kernel_ms < best_t * margin) {
-                best_t = c->kernel_ms;
+            if (ms < best_t * margin) {
+                best_t = ms;
                 best = LaunchShape{t, k};
             }
         }
+    c->shape = best;
+    // 2. image form (how the bitBlock is staged for the walk): the unfolded
+    // Z-order image with the register chunk cache (the default) against the
+    // folded brick-ordered bitBlock words
+    int best_form = saved_unfolded;
+    if (saved_unfolded != 0) {
+        c->unfolded = 0;
+        double ms = 0.0;
+        if ((s = trial(ms, false)) != RWNMR_OK) {
+            restore();
+            return s;
+        }
+        if (ms < best_t) {
+            best_t = ms;
+            best_form = 0;
+        }
+        c->unfolded = best_form;
+    }
+    // 3. echoes per launch of locality-ordered runs (the re-sort interval),
+    // timed over the whole run with its re-sorts
+    uint32_t best_resort = saved_resort;
+    if (c->locality && n >= c->locality_min_walkers && trial_echoes >= 4) {
+        double best_whole = 1e300;
+        const uint32_t cands[] = {trial_echoes / 4, trial_echoes / 2, trial_echoes};
+        for (uint32_t r : cands) {
+            c->resort_every = std::max<uint32_t>(1u, r);
+            double ms = 0.0;
+            if ((s = trial(ms, true)) != RWNMR_OK) {
+                restore();
+                return s;
+            }
+            if (ms < best_whole) {
+                best_whole = ms;
+                best_resort = c->resort_every;
+            }
+        }
+    }
     c->timing = was_timing;
     c->kernel_ms = saved_ms;
     c->kernel_launches = saved_launches;
     c->shape = best;
+    c->unfolded = best_form;
+    c->resort_every = best_resort;
     cleanup();
     if (best_threads)
         *best_threads = best.threads;
@@ -1254,6 +1320,22 @@ rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *c, const rwnmr_image *img, const rwnm
         *best_k = best.k;
     if (best_ms)
         *best_ms = best_t;
+    return RWNMR_OK;
+}
+
+rwnmr_status rwnmr_ctx_get_tuning(const rwnmr_ctx *c, uint32_t *threads_per_block, uint32_t *walkers_per_lane,
+                                  int32_t *unfolded, uint32_t *resort_every)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if (threads_per_block)
+        *threads_per_block = c->shape.threads;
+    if (walkers_per_lane)
+        *walkers_per_lane = c->shape.k;
+    if (unfolded)
+        *unfolded = c->unfolded;
+    if (resort_every)
+        *resort_every = c->resort_every;
     return RWNMR_OK;
 }
 
