@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     __shared__ uint2 fix[6];
     __shared__ uint8_t keys[kZKeyEntries];
     __shared__ double ldelta[kMaxLabels];
-    __shared__ double racc[kMaxLabels * kZBackThreads];   // R_wl, label-major
+    __shared__ double racc[kLhStride * kZBackThreads];   // R_wl, one row of kLhStride per walker
     uint32_t seg;
     uint64_t tile;
     if (!backward_block(b, seg, tile))
@@ -123,8 +123,9 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     zbuild_key_table(keys, 16u);
     if (threadIdx.x < kMaxLabels)
         ldelta[threadIdx.x] = (int)threadIdx.x < b.n_labels ? a.label_delta[threadIdx.x] : 1.0;
+    double *my_r = racc + threadIdx.x * (uint32_t)kLhStride;
     for (int l = 0; l < kMaxLabels; ++l)
-        racc[(uint32_t)l * blockDim.x + threadIdx.x] = 0.0;
+        my_r[l] = 0.0;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -145,19 +146,17 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     const zrsrc_t zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
-    double *my_r = racc + threadIdx.x;
     ZEntry T;
     zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
-    // a hit's label word is loaded at the hit and applied one step later (or
-    // at the echo's end, before Q moves on), so its latency overlaps the next
+    // a hit's label is loaded at the hit and applied one step later (or at
+    // the echo's end, before Q moves on), so its latency overlaps the next
     // step's chunk load
-    uint32_t pw = 0u, psh = 0u;
+    uint32_t lab = 0u;
     bool pend = false;
     auto settle = [&]() {
         if (pend) {
-            const uint32_t lab = (pw >> psh) & 0xFu;
             e *= ldelta[lab];
-            __hip_atomic_fetch_add(my_r + lab * blockDim.x, Q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(my_r + lab, Q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             pend = false;
         }
     };
@@ -167,9 +166,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
         zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
         settle();
         if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
-            const uint32_t cl = a.zlfold ? zfold_first_octant(c, a.zl) : c;
-            pw = a.zlabels[cl >> 3];
-            psh = (cl & 7u) * 4u;
+            lab = zlabel_at(a, c);
             pend = true;
         } else {
             A = c;
@@ -194,7 +191,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
         constexpr int l = decltype(lc)::value;
         double G = 0.0;
         if (active && l < n_labels)
-            G = (double)h_end[(uint64_t)l * n + j] * Q - my_r[l * blockDim.x];
+            G = (double)h_end[(uint64_t)l * n + j] * Q - my_r[l];
         const double v = wave_sum(G);
         if (lane == 0)
             lds[l][wave] = v;

@@ -159,7 +159,8 @@ struct ZImage {
     uint32_t *words = nullptr;
     rwnmr::ZLayout zl{};
     int state = 0;   // 0: not built, 1: built, -1: not available (over 2^32 voxels or no memory)
-    uint32_t *labels = nullptr;   // 4-bit solid labels in the same order (per-label walks)
+    uint8_t *labels = nullptr;    // solid labels, one byte each, in the same order (per-label walks;
+                                  // the first octant only when zoctant_layout(zl))
     int lstate = 0;  // as state
 };
 
@@ -447,9 +448,9 @@ void free_zimages(rwnmr_image *img)
     }
 }
 
-// Unfolded label volume of an image with solid labels (built on first use);
-// null when unavailable.
-const uint32_t *zlabels_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
+// Label volume of an image with solid labels (built on first use); null
+// when unavailable.
+const uint8_t *zlabels_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
 {
     if (!img->labels_blocked)
         return nullptr;
@@ -457,15 +458,18 @@ const uint32_t *zlabels_for(rwnmr_ctx *c, const rwnmr_image *img, int bc)
     if (!zi)
         return nullptr;
     ZImage &z = img->z[bc == rwnmr::kMirror ? 1 : 0];
+    if (z.lstate == 0 && z.zl.bits > 31)
+        z.lstate = -1;   // labelled walks on the unfolded image address below 2^31
     if (z.lstate == 0) {
-        if (hipMalloc((void **)&z.labels, rwnmr::zlabel_words(z.zl) * 4) != hipSuccess) {
+        const bool octant = bc == rwnmr::kMirror && rwnmr::zoctant_layout(z.zl);
+        if (hipMalloc((void **)&z.labels, rwnmr::zlabel_bytes(z.zl, octant)) != hipSuccess) {
             (void)hipGetLastError();
             z.labels = nullptr;
             z.lstate = -1;
             return nullptr;
         }
         if (rwnmr::launch_zlabels_build(img->labels_blocked, img->layout, img->W, img->H, img->D,
-                                        bc == rwnmr::kMirror ? 1 : 0, z.zl, z.labels, c->stream) != hipSuccess) {
+                                        bc == rwnmr::kMirror ? 1 : 0, z.zl, octant, z.labels, c->stream) != hipSuccess) {
             (void)hipStreamSynchronize(c->stream);
             (void)hipFree(z.labels);
             z.labels = nullptr;
@@ -492,7 +496,7 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
 {
     a.zwords = nullptr;
     a.zlabels = nullptr;
-    a.zlfold = 0;
+    rwnmr::zoctant_constants(a, false);
     a.znoflux = 0;
     a.zmflags = 0;
     if (emode < 0 || emode > 2 || a.shift != 0)
@@ -503,7 +507,7 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
             return;
         zbc = rwnmr::kPeriodic;
     }
-    const uint32_t *zlab = nullptr;
+    const uint8_t *zlab = nullptr;
     if (emode == 2 && (!build || !(zlab = zlabels_for(c, img, zbc))))
         return;
     if (const ZImage *z = zimage_for(c, img, zbc, build)) {
@@ -511,8 +515,9 @@ void attach_zimage(rwnmr_ctx *c, const rwnmr_image *img, int bc, int emode, Walk
         a.zl = z->zl;
         a.zcache = c->unfolded >= 2 ? (uint32_t)(c->unfolded - 1) : 0u;
         a.zlabels = zlab;
-        // mirror image of power-of-two periods: labels read at the first-octant address
-        a.zlfold = (zlab && bc == rwnmr::kMirror && !z->zl.wrapfix) ? 1u : 0u;
+        // mirror image of equal power-of-two periods: labels kept for the
+        // first octant and read at the folded address
+        rwnmr::zoctant_constants(a, zlab && bc == rwnmr::kMirror && rwnmr::zoctant_layout(z->zl));
         a.znoflux = bc == rwnmr::kNoflux ? 1u : 0u;
         a.zmflags = (bc == rwnmr::kMirror && mflags_ok && c->unfolded >= 3) ? 1u : 0u;
     }

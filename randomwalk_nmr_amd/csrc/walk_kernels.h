@@ -10,6 +10,9 @@ namespace rwnmr {
 static constexpr uint32_t kXcds = 8;     // MI355X: 8 accelerator dies, each with its own L2
 
 static constexpr int kMaxLabels = 8;   // solid labels per image (per-label relaxivity)
+// per-label hit counts of a labelled walk in LDS: one row of kMaxLabels
+// counts per walker, padded to an odd stride so a wave's lanes hit distinct banks
+static constexpr int kLhStride = kMaxLabels + 1;
 
 // Everything one walk launch reads or writes.  Walker arrays are device
 // pointers of length n (structure of arrays).
@@ -42,8 +45,10 @@ struct WalkArgs {
     uint32_t zcache;                 // chunk cache in registers: 0 none, 1 16-byte (word select), 2/3 16-byte, pipelined
     uint32_t znoflux;                // 1: noflux walk on the periodic unfolded image (zl of that image)
     uint32_t zmflags;                // 1: mirror walk started in the first octant with reflection flags
-    const uint32_t *zlabels;         // unfolded solid labels, 4 bits per voxel (per-label walks)
-    uint32_t zlfold;                 // 1: labels read at the first-octant address (mirror, power-of-two periods)
+    const uint8_t *zlabels;          // solid labels, one byte per voxel in the unfolded image's address order
+                                     // (per-label walks; with zlfold the first octant's addresses only)
+    uint32_t zlfold;                 // 1: labels read at the first-octant address (zoctant)
+    uint32_t zlf_shift, zlf_mul, zlf_shift2, zlf_mask;   // zoctant constants (zoctant_constants)
     uint32_t n_label_sets;           // label-set walks: label_delta holds [n_label_sets][n_labels]
     double *set_energy;              // label-set walks: nullable [8][n] energies carried between launches
     // internal-field CPMG (emode 3): field[x + y W + z W H], phase step gammatau,

@@ -339,36 +339,25 @@ __device__ __forceinline__ uint32_t zchunk_wall(uint32_t c, const uint4 &QV)
     return top >> 31;
 }
 
-// Solid label (4 bits per voxel, same address order) of address c.
-__device__ __forceinline__ uint32_t zlabel(const uint32_t *zlabels, uint32_t c)
+// Label-volume address of address c: the first-octant address when
+// a.zlfold, else c itself (t = 0, a.zlf_mask all ones).  The mirror image of
+// equal power-of-two periods 2^k is its own reflection u -> 2^k - 1 - u
+// along every axis, which complements the axis' bits, and the voxel there
+// carries the same label.  Address bit 3i + a holds bit i of axis a, so the
+// axes to reflect are the bits of t = c >> (3k - 3), and their complement
+// masks below the top three bits add up to t x (1 + 8 + ... + 8^(k-2))
+// (disjoint bit positions, no carries); zoctant_constants splits that
+// multiplier into a 24-bit part (a.zlf_mul) and a shifted top term.
+__device__ __forceinline__ uint32_t zoctant(uint32_t c, const WalkArgs &a)
 {
-    return (zlabels[c >> 3] >> ((c & 7u) * 4u)) & 0xFu;
+    const uint32_t t = __builtin_amdgcn_ubfe(c, a.zlf_shift, 3u);
+    return (c ^ ((t << a.zlf_shift2) + __umul24(t, a.zlf_mul))) & a.zlf_mask;
 }
 
-// First-octant address of c in the mirror image of power-of-two periods:
-// the image is its own reflection u -> period - 1 - u along every axis, and
-// along an axis of 2^k addresses that reflection complements the axis' bits,
-// so each axis whose top bit is set has all its bits flipped (its top bit
-// then clears).  The voxel there carries the same label, and the label reads
-// of a walk stay in one eighth of the label volume.
-__device__ __forceinline__ uint32_t zfold_first_octant(uint32_t c, const ZLayout &zl)
-{
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        const uint32_t m = zl.m[ax];
-        if (m) {
-            const uint32_t top = 31u - (uint32_t)__builtin_clz(m);
-            c ^= (uint32_t)__builtin_amdgcn_sbfe((int)c, top, 1u) & m;
-        }
-    }
-    return c;
-}
-
-// Label of the voxel at address c of a labelled walk (a.zlfold: read at the
-// first-octant address).
+// Label of the voxel at address c of a labelled walk.
 __device__ __forceinline__ uint32_t zlabel_at(const WalkArgs &a, uint32_t c)
 {
-    return zlabel(a.zlabels, a.zlfold ? zfold_first_octant(c, a.zl) : c);
+    return a.zlabels[zoctant(c, a)];
 }
 
 // One tile (blockDim.x * K consecutive walkers from tile * blockDim.x * K)
@@ -384,9 +373,9 @@ __device__ __forceinline__ uint32_t zlabel_at(const WalkArgs &a, uint32_t c)
 // the byte-key table).
 // NOFLUX: noflux walk on the periodic unfolded image (tab: the noflux table,
 // keys built for 32-byte entries); walkers must start inside the image.
-// EMODE 2 with K == 1 keeps the per-label hit counts in LDS (lh: kMaxLabels
-// x blockDim entries, label-major; one LDS add per hit instead of a compare
-// and add per label), else in registers.
+// EMODE 2 with K == 1 keeps the per-label hit counts in LDS (lh: blockDim
+// rows of kLhStride entries; one LDS add per hit instead of a compare and
+// add per label), else in registers.
 // MFLAGS: mirror walk with reflection flags (tab / fix: zbuild_table_flags).
 // NS == 8 (EMODE 2, K == 1, pipelined chunk cache): label-set walk, each
 // walker carrying 8 energies, one per set of label deltas (ldelta: 8 rows of
@@ -425,11 +414,12 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     double E[K], DL[K];
     double ES[NS];   // label-set walks: one energy per set
     uint32_t LH[K][NL];
-    // labelled walks: a hit's label word is loaded at the hit and applied one
-    // step later (or at the echo's end), so its latency overlaps the next
-    // step's chunk load instead of following the hit's own
-    uint32_t PW[K], PSH[K];
+    // labelled walks: a hit's label is loaded at the hit and applied one step
+    // later (or at the echo's end), so its latency overlaps the next step's
+    // chunk load instead of following the hit's own
+    uint32_t PW[K];
     bool PEND[K];
+    uint32_t *my_lh = LH_LDS ? lh + threadIdx.x * (uint32_t)kLhStride : nullptr;
     bool act[K];
     static_for<K>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
@@ -464,7 +454,6 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         }
         P[k] = 0;
         PW[k] = 0u;
-        PSH[k] = 0u;
         PEND[k] = false;
         if (NS > 1)
             static_for<NS>([&](auto qc) {
@@ -480,8 +469,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         });
         if (LH_LDS)
             for (int l = 0; l < kMaxLabels; ++l)
-                lh[(uint32_t)l * blockDim.x + threadIdx.x] =
-                    (a.label_hits && act[k] && l < n_labels) ? a.label_hits[(uint64_t)l * n + j] : 0u;
+                my_lh[l] = (a.label_hits && act[k] && l < n_labels) ? a.label_hits[(uint64_t)l * n + j] : 0u;
     });
 
     auto echo_value = [&]() {
@@ -516,7 +504,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     // the per-label hit count, in hit order
     auto apply_label = [&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const uint32_t lab = (PW[k] >> PSH[k]) & 0xFu;
+        const uint32_t lab = PW[k];
         if constexpr (NS > 1) {
             static_for<NS>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -526,8 +514,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             E[k] *= ldelta[lab];
         }
         if (LH_LDS) {
-            __hip_atomic_fetch_add(&lh[lab * blockDim.x + threadIdx.x], 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(my_lh + lab, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             static_for<NL>([&](auto lc) {
                 constexpr int l = decltype(lc)::value;
@@ -615,9 +602,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             if (CACHE >= 64 && EMODE == 2) {
                 if (__builtin_expect(wall != 0u, 0)) {
                     Hc[k] += 1u;
-                    const uint32_t c = a.zlfold ? zfold_first_octant(C[k], a.zl) : C[k];
-                    PW[k] = a.zlabels[c >> 3];
-                    PSH[k] = (c & 7u) * 4u;
+                    PW[k] = zlabel_at(a, C[k]);
                     PEND[k] = true;
                 } else {
                     A[k] = C[k];
@@ -634,8 +619,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 const uint32_t lab = zlabel_at(a, C[k]);
                 E[k] *= ldelta[lab];
                 if (LH_LDS) {
-                    __hip_atomic_fetch_add(&lh[lab * blockDim.x + threadIdx.x], 1u, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(my_lh + lab, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 } else {
                     static_for<NL>([&](auto lc) {
                         constexpr int l = decltype(lc)::value;
@@ -706,7 +690,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             if (EMODE == 2 && a.label_hits) {
                 if (LH_LDS) {
                     for (int l = 0; l < n_labels; ++l)
-                        a.label_hits[(uint64_t)l * n + id] = lh[(uint32_t)l * blockDim.x + threadIdx.x];
+                        a.label_hits[(uint64_t)l * n + id] = my_lh[l];
                 } else {
                     static_for<NL>([&](auto lc) {
                         constexpr int l = decltype(lc)::value;

@@ -22,13 +22,50 @@ inline uint64_t zimage_words(const ZLayout &zl) { return zl.bits > 7 ? (1ull << 
 hipError_t launch_zimage_build(const uint64_t *blocks, const BrickLayout &L, int W, int H, int D, int mirror,
                                const ZLayout &zl, uint32_t *out, hipStream_t st);
 
-// Words of the unfolded 4-bit label volume (8 voxels per word, at least one).
-inline uint64_t zlabel_words(const ZLayout &zl) { return zl.bits > 3 ? (1ull << (zl.bits - 3)) : 1ull; }
+// Mirror image of three equal power-of-two periods 2^k, address bit 3i + a
+// holding bit i of axis a: the top three address bits are the axes' top
+// bits, so the first octant is the first eighth of the addresses and the
+// labels can be kept for it alone (read at zoctant(c)).
+inline bool zoctant_layout(const ZLayout &zl)
+{
+    if (zl.wrapfix || zl.bits < 3 || zl.bits % 3 != 0)
+        return false;
+    uint32_t p = 0;
+    for (uint32_t i = 0; i < zl.bits / 3; ++i)
+        p |= 1u << (3 * i);
+    return zl.m[0] == p && zl.m[1] == p << 1 && zl.m[2] == p << 2;
+}
 
-// Build the unfolded label volume from the device labels (64 bytes per block
-// of `blocks`' brick layout L); 0 outside the periods.
+// Constants of zoctant in a: the fold when `fold` (zoctant_layout(a.zl)
+// holds), else the identity (t = c >> 31 = 0 below 2^31 addresses).  With k
+// = bits / 3 the fold's multiplier is P = 1 + 8 + ... + 8^(k-2); it fits the
+// 24-bit multiply up to k = 9, and for k = 10 its top term 2^24 is added as
+// t << 24 instead (a shift of 31 otherwise leaves only bits above the mask).
+inline void zoctant_constants(WalkArgs &a, bool fold)
+{
+    a.zlfold = fold ? 1u : 0u;
+    a.zlf_shift = fold ? a.zl.bits - 3 : 31u;
+    uint32_t p = 0;
+    for (uint32_t i = 0; fold && i + 1 < a.zl.bits / 3; ++i)
+        p |= 1u << (3 * i);
+    a.zlf_shift2 = (p >> 24) ? 24u : 31u;
+    a.zlf_mul = p & 0xFFFFFFu;
+    a.zlf_mask = fold ? (1u << (a.zl.bits - 3)) - 1u : 0xFFFFFFFFu;
+}
+
+// Bytes of the label volume: one per address, or one per first-octant
+// address (octant); a multiple of 4.
+inline uint64_t zlabel_bytes(const ZLayout &zl, bool octant)
+{
+    const uint64_t b = 1ull << (octant ? zl.bits - 3 : zl.bits);
+    return b < 4 ? 4 : b;
+}
+
+// Build the label volume (one byte per voxel, the unfolded image's address
+// order; octant: the first octant's addresses only) from the device labels
+// (64 bytes per block of `blocks`' brick layout L); 0 outside the periods.
 hipError_t launch_zlabels_build(const uint8_t *labels_blocked, const BrickLayout &L, int W, int H, int D, int mirror,
-                                const ZLayout &zl, uint32_t *out, hipStream_t st);
+                                const ZLayout &zl, bool octant, uint8_t *out, hipStream_t st);
 
 // keys[i] = unfolded address of walker i >> 5 (its 32-bit word), iota[i] = i.
 hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, uint64_t n, const ZLayout &zl, int mflags,

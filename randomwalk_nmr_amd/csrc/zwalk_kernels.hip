@@ -20,7 +20,7 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
     __shared__ uint2 fix[MFLAGS ? 8 * kZFlagStride / 16 : 6];
     __shared__ uint8_t keys[TAB == 2 ? kZKeyEntries : 4];
     __shared__ double ldelta[kMaxLabels];
-    extern __shared__ uint32_t zlh[];   // EMODE 2, K == 1: per-label hit counts (kMaxLabels x blockDim)
+    extern __shared__ uint32_t zlh[];   // EMODE 2, K == 1: per-label hit counts (blockDim x kLhStride)
     const uint64_t tile = tile_of_block(a);
     if (a.tile_map && tile >= a.n_tiles)
         return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
@@ -263,26 +263,27 @@ __global__ __launch_bounds__(256) void zimage_build_kernel(const uint64_t *block
     out[w] = word;
 }
 
-// One thread per output word (8 labels of 4 bits).
+// One thread per output word (4 labels of one byte, addresses 4w .. 4w + 3;
+// with octant these are first-octant addresses, whose axis coordinates
+// stay below the half periods).
 __global__ __launch_bounds__(256) void zlabels_build_kernel(const uint8_t *labels_blocked, BrickLayout L, int W, int H,
                                                            int D, int mirror, ZLayout zl, uint32_t *out,
-                                                           uint64_t nwords)
+                                                           uint64_t nwords, uint64_t naddr)
 {
     const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nwords)
         return;
-    const uint64_t nbits = 1ull << zl.bits;
     uint32_t word = 0;
-    for (uint32_t j = 0; j < 8; ++j) {
-        const uint64_t A64 = w * 8u + j;
-        if (A64 >= nbits)
+    for (uint32_t j = 0; j < 4; ++j) {
+        const uint64_t A64 = w * 4u + j;
+        if (A64 >= naddr)
             break;
         const uint32_t A = (uint32_t)A64;
         const uint32_t ux = zextract(A, zl.m[0]), uy = zextract(A, zl.m[1]), uz = zextract(A, zl.m[2]);
         if (ux < zl.period[0] && uy < zl.period[1] && uz < zl.period[2]) {
             const int ix = zunfold(ux, W, mirror), iy = zunfold(uy, H, mirror), iz = zunfold(uz, D, mirror);
             const uint32_t lab = labels_blocked[(uint64_t)brick_block_index(ix, iy, iz, L) * 64u + bit_index(ix, iy, iz)];
-            word |= (lab & 0xFu) << (4u * j);
+            word |= (lab & 0xFFu) << (8u * j);
         }
     }
     out[w] = word;
@@ -316,11 +317,12 @@ hipError_t launch_zimage_build(const uint64_t *blocks, const BrickLayout &L, int
 }
 
 hipError_t launch_zlabels_build(const uint8_t *labels_blocked, const BrickLayout &L, int W, int H, int D, int mirror,
-                                const ZLayout &zl, uint32_t *out, hipStream_t st)
+                                const ZLayout &zl, bool octant, uint8_t *out, hipStream_t st)
 {
-    const uint64_t nw = zlabel_words(zl);
+    const uint64_t nw = zlabel_bytes(zl, octant) / 4;
+    const uint64_t naddr = 1ull << (octant ? zl.bits - 3 : zl.bits);
     hipLaunchKernelGGL(zlabels_build_kernel, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, st, labels_blocked, L, W,
-                       H, D, mirror, zl, out, nw);
+                       H, D, mirror, zl, reinterpret_cast<uint32_t *>(out), nw, naddr);
     return hipGetLastError();
 }
 
@@ -338,7 +340,7 @@ template <int EMODE, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false, bool
 static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
     // EMODE 2 with one walker per lane keeps the per-label hit counts in LDS
-    const size_t lh_bytes = EMODE == 2 ? (size_t)kMaxLabels * ls.threads * sizeof(uint32_t) : 0;
+    const size_t lh_bytes = EMODE == 2 ? (size_t)kLhStride * ls.threads * sizeof(uint32_t) : 0;
     switch (ls.k) {
     case 1: hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS>), dim3(blocks), dim3(ls.threads), lh_bytes, st, a); break;
     case 2: hipLaunchKernelGGL((zwalk_kernel<EMODE, 2, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS>), dim3(blocks), dim3(ls.threads), 0, st, a); break;
