@@ -114,20 +114,54 @@ __device__ __forceinline__ void wave_halve(double *v, bool bit)
     }
 }
 
-// Sums over the 64 lanes of 8 values per lane at once (a reduce-scatter: 3
-// halving steps inside rows, then a lane pair and the 4 rows): every lane
-// returns the wave total of value (lane >> 1) & 7.  Fixed order, so
-// deterministic; about 7 VALU per value instead of a 6-step shuffle each.
-__device__ __forceinline__ double wave_reduce_scatter8(double (&v)[8])
+// Lanes l and l ^ 32 (ROW16 false) or l ^ 16 (ROW16 true) both hold x and
+// y: the lane with that bit clear returns x summed over the pair, the lane
+// with it set y summed over the pair, the lower lane's value first in both.
+// One gfx950 permlane swap per 32-bit half moves both values (the swap
+// exchanges the upper lanes of its first operand with the lower lanes of
+// its second), so a halving step costs 3 VALU and no select.
+template <bool ROW16>
+__device__ __forceinline__ double swap_halve(double x, double y)
 {
+    const int xl = __double2loint(x), xh = __double2hiint(x), yl = __double2loint(y), yh = __double2hiint(y);
+    const auto l = ROW16 ? __builtin_amdgcn_permlane16_swap(xl, yl, false, false)
+                         : __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto h = ROW16 ? __builtin_amdgcn_permlane16_swap(xh, yh, false, false)
+                         : __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+
+// Sums over the 64 lanes of N = 4, 8 or 16 values per lane at once (a
+// reduce-scatter): halvings across lane ^ 32 and lane ^ 16 by swap_halve,
+// then (N >= 8) across bit 3 by the row mirror and (N == 16) bit 2 by the
+// half-row mirror with selects, then plain DPP sums over the remaining lane
+// bits.  Lane l returns the wave total of value l >> (6 - log2 N): lanes
+// 16q (N = 4), 8q (N = 8) or 4q (N = 16) hold value q.  About 5.3, 4.3 and
+// 3.9 VALU per value.  Fixed order, so deterministic.  The whole wave must
+// be active; v is overwritten.
+template <int N>
+__device__ __forceinline__ double wave_reduce_scatter(double (&v)[N])
+{
+    static_assert(N == 4 || N == 8 || N == 16, "4, 8 or 16 values");
     const int lane = threadIdx.x & (kWave - 1);
-    wave_halve<0x140, 4>(v, (lane & 8) != 0);   // row mirror: bit 3
-    wave_halve<0x141, 2>(v, (lane & 4) != 0);   // half-row mirror: bit 2
-    wave_halve<0x4E, 1>(v, (lane & 2) != 0);    // lane ^ 2: bit 1
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i)
+        v[i] = swap_halve<false>(v[i], v[i + N / 2]);   // bit 5
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i)
+        v[i] = swap_halve<true>(v[i], v[i + N / 4]);    // bit 4
+    if constexpr (N >= 8)
+        wave_halve<0x140, N / 8>(v, (lane & 8) != 0);   // row mirror: bit 3
+    if constexpr (N >= 16)
+        wave_halve<0x141, N / 16>(v, (lane & 4) != 0);  // half-row mirror: bit 2
     double r = v[0];
-    r += dpp_f64<0xB1>(r);                      // lane ^ 1
-    r = xor_rows_sum<true>(r);                  // lane ^ 16
-    return xor_rows_sum<false>(r);              // lane ^ 32
+    r += dpp_f64<0xB1>(r);                              // lane ^ 1
+    r += dpp_f64<0x4E>(r);                              // lane ^ 2
+    if constexpr (N < 16)
+        r += dpp_f64<0x141>(r);                         // half-row mirror
+    if constexpr (N < 8)
+        r += dpp_f64<0x140>(r);                         // row mirror
+    return r;
 }
 
 // Image geometry as plain values (uniform across the grid).
@@ -300,11 +334,88 @@ __device__ __forceinline__ void stage_echo(double (*lds)[kEchoCols], uint32_t ec
     }
 }
 
-// Echo sums of 8 energy sets per walker: one reduce-scatter gives every lane
-// the wave total of set (lane >> 1) & 7, lanes 0, 2, ..., 14 stage their
-// set's wave total in LDS for kEchoChunk echoes, and the block then writes
-// one partial per (set, echo): out[set * set_stride + row * row_stride].
-// About 7 VALU per set and echo instead of 15 for separate half-wave sums.
+// Stage the block sums of the echoes echo & ~3 .. echo, whose per-lane
+// values are v[0 .. (echo & 3)] (entries past echo & 3 are ignored; v is
+// overwritten, so the reduction works in place): one 4-value reduce-scatter
+// (21 VALU, about 5 per echo instead of a 15-VALU half-wave sum each)
+// leaves the wave total of v[r] in row r, whose first lane stages it.
+// Called when echo & 3 == 3 or echo == last_echo; every kEchoChunk echoes
+// (a multiple of 4) and at the last echo the block writes its partials
+// out[row * row_stride] for the staged rows.  All threads of the block must
+// call this, with the whole wave active.
+__device__ __forceinline__ void stage_echo4(double (*lds)[kEchoCols], uint32_t echo, uint32_t last_echo,
+                                            double (&v)[4], double *out, uint64_t row_stride)
+{
+    static_assert(kEchoChunk % 4 == 0, "echo groups of 4 inside a chunk");
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const double r = wave_reduce_scatter<4>(v);
+    const uint32_t g0 = echo & ~3u;
+    const uint32_t row = (uint32_t)lane >> 4;
+    if ((lane & 15) == 0 && g0 + row <= echo)
+        lds[(g0 + row) % kEchoChunk][wave] = r;
+    const int slot = echo % kEchoChunk;
+    if (slot == kEchoChunk - 1 || echo == last_echo) {
+        __syncthreads();
+        const int cnt = slot + 1;
+        if ((int)threadIdx.x < cnt) {
+            double s = 0.0;
+            for (int q = 0; q < nw; ++q)
+                s += lds[threadIdx.x][q];
+            const uint64_t orow = (uint64_t)(echo - slot) + threadIdx.x;
+            out[orow * row_stride] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// Walk echoes 1 .. echoes (walk_echo() advances every lane's walkers by one
+// echo; echo_value() is the lane's value for the echo sums) and stage the
+// echo sums into part (nullable; staging as stage_echo / stage_echo4).
+// GROUP4 reduces the values of 4 echoes at once (echoes 4g .. 4g + 3, echo
+// 0 the start state), with the loop unrolled over a group so the values
+// stay in registers; otherwise one half-wave sum per echo.  The image walks
+// of one walker per lane (folded, unfolded, batched, internal field) use
+// GROUP4, so the echo sums of the same energies agree bit for bit between
+// their kernels.
+// All threads of the block call it.
+template <bool GROUP4, class WalkEcho, class EchoValue>
+__device__ __forceinline__ void run_echoes(uint32_t echoes, double (*lds)[kEchoCols], double *part,
+                                           uint64_t part_stride, WalkEcho &&walk_echo, EchoValue &&echo_value)
+{
+    if constexpr (!GROUP4) {
+        if (part)
+            stage_echo(lds, 0, echoes, echo_value(), part, part_stride);
+        for (uint32_t echo = 1; echo <= echoes; ++echo) {
+            walk_echo();
+            if (part)
+                stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
+        }
+    } else {
+        double EB[4] = {echo_value(), 0.0, 0.0, 0.0};
+        for (uint32_t g0 = 0;; g0 += 4) {
+            static_for<4>([&](auto jc) {
+                constexpr uint32_t j = decltype(jc)::value;
+                if ((j > 0 || g0 > 0) && g0 + j <= echoes) {   // uniform
+                    walk_echo();
+                    EB[j] = echo_value();
+                }
+            });
+            const bool end = g0 + 3 >= echoes;
+            if (part)
+                stage_echo4(lds, end ? echoes : g0 + 3, echoes, EB, part, part_stride);
+            if (end)
+                break;
+        }
+    }
+}
+
+// Echo sums of 8 energy sets per walker: one reduce-scatter gives lanes 8q
+// .. 8q + 7 the wave total of set q, lanes 0, 8, ..., 56 stage their set's
+// wave total in LDS for kEchoChunk echoes, and the block then writes one
+// partial per (set, echo): out[set * set_stride + row * row_stride].  About
+// 4.3 VALU per set and echo instead of 15 for separate half-wave sums.
 // Fixed order, so deterministic.  All threads of the block call it, with
 // the whole wave active.
 __device__ __forceinline__ void stage_echo_sets8(double (*lds8)[8][kMaxWaves], uint32_t echo, uint32_t last_echo,
@@ -318,10 +429,10 @@ __device__ __forceinline__ void stage_echo_sets8(double (*lds8)[8][kMaxWaves], u
 #pragma unroll
     for (int q = 0; q < 8; ++q)
         t[q] = v[q];
-    const double r = wave_reduce_scatter8(t);
+    const double r = wave_reduce_scatter<8>(t);
     const int slot = echo % kEchoChunk;
-    if (lane < 16 && (lane & 1) == 0)
-        lds8[slot][lane >> 1][wave] = r;
+    if ((lane & 7) == 0)
+        lds8[slot][lane >> 3][wave] = r;
     if (slot == kEchoChunk - 1 || echo == last_echo) {
         __syncthreads();
         const int cnt = slot + 1;
@@ -370,8 +481,8 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
     (void)active;
     for (uint32_t k0 = 0; k0 < p.n_k; k0 += kPfgseKChunk) {
         const uint32_t kc = p.n_k - k0 < (uint32_t)kPfgseKChunk ? p.n_k - k0 : (uint32_t)kPfgseKChunk;
-        // groups of 8 k: the cosines of the group, then one reduce-scatter
-        // per energy set (lane 2j of lanes 0..15 holds the wave sum of k j)
+        // groups of 8 k: the cosines of the group, then the reduce-scatters
+        // of their terms
         for (uint32_t g0 = 0; g0 < kc; g0 += 8) {
             double cs[8];
 #pragma unroll
@@ -390,17 +501,32 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
                     }
                 }
             }
-            static_for<MAXSETS>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                if (q < nsets) {
+            // sets in pairs: one 16-value reduce-scatter (lanes 4(8p + j) ..
+            // + 3 hold k j of set q0 + p), a lone last set by an 8-value one
+            // (lanes 8j .. 8j + 7)
+            static_for<(MAXSETS + 1) / 2>([&](auto pc) {
+                constexpr int q0 = 2 * decltype(pc)::value;
+                if (q0 + 1 < nsets) {
+                    double t[16];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        t[j] = cs[j] * e[q0];
+                        t[8 + j] = cs[j] * e[q0 + 1];
+                    }
+                    const double v = wave_reduce_scatter<16>(t);
+                    const uint32_t q = (uint32_t)q0 + ((uint32_t)lane >> 5);
+                    const uint32_t kk = g0 + (((uint32_t)lane >> 2) & 7u);
+                    if ((lane & 3) == 0 && kk < kc)
+                        lds[q * kPfgseKChunk + kk][wave] = v;
+                } else if (q0 < nsets) {
                     double t[8];
 #pragma unroll
                     for (int j = 0; j < 8; ++j)
-                        t[j] = cs[j] * e[q];
-                    const double v = wave_reduce_scatter8(t);
-                    const uint32_t kk = g0 + (uint32_t)(lane >> 1);
-                    if ((lane & 0x31) == 0 && kk < kc)
-                        lds[q * kPfgseKChunk + kk][wave] = v;
+                        t[j] = cs[j] * e[q0];
+                    const double v = wave_reduce_scatter<8>(t);
+                    const uint32_t kk = g0 + ((uint32_t)lane >> 3);
+                    if ((lane & 7) == 0 && kk < kc)
+                        lds[q0 * kPfgseKChunk + kk][wave] = v;
                 }
             });
         }
@@ -496,10 +622,7 @@ __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, doub
         });
         return v;
     };
-    if (part)
-        stage_echo(lds, 0, echoes, echo_value(), part, part_stride);
-
-    for (uint32_t echo = 1; echo <= echoes; ++echo) {
+    run_echoes<K == 1>(echoes, lds, part, part_stride, [&]() {
         for (uint32_t st = 0; st < spe; ++st) {
             Move mv[K];
             uint64_t word[K];
@@ -531,9 +654,7 @@ __device__ __forceinline__ void walk_tile(const WalkArgs &a, uint64_t tile, doub
                 }
             });
         }
-        if (part)
-            stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
-    }
+    }, echo_value);
 
     static_for<K>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
@@ -612,9 +733,7 @@ __device__ __forceinline__ void walk_tile_field(const WalkArgs &a, uint64_t tile
         });
         return v;
     };
-    if (part)
-        stage_echo(lds, 0, echoes, echo_value(), part, part_stride);
-    for (uint32_t echo = 1; echo <= echoes; ++echo) {
+    run_echoes<K == 1>(echoes, lds, part, part_stride, [&]() {
         for (int inv = 0; inv < 2; ++inv) {
             static_for<K>([&](auto kc) { PH[decltype(kc)::value] = -PH[decltype(kc)::value]; });
             for (uint32_t st = 0; st < half; ++st) {
@@ -641,9 +760,7 @@ __device__ __forceinline__ void walk_tile_field(const WalkArgs &a, uint64_t tile
                 });
             }
         }
-        if (part)
-            stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
-    }
+    }, echo_value);
     static_for<K>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         if (w[k].active) {

@@ -631,16 +631,8 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     };
 
     const uint64_t set_stride = (uint64_t)(echoes + 1) * part_stride;
-    auto stage = [&](uint32_t echo) {
-        if constexpr (NS > 1)
-            stage_echo_sets8(lds8, echo, echoes, ES, part, part_stride, set_stride);
-        else
-            stage_echo(lds, echo, echoes, echo_value(), part, part_stride);
-    };
-    if (part)
-        stage(0);
     uint32_t since = 0;
-    for (uint32_t echo = 1; echo <= echoes; ++echo) {
+    auto walk_echo = [&]() {
         uint32_t left = spe;
         while (left) {
             const uint32_t ns = left < kZFlush - since ? left : kZFlush - since;
@@ -661,8 +653,20 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             }
         }
         settle_labels();
+    };
+    if constexpr (NS > 1) {
         if (part)
-            stage(echo);
+            stage_echo_sets8(lds8, 0, echoes, ES, part, part_stride, set_stride);
+        for (uint32_t echo = 1; echo <= echoes; ++echo) {
+            walk_echo();
+            if (part)
+                stage_echo_sets8(lds8, echo, echoes, ES, part, part_stride, set_stride);
+        }
+    } else {
+        // one walker per lane: echoes reduced in groups of 4; several walkers
+        // per lane stage each echo (the group buffer would push those
+        // register-heavy forms into scratch)
+        run_echoes<K == 1>(echoes, lds, part, part_stride, walk_echo, echo_value);
     }
     flush();
     if (NS > 1)
