@@ -413,12 +413,13 @@ __device__ __forceinline__ void run_echoes(uint32_t echoes, double (*lds)[kEchoC
 
 // Echo sums of 8 energy sets per walker: one reduce-scatter gives lanes 8q
 // .. 8q + 7 the wave total of set q, lanes 0, 8, ..., 56 stage their set's
-// wave total in LDS for kEchoChunk echoes, and the block then writes one
-// partial per (set, echo): out[set * set_stride + row * row_stride].  About
-// 4.3 VALU per set and echo instead of 15 for separate half-wave sums.
-// Fixed order, so deterministic.  All threads of the block call it, with
-// the whole wave active.
-__device__ __forceinline__ void stage_echo_sets8(double (*lds8)[8][kMaxWaves], uint32_t echo, uint32_t last_echo,
+// wave total in LDS for kEchoChunk echoes (lds8: kEchoChunk x 8 x waves of
+// the block, [slot][set][wave]), and the block then writes one partial per
+// (set, echo): out[set * set_stride + row * row_stride].  About 4.3 VALU per
+// set and echo instead of 15 for separate half-wave sums.  Fixed order, so
+// deterministic.  All threads of the block call it, with the whole wave
+// active.
+__device__ __forceinline__ void stage_echo_sets8(double *lds8, uint32_t echo, uint32_t last_echo,
                                                  const double (&v)[8], double *out, uint64_t row_stride,
                                                  uint64_t set_stride)
 {
@@ -432,7 +433,7 @@ __device__ __forceinline__ void stage_echo_sets8(double (*lds8)[8][kMaxWaves], u
     const double r = wave_reduce_scatter<8>(t);
     const int slot = echo % kEchoChunk;
     if ((lane & 7) == 0)
-        lds8[slot][lane >> 3][wave] = r;
+        lds8[(slot * 8 + (lane >> 3)) * nw + wave] = r;
     if (slot == kEchoChunk - 1 || echo == last_echo) {
         __syncthreads();
         const int cnt = slot + 1;
@@ -440,7 +441,7 @@ __device__ __forceinline__ void stage_echo_sets8(double (*lds8)[8][kMaxWaves], u
             const int row = q >> 3, set = q & 7;
             double s = 0.0;
             for (int w = 0; w < nw; ++w)
-                s += lds8[row][set][w];
+                s += lds8[q * nw + w];
             out[(uint64_t)set * set_stride + ((uint64_t)(echo - slot) + (uint64_t)row) * row_stride] = s;
         }
         __syncthreads();

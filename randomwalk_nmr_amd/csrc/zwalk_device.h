@@ -390,7 +390,7 @@ template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = 
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
                                            const double *ldelta = nullptr, const uint8_t *keys = nullptr,
-                                           uint32_t *lh = nullptr, double (*lds8)[8][kMaxWaves] = nullptr)
+                                           uint32_t *lh = nullptr, double *lds8 = nullptr)
 {
     constexpr bool WIDE = TAB == 1;
     static_assert(!NOFLUX || CACHE >= 64, "noflux: pipelined chunk-cache walk only");

@@ -50,8 +50,7 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
 template <bool WRAPFIX, bool NOFLUX, bool MFLAGS>
 __global__ __launch_bounds__(1024) void zwalk_sets_kernel(const WalkArgs a)
 {
-    __shared__ double lds[kEchoChunk][kEchoCols];
-    __shared__ double lds8[kEchoChunk][8][kMaxWaves];
+    extern __shared__ double lds8[];   // kEchoChunk x 8 sets x waves of the block
     __shared__ uint4 tab[MFLAGS ? 8 * kZFlagStride / 16 : (NOFLUX ? 12 : 6)];
     __shared__ uint2 fix[MFLAGS ? 8 * kZFlagStride / 16 : 6];
     __shared__ uint8_t keys[kZKeyEntries];
@@ -72,7 +71,8 @@ __global__ __launch_bounds__(1024) void zwalk_sets_kernel(const WalkArgs a)
     }
     __syncthreads();
     zwalk_tile<2, 1, WRAPFIX, 128, 2, NOFLUX, MFLAGS, 8>(a, tile, a.partials ? a.partials + tile : nullptr,
-                                                         a.partial_stride, lds, tab, fix, ldelta, keys, nullptr, lds8);
+                                                         a.partial_stride, nullptr, tab, fix, ldelta, keys, nullptr,
+                                                         lds8);
 }
 
 // PFGSE on the unfolded image: the walk between diffusion times counts hits
@@ -391,17 +391,20 @@ hipError_t launch_zwalk_sets(const WalkArgs &a, uint32_t threads, hipStream_t st
     if (tiles > 0x7FFFFFF0ull || (a.tile_map && a.n_tiles != tiles))
         return hipErrorInvalidValue;
     const uint32_t blocks = a.tile_map ? (uint32_t)(((tiles + kXcds - 1) / kXcds) * kXcds) : (uint32_t)tiles;
+    // the per-set echo staging, sized for this block's waves
+    const size_t lds8_bytes = (size_t)kEchoChunk * 8 * (threads / kWave) * sizeof(double);
     if (a.znoflux) {
-        hipLaunchKernelGGL((zwalk_sets_kernel<false, true, false>), dim3(blocks), dim3(threads), 0, st, a);
+        hipLaunchKernelGGL((zwalk_sets_kernel<false, true, false>), dim3(blocks), dim3(threads), lds8_bytes, st, a);
     } else if (a.zmflags) {
         if (a.zl.wrapfix)
-            hipLaunchKernelGGL((zwalk_sets_kernel<true, false, true>), dim3(blocks), dim3(threads), 0, st, a);
+            hipLaunchKernelGGL((zwalk_sets_kernel<true, false, true>), dim3(blocks), dim3(threads), lds8_bytes, st, a);
         else
-            hipLaunchKernelGGL((zwalk_sets_kernel<false, false, true>), dim3(blocks), dim3(threads), 0, st, a);
+            hipLaunchKernelGGL((zwalk_sets_kernel<false, false, true>), dim3(blocks), dim3(threads), lds8_bytes, st,
+                               a);
     } else if (a.zl.wrapfix) {
-        hipLaunchKernelGGL((zwalk_sets_kernel<true, false, false>), dim3(blocks), dim3(threads), 0, st, a);
+        hipLaunchKernelGGL((zwalk_sets_kernel<true, false, false>), dim3(blocks), dim3(threads), lds8_bytes, st, a);
     } else {
-        hipLaunchKernelGGL((zwalk_sets_kernel<false, false, false>), dim3(blocks), dim3(threads), 0, st, a);
+        hipLaunchKernelGGL((zwalk_sets_kernel<false, false, false>), dim3(blocks), dim3(threads), lds8_bytes, st, a);
     }
     return hipGetLastError();
 }
