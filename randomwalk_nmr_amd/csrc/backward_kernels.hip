@@ -114,7 +114,9 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     __shared__ uint2 fix[6];
     __shared__ uint8_t keys[kZKeyEntries];
     __shared__ double ldelta[kMaxLabels];
-    __shared__ double racc[kLhStride * kZBackThreads];   // R_wl, one row of kLhStride per walker
+    // R_wl, label-major: the block size is fixed, so a label's offset is one
+    // shift-and-add, and 32 KiB keeps 4 blocks (8 waves per SIMD) per CU
+    __shared__ double racc[kMaxLabels * kZBackThreads];
     uint32_t seg;
     uint64_t tile;
     if (!backward_block(b, seg, tile))
@@ -123,9 +125,9 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     zbuild_key_table(keys, 16u);
     if (threadIdx.x < kMaxLabels)
         ldelta[threadIdx.x] = (int)threadIdx.x < b.n_labels ? a.label_delta[threadIdx.x] : 1.0;
-    double *my_r = racc + threadIdx.x * (uint32_t)kLhStride;
+    double *my_r = racc + threadIdx.x;
     for (int l = 0; l < kMaxLabels; ++l)
-        my_r[l] = 0.0;
+        my_r[l * kZBackThreads] = 0.0;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
     auto settle = [&]() {
         if (pend) {
             e *= ldelta[lab];
-            __hip_atomic_fetch_add(my_r + lab, Q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(my_r + lab * kZBackThreads, Q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             pend = false;
         }
     };
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
         constexpr int l = decltype(lc)::value;
         double G = 0.0;
         if (active && l < n_labels)
-            G = (double)h_end[(uint64_t)l * n + j] * Q - my_r[l];
+            G = (double)h_end[(uint64_t)l * n + j] * Q - my_r[l * kZBackThreads];
         const double v = wave_sum(G);
         if (lane == 0)
             lds[l][wave] = v;
