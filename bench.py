@@ -289,6 +289,12 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
         peak = VALU_WAVE_INSTR_PER_S * 64.0 / vps
         out.update(peak=peak, frac=steps_per_s / peak, valu_per_wave_step=vps,
                    valu_busy_measured=prof.get("valu_busy"), counters=os.path.relpath(COUNTERS_FILE, ROOT))
+        if prof.get("sclk_GHz_measured"):
+            # the same ceiling at the shader clock the counters saw during the
+            # walk (sustained load runs below the 2.4 GHz peak)
+            f = float(prof["sclk_GHz_measured"]) * 1e9
+            sec["valu_at_measured_clock"] = {"clock_GHz": f / 1e9, "peak": peak * f / CLOCK_HZ,
+                                             "frac": steps_per_s / (peak * f / CLOCK_HZ)}
         # HBM bytes per launch scale with the launch's walker-steps
         scale = steps_per_launch / float(prof["walker_steps_per_launch"])
         hbm = float(prof["hbm_bytes_per_launch"]) * scale
