@@ -313,8 +313,10 @@ __device__ __forceinline__ void zchunk_fill_buf(const zrsrc_t &rsrc, uint32_t c,
     if (cid != QB) {
         const zv4u v = zchunk_load(rsrc, cid);
         QV = make_uint4(v.x, v.y, v.z, v.w);
-        QB = cid;
     }
+    // unconditional (equal when no load was needed): the chunk id then lives
+    // in the register it was computed in instead of being copied on reloads
+    QB = cid;
 }
 
 template <int CW>
