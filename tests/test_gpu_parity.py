@@ -209,6 +209,39 @@ def test_cpmg_labels_unfolded_and_folded_match_oracle(ctx, bc, dims, unfolded):
     assert np.array_equal(g["label_hits"], o["label_hits"].T)
 
 
+@pytest.mark.parametrize("side", [16, 64, 512])
+def test_cpmg_labels_octant_fold_match_oracle(ctx, side):
+    """Mirror images of equal power-of-two sides keep one byte per voxel of
+    the first octant only and read a hit's label at the folded address (one
+    extract, a 24-bit multiply-add and a masked xor; at 512^3 the multiplier's
+    top term is added as a shift).  Walkers start next to the faces so that
+    their candidate voxels cross into the other octants of the mirror image;
+    walkers, energies and per-label hit counts bit-exact."""
+    print(f"[test] labels octant fold side={side}", flush=True)
+    solid, _ = R.sphere_pack(side, side, side, 3.0 if side < 64 else 6.0, 3.0 if side < 64 else 12.0, 0.3, 31)
+    labels = R.voronoi_labels(solid, 16, 8, 37)
+    seeds, x, y, z = R.seed_and_place(solid, 41, 20000 if side < 512 else 200000)
+    if side == 512:
+        near = np.minimum.reduce([x, y, z, side - 1 - x, side - 1 - y, side - 1 - z]) < 3
+        keep = np.flatnonzero(near)[:3000]
+        seeds, x, y, z = seeds[keep], x[keep], y[keep], z[keep]
+    else:
+        seeds, x, y, z = seeds[:3000], x[:3000], y[:3000], z[:3000]
+    ld = np.linspace(0.95, 0.999, 8)
+    img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
+    g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=30, steps_per_echo=4, bc="mirror", label_delta=ld,
+                   label_hits=True)
+    img.close()
+    o = O.walk(O.bitblock_pack(solid), (side, side, side), x, y, z, seeds, echoes=30, steps_per_echo=4,
+               bc="mirror", labels=labels, label_delta=ld, label_hits=True)
+    # the walks left the first octant (reflected copies) and hit walls there
+    assert np.any((np.asarray(o["x"]) < 0) | (np.asarray(o["x"]) >= side) | (np.asarray(o["y"]) < 0) |
+                  (np.asarray(o["z"]) >= side))
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+    assert np.array_equal(g["label_hits"], o["label_hits"].T)
+
+
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("shape", [(256, 1), (1024, 1), (512, 2), (256, 4)])
 def test_cpmg_labels_launch_shapes_match_oracle(ctx, bc, shape):
@@ -433,15 +466,17 @@ def test_pfgse_phase_paths_and_kernels_match_oracle(ctx, bc, kind, unfolded):
     assert np.array_equal(g["msd"], o["msd"])
 
 
-def test_pfgse_energy_sets(ctx):
-    """Several uniform deltas on one trajectory equal separate runs."""
-    print("[test] pfgse sets", flush=True)
+@pytest.mark.parametrize("n_sets", [3, 8])
+def test_pfgse_energy_sets(ctx, n_sets):
+    """Several uniform deltas on one trajectory equal separate runs (sets
+    summed over the wave in pairs, a lone last set on its own)."""
+    print(f"[test] pfgse sets {n_sets}", flush=True)
     solid, seeds, x, y, z, _ = _case(24, 24, 24, 4, 0.3, 800, 4)
     img = R.Image.from_solid(ctx, solid)
     si = np.array([4, 8, 16], np.int64)
-    kx = np.linspace(0, 1.0, 8)
-    zeros = np.zeros(8)
-    sets = np.array([1.0, 0.99, 0.97])
+    kx = np.linspace(0, 1.0, 12)
+    zeros = np.zeros(12)
+    sets = np.linspace(1.0, 0.95, n_sets)
     g = R.pfgse_run(ctx, img, x, y, z, seeds, steps_inc=si, kx=kx, ky=zeros, kz=zeros, voxel=1.0,
                     set_delta=sets)
     for q, d in enumerate(sets):
