@@ -1784,14 +1784,23 @@ rwnmr_status pfgse_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w
     if (w->n == 0)
         return fail(c, RWNMR_ERR_INVALID, "pfgse_run: no walkers");
     const uint32_t threads = rwnmr::kPfgseThreads;
+    if (n_samples && w->n / n_samples == 0)
+        return fail(c, RWNMR_ERR_INVALID, "pfgse sampling: fewer walkers than samples");
+    WalkerView v;
+    if ((s = clear_flag(c)) != RWNMR_OK)
+        return s;
+    if ((s = stage_in(c, w, v)) != RWNMR_OK)
+        return s;
+    WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
+    attach_zimage(c, img, bc, 1, a, true);
     PfgseArgs p;
     memset(&p, 0, sizeof(p));
+    // the unfolded-image kernel walks kPfgseK walkers per lane
+    p.walkers_per_block = a.zwords ? threads * (uint32_t)rwnmr::kPfgseK : threads;
     if (n_samples) {
         // NMR_PFGSE::recoverDsatWithSampling: walkersPerSample = N / samples
         p.group_walkers = w->n / n_samples;
-        if (p.group_walkers == 0)
-            return fail(c, RWNMR_ERR_INVALID, "pfgse sampling: fewer walkers than samples");
-        p.blocks_per_group = (uint32_t)((p.group_walkers + threads - 1) / threads);
+        p.blocks_per_group = (uint32_t)((p.group_walkers + p.walkers_per_block - 1) / p.walkers_per_block);
         p.full_groups = n_samples;
     }
     const uint32_t blocks = rwnmr::pfgse_grid_blocks(p, w->n);
@@ -1799,11 +1808,6 @@ rwnmr_status pfgse_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w
     const uint64_t rows = mrows + (msd ? 3ull * n_delta : 0ull);
     if (rows * blocks * 8ull > (8ull << 30))
         return fail(c, RWNMR_ERR_INVALID, "pfgse_run: partial buffer above 8 GiB; split the k or delta set");
-    WalkerView v;
-    if ((s = clear_flag(c)) != RWNMR_OK)
-        return s;
-    if ((s = stage_in(c, w, v)) != RWNMR_OK)
-        return s;
     p.n_delta = n_delta;
     p.n_k = n_k;
     p.k_delta_stride = k_per_delta ? n_k : 0u;
@@ -1863,8 +1867,6 @@ rwnmr_status pfgse_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w
         RW_HIP(c, hipStreamSynchronize(c->stream));
         p.k_rot = k_rows_arithmetic(hk.data(), nkv, n_k, k_per_delta ? n_delta : 1u) ? 1u : 0u;
     }
-    WalkArgs a = base_args(img, v, w, shift, c->flag.as<uint32_t>());
-    attach_zimage(c, img, bc, 1, a, true);
     double *Mdev = M, *msd_dev = msd;
     if (!w->on_device) {
         RW_HIP(c, c->sums.reserve(8ull * rows + 8));

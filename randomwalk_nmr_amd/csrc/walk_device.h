@@ -448,60 +448,95 @@ __device__ __forceinline__ void stage_echo_sets8(double *lds8, uint32_t echo, ui
     }
 }
 
-// PFGSE measurement at diffusion time i for one walker with displacement
-// (dX, dY, dZ) voxels and energies e[0..nsets): block sums of
-// e[q] cos((k . dr) voxel) for every k of the time's k row and every set q,
-// staged in LDS (lds rows q * kPfgseKChunk + k) and written as partials
+// PFGSE measurement at diffusion time i for the K walkers of each lane, with
+// displacements (dX, dY, dZ)[k] voxels and energies e[k][0..nsets): block
+// sums of e cos((k . dr) voxel) for every k of the time's k row and every set
+// q, staged in LDS (lds rows q * kPfgseKChunk + k) and written as partials
 // [(q * n_delta + i) * n_k + k][blockIdx.x].  cos is evaluated once per k and
-// shared by the sets; with ROT (k row k_0 + j d) the cosines follow the
-// three-term recurrence cos(ph0 + (j+1) phd) = 2 cos(phd) cos(ph0 + j phd) -
-// cos(ph0 + (j-1) phd), one FMA per k instead of a cos (over 64 k the error
-// stays below 5e-13 of a term, far inside the 1e-11 M(0) tolerance); the
-// recurrence runs through whole groups of 8 k, the values past the last k
-// unused.  Idle lanes carry zero energies, so their terms are zero without
-// a select.  All threads of the block call it.
-template <bool ROT, int MAXSETS = kMaxSets>
-__device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, double dX, double dY, double dZ,
-                                              const double *e, int nsets, bool active,
-                                              double (*lds)[kPfgseWaves])
+// walker and shared by the sets; with ROT (k row k_0 + j d) the cosines
+// follow the three-term recurrence cos(ph0 + (j+1) phd) = 2 cos(phd)
+// cos(ph0 + j phd) - cos(ph0 + (j-1) phd), one FMA per k instead of a cos
+// (over 64 k the error stays below 5e-13 of a term, far inside the 1e-11
+// M(0) tolerance); the recurrence runs through whole groups of 8 k, the
+// values past the last k unused.  A lane adds its K walkers' terms before
+// the cross-lane reduce-scatter, so that cost is paid once per K walkers.
+// Idle walkers carry zero energies, so their terms are zero without a
+// select.  All threads of the block call it.
+template <bool ROT, int MAXSETS, int K>
+__device__ __forceinline__ void pfgse_measure_k(const PfgseArgs &p, uint32_t i, const double (&dX)[K],
+                                                const double (&dY)[K], const double (&dZ)[K],
+                                                const double (&e)[K][MAXSETS], int nsets,
+                                                double (*lds)[kPfgseWaves])
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const uint64_t krow = (uint64_t)i * p.k_delta_stride;
-    double c = 0.0, cprev = 0.0, twoc1 = 2.0;
-    if (ROT) {
-        const double ph0 = (p.kx[krow] * dX + p.ky[krow] * dY + p.kz[krow] * dZ) * p.voxel;
-        const double phd = ((p.kx[krow + 1] - p.kx[krow]) * dX + (p.ky[krow + 1] - p.ky[krow]) * dY +
-                            (p.kz[krow + 1] - p.kz[krow]) * dZ) * p.voxel;
-        double s = 0.0, s1 = 0.0, c1 = 1.0;
-        sincos(ph0, &s, &c);
-        sincos(phd, &s1, &c1);
-        cprev = c * c1 + s * s1;   // cos(ph0 - phd)
-        twoc1 = 2.0 * c1;
-    }
-    (void)active;
+    double c[K], cprev[K], twoc1[K];
+    static_for<K>([&](auto kc_) {
+        constexpr int w = decltype(kc_)::value;
+        c[w] = 0.0;
+        cprev[w] = 0.0;
+        twoc1[w] = 2.0;
+        if (ROT) {
+            const double ph0 = (p.kx[krow] * dX[w] + p.ky[krow] * dY[w] + p.kz[krow] * dZ[w]) * p.voxel;
+            const double phd = ((p.kx[krow + 1] - p.kx[krow]) * dX[w] + (p.ky[krow + 1] - p.ky[krow]) * dY[w] +
+                                (p.kz[krow + 1] - p.kz[krow]) * dZ[w]) * p.voxel;
+            double sn = 0.0, s1 = 0.0, c1 = 1.0;
+            sincos(ph0, &sn, &c[w]);
+            sincos(phd, &s1, &c1);
+            cprev[w] = c[w] * c1 + sn * s1;   // cos(ph0 - phd)
+            twoc1[w] = 2.0 * c1;
+        }
+    });
     for (uint32_t k0 = 0; k0 < p.n_k; k0 += kPfgseKChunk) {
         const uint32_t kc = p.n_k - k0 < (uint32_t)kPfgseKChunk ? p.n_k - k0 : (uint32_t)kPfgseKChunk;
         // groups of 8 k: the cosines of the group, then the reduce-scatters
         // of their terms
         for (uint32_t g0 = 0; g0 < kc; g0 += 8) {
-            double cs[8];
+            // the 8 cosines of walker w (ROT: the recurrence advanced 8 k)
+            // handed to use(j, cos) in k order
+            double cs[ROT ? 1 : K][8];
+            if (!ROT)
+                static_for<K>([&](auto kc_) {
+                    constexpr int w = decltype(kc_)::value;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                if (ROT) {
-                    cs[j] = c;
-                    const double cn = fma(twoc1, c, -cprev);
-                    cprev = c;
-                    c = cn;
-                } else {
-                    const uint32_t kk = g0 + (uint32_t)j;
-                    cs[j] = 0.0;
-                    if (kk < kc) {   // uniform
-                        const uint64_t kk_i = krow + k0 + kk;
-                        cs[j] = cos((p.kx[kk_i] * dX + p.ky[kk_i] * dY + p.kz[kk_i] * dZ) * p.voxel);
+                    for (int j = 0; j < 8; ++j) {
+                        const uint32_t kk = g0 + (uint32_t)j;
+                        cs[w][j] = 0.0;
+                        if (kk < kc) {   // uniform
+                            const uint64_t kk_i = krow + k0 + kk;
+                            cs[w][j] = cos((p.kx[kk_i] * dX[w] + p.ky[kk_i] * dY[w] + p.kz[kk_i] * dZ[w]) * p.voxel);
+                        }
                     }
+                });
+            double c0[K], cp0[K];   // ROT: the recurrence state at the group's first k
+            static_for<K>([&](auto kc_) {
+                constexpr int w = decltype(kc_)::value;
+                c0[w] = c[w];
+                cp0[w] = cprev[w];
+            });
+            auto cosines = [&](auto wc, auto &&use) {
+                constexpr int w = decltype(wc)::value;
+                if (ROT) {
+                    // every set pair runs the 8 recurrence steps from the
+                    // group's start (the cosines are not kept across pairs)
+                    double cc = c0[w], cp = cp0[w];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        use(j, cc);
+                        const double cn = fma(twoc1[w], cc, -cp);
+                        cp = cc;
+                        cc = cn;
+                    }
+                    c[w] = cc;
+                    cprev[w] = cp;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        use(j, cs[w][j]);
                 }
-            }
+            };
+            // the lane's terms: its walkers' products, added in walker order
             // sets in pairs: one 16-value reduce-scatter (lanes 4(8p + j) ..
             // + 3 hold k j of set q0 + p), a lone last set by an 8-value one
             // (lanes 8j .. 8j + 7)
@@ -509,11 +544,13 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
                 constexpr int q0 = 2 * decltype(pc)::value;
                 if (q0 + 1 < nsets) {
                     double t[16];
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        t[j] = cs[j] * e[q0];
-                        t[8 + j] = cs[j] * e[q0 + 1];
-                    }
+                    static_for<K>([&](auto wc) {
+                        constexpr int w = decltype(wc)::value;
+                        cosines(wc, [&](int j, double cv) {
+                            t[j] = w == 0 ? cv * e[w][q0] : fma(cv, e[w][q0], t[j]);
+                            t[8 + j] = w == 0 ? cv * e[w][q0 + 1] : fma(cv, e[w][q0 + 1], t[8 + j]);
+                        });
+                    });
                     const double v = wave_reduce_scatter<16>(t);
                     const uint32_t q = (uint32_t)q0 + ((uint32_t)lane >> 5);
                     const uint32_t kk = g0 + (((uint32_t)lane >> 2) & 7u);
@@ -521,9 +558,12 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
                         lds[q * kPfgseKChunk + kk][wave] = v;
                 } else if (q0 < nsets) {
                     double t[8];
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        t[j] = cs[j] * e[q0];
+                    static_for<K>([&](auto wc) {
+                        constexpr int w = decltype(wc)::value;
+                        cosines(wc, [&](int j, double cv) {
+                            t[j] = w == 0 ? cv * e[w][q0] : fma(cv, e[w][q0], t[j]);
+                        });
+                    });
                     const double v = wave_reduce_scatter<8>(t);
                     const uint32_t kk = g0 + ((uint32_t)lane >> 3);
                     if ((lane & 7) == 0 && kk < kc)
@@ -544,16 +584,40 @@ __device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, do
     }
 }
 
-// Sums of squared displacements per axis at diffusion time i (voxel^2;
-// integer-valued, so the FP64 sums are exact in any order).
-__device__ __forceinline__ void pfgse_msd(const PfgseArgs &p, uint32_t i, double dX, double dY, double dZ,
-                                          bool active, double (*lds)[kPfgseWaves])
+// One walker per lane (the folded PFGSE kernel).
+template <bool ROT, int MAXSETS = kMaxSets>
+__device__ __forceinline__ void pfgse_measure(const PfgseArgs &p, uint32_t i, double dX, double dY, double dZ,
+                                              const double *e, int nsets, bool active,
+                                              double (*lds)[kPfgseWaves])
+{
+    (void)active;
+    const double x[1] = {dX}, y[1] = {dY}, z[1] = {dZ};
+    double ee[1][MAXSETS];
+    for (int q = 0; q < MAXSETS; ++q)
+        ee[0][q] = e[q];
+    pfgse_measure_k<ROT, MAXSETS, 1>(p, i, x, y, z, ee, nsets, lds);
+}
+
+// Sums of squared displacements per axis at diffusion time i over the K
+// walkers of each lane (voxel^2; integer-valued, so the FP64 sums are exact
+// in any order).
+template <int K>
+__device__ __forceinline__ void pfgse_msd_k(const PfgseArgs &p, uint32_t i, const double (&dX)[K],
+                                            const double (&dY)[K], const double (&dZ)[K], const bool (&active)[K],
+                                            double (*lds)[kPfgseWaves])
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const double sx = wave_sum(active ? dX * dX : 0.0);
-    const double sy = wave_sum(active ? dY * dY : 0.0);
-    const double sz = wave_sum(active ? dZ * dZ : 0.0);
+    double xx = 0.0, yy = 0.0, zz = 0.0;
+    static_for<K>([&](auto kc_) {
+        constexpr int w = decltype(kc_)::value;
+        xx += active[w] ? dX[w] * dX[w] : 0.0;
+        yy += active[w] ? dY[w] * dY[w] : 0.0;
+        zz += active[w] ? dZ[w] * dZ[w] : 0.0;
+    });
+    const double sx = wave_sum(xx);
+    const double sy = wave_sum(yy);
+    const double sz = wave_sum(zz);
     if (lane == 0) {
         lds[0][wave] = sx;
         lds[1][wave] = sy;
@@ -568,6 +632,14 @@ __device__ __forceinline__ void pfgse_msd(const PfgseArgs &p, uint32_t i, double
         p.partials[row * p.partial_stride + blockIdx.x] = sum;
     }
     __syncthreads();
+}
+
+__device__ __forceinline__ void pfgse_msd(const PfgseArgs &p, uint32_t i, double dX, double dY, double dZ,
+                                          bool active, double (*lds)[kPfgseWaves])
+{
+    const double x[1] = {dX}, y[1] = {dY}, z[1] = {dZ};
+    const bool act[1] = {active};
+    pfgse_msd_k<1>(p, i, x, y, z, act, lds);
 }
 
 // One tile of walkers (blockDim.x * K consecutive walkers starting at

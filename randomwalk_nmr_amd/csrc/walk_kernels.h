@@ -70,6 +70,7 @@ __host__ __device__ constexpr uint32_t max_threads_for_k(int k) { return k >= 4 
 
 static constexpr uint32_t kPfgseThreads = 256;
 static constexpr int kPfgseWaves = (int)kPfgseThreads / 64;
+static constexpr int kPfgseK = 2;   // walkers per lane of the unfolded-image PFGSE kernel
 static constexpr int kMaxSets = 8;       // energy sets evaluated on one PFGSE trajectory
 static constexpr int kPfgseKChunk = 64;  // k values staged in LDS per block sync
 
@@ -97,6 +98,7 @@ struct PfgseArgs {
     // blocks.  group_walkers == 0: plain blocks for every walker.
     uint64_t group_walkers;
     uint32_t blocks_per_group, full_groups;
+    uint32_t walkers_per_block;      // kPfgseThreads x walkers per lane of the kernel that runs
 };
 
 // Walker index of thread threadIdx.x of block b under the sample grouping of
@@ -123,13 +125,15 @@ __host__ __device__ inline uint64_t pfgse_walker(const PfgseArgs &p, uint64_t b,
 
 uint32_t walk_grid_blocks(uint64_t n, const LaunchShape &ls, int shift);
 
-// Blocks of a PFGSE launch of n walkers under the grouping of p.
+// Blocks of a PFGSE launch of n walkers under the grouping of p
+// (p.walkers_per_block walkers per block).
 inline uint32_t pfgse_grid_blocks(const PfgseArgs &p, uint64_t n)
 {
+    const uint64_t per = p.walkers_per_block;
     if (p.group_walkers == 0)
-        return (uint32_t)((n + kPfgseThreads - 1) / kPfgseThreads);
+        return (uint32_t)((n + per - 1) / per);
     const uint64_t rest = n - p.group_walkers * p.full_groups;
-    return (uint32_t)((uint64_t)p.blocks_per_group * p.full_groups + (rest + kPfgseThreads - 1) / kPfgseThreads);
+    return (uint32_t)((uint64_t)p.blocks_per_group * p.full_groups + (rest + per - 1) / per);
 }
 hipError_t launch_pfgse(const WalkArgs &a, const PfgseArgs &p, int bc, uint32_t threads, hipStream_t st);
 hipError_t launch_walk(const WalkArgs &a, int bc, int emode, const LaunchShape &ls, hipStream_t st);

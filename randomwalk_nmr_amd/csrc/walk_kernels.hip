@@ -209,6 +209,9 @@ hipError_t launch_pfgse(const WalkArgs &a, const PfgseArgs &p, int bc, uint32_t 
         return hipSuccess;
     if (threads != kPfgseThreads)
         return hipErrorInvalidValue;
+    // the grid and the sample grouping were sized for the kernel that runs
+    if (p.walkers_per_block != (a.zwords ? kPfgseThreads * (uint32_t)kPfgseK : kPfgseThreads))
+        return hipErrorInvalidValue;
     if (a.zwords)
         return launch_zpfgse(a, p, st);
     const uint32_t blocks = pfgse_grid_blocks(p, a.n);

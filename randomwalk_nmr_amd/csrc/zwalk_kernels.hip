@@ -78,12 +78,19 @@ __global__ __launch_bounds__(1024) void zwalk_sets_kernel(const WalkArgs a)
 // PFGSE on the unfolded image: the walk between diffusion times counts hits
 // (register chunk cache, packed displacement flushed every kZFlush steps and
 // at every diffusion time), then pow(delta, hits) per energy set and the
-// shared measurement (pfgse_measure).  One walker per thread.
+// shared measurement (pfgse_measure_k).  K walkers per lane (walkers
+// threadIdx.x + k blockDim.x of the block's blockDim.x K): their steps
+// interleave, so each wave keeps K chunk reloads in flight, and their
+// measurement terms are added in the lane before the cross-lane sums.
 // NOFLUX: noflux walk on the periodic unfolded image (face test and
 // turn-around, zmove_noflux); walkers must start inside the image.  MAXSETS:
 // the most energy sets the instance handles (1, 2 or kMaxSets; registers).
-template <bool WRAPFIX, bool ROT, bool NOFLUX, int MAXSETS>
-__global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a, const PfgseArgs p)
+// Two walkers per lane fit 168 VGPRs (3 waves per SIMD) for rotation k rows,
+// up to two sets and the periodic / mirror walk; the other instances take
+// more registers and 2 waves.
+template <bool WRAPFIX, bool ROT, bool NOFLUX, int MAXSETS, int K>
+__global__ __launch_bounds__(kPfgseThreads, (K > 1 && ROT && MAXSETS <= 2 && !NOFLUX) ? 3 : 1) void zpfgse_kernel(
+    const WalkArgs a, const PfgseArgs p)
 {
     __shared__ double lds[kMaxSets * kPfgseKChunk][kPfgseWaves];
     __shared__ uint4 tab[NOFLUX ? 12 : 6];
@@ -95,69 +102,95 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
         zbuild_table(a.zl, tab, fix);
     zbuild_key_table(keys, NOFLUX ? 32u : 16u);
     __syncthreads();
-    bool inside;
-    const uint64_t id = pfgse_walker(p, blockIdx.x, threadIdx.x, blockDim.x, a.n, inside);
-    bool active = inside;
-    uint64_t j = active ? id : 0;
-    int32_t X = active ? a.x[j] : 0, Y = active ? a.y[j] : 0, Z = active ? a.z[j] : 0;
-    if (NOFLUX && active && !((unsigned)X < (unsigned)a.W && (unsigned)Y < (unsigned)a.H && (unsigned)Z < (unsigned)a.D)) {
-        active = false;   // a noflux walker must start inside the image
-        atomicOr(a.error_flag, 1u);
-        j = 0;
-        X = Y = Z = 0;
-    }
-    uint64_t s = active ? a.seed[j] : 1ull;
-    uint32_t hits = (active && a.hits) ? a.hits[j] : 0u;
-    const int32_t x0 = active ? p.x0[j] : 0, y0 = active ? p.y0[j] : 0, z0 = active ? p.z0[j] : 0;
-    const double own_delta = (active && a.penalty) ? a.penalty[j] : a.delta_uniform;
     const int nsets = p.n_sets > 0 ? (int)p.n_sets : 1;
-    double e[MAXSETS];
-    static_for<MAXSETS>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        e[q] = !active ? 0.0 : ((q == 0 && a.energy) ? a.energy[j] : 1.0);
-    });
-    uint32_t A = zaddress(X, Y, Z, a.zl);
-    uint32_t QB = 0xFFFFFFFFu;
-    uint4 QV = make_uint4(0u, 0u, 0u, 0u);
-    int32_t P = 0;
-    uint32_t since = 0;
+    bool active[K];
+    uint64_t id[K];
+    int32_t X[K], Y[K], Z[K], x0[K], y0[K], z0[K], P[K];
+    uint64_t S[K];
+    uint32_t hits[K], A[K], QB[K], H[K];
+    uint4 QV[K];
+    double own_delta[K], e[K][MAXSETS];
+    ZEntry T[K];
     const zrsrc_t zrsrc = zchunk_rsrc(a.zwords, a.zl);
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
     const char *fix_bytes = reinterpret_cast<const char *>(fix);
-    // the next direction is drawn a step ahead, during the chunk load (the
-    // state is rewound by one step at the end)
-    ZEntry T;
-    zdirection<WRAPFIX, 2, NOFLUX>(s, tab_bytes, fix_bytes, T, keys);
+    static_for<K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        bool inside;
+        id[k] = pfgse_walker(p, blockIdx.x, threadIdx.x + (uint32_t)k * blockDim.x, blockDim.x * K, a.n, inside);
+        active[k] = inside;
+        uint64_t j = active[k] ? id[k] : 0;
+        X[k] = active[k] ? a.x[j] : 0;
+        Y[k] = active[k] ? a.y[j] : 0;
+        Z[k] = active[k] ? a.z[j] : 0;
+        if (NOFLUX && active[k] &&
+            !((unsigned)X[k] < (unsigned)a.W && (unsigned)Y[k] < (unsigned)a.H && (unsigned)Z[k] < (unsigned)a.D)) {
+            active[k] = false;   // a noflux walker must start inside the image
+            atomicOr(a.error_flag, 1u);
+            j = 0;
+            X[k] = Y[k] = Z[k] = 0;
+        }
+        S[k] = active[k] ? a.seed[j] : 1ull;
+        hits[k] = (active[k] && a.hits) ? a.hits[j] : 0u;
+        x0[k] = active[k] ? p.x0[j] : 0;
+        y0[k] = active[k] ? p.y0[j] : 0;
+        z0[k] = active[k] ? p.z0[j] : 0;
+        own_delta[k] = (active[k] && a.penalty) ? a.penalty[j] : a.delta_uniform;
+        static_for<MAXSETS>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            e[k][q] = !active[k] ? 0.0 : ((q == 0 && a.energy) ? a.energy[j] : 1.0);
+        });
+        A[k] = zaddress(X[k], Y[k], Z[k], a.zl);
+        QB[k] = 0xFFFFFFFFu;
+        QV[k] = make_uint4(0u, 0u, 0u, 0u);
+        P[k] = 0;
+        // the next direction is drawn a step ahead, during the chunk load
+        // (the state is rewound by one step at the end)
+        zdirection<WRAPFIX, 2, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys);
+    });
+    uint32_t since = 0;
     auto flush = [&]() {
-        int32_t dx, dy, dz;
-        zunpack(P, dx, dy, dz);
-        X += dx;
-        Y += dy;
-        Z += dz;
-        P = 0;
+        static_for<K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            int32_t dx, dy, dz;
+            zunpack(P[k], dx, dy, dz);
+            X[k] += dx;
+            Y[k] += dy;
+            Z[k] += dz;
+            P[k] = 0;
+        });
         since = 0;
     };
     for (uint32_t i = 0; i < p.n_delta; ++i) {
-        uint32_t h = 0;
+        static_for<K>([&](auto kc) { H[decltype(kc)::value] = 0u; });
         uint32_t left = p.steps_inc[i];
         auto step = [&]() {
-            int32_t dp;
-            uint32_t c;
-            if (NOFLUX) {
-                c = zmove_noflux(A, T, dp);
-            } else {
-                c = zmove_entry<WRAPFIX>(A, T);
-                dp = (int32_t)T.dp;
-            }
-            zchunk_fill_buf(zrsrc, c, QB, QV);
-            zdirection<WRAPFIX, 2, NOFLUX>(s, tab_bytes, fix_bytes, T, keys);
-            // two short divergent paths instead of selects
-            if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
-                h += 1u;
-            } else {
-                A = c;
-                P += dp;
-            }
+            uint32_t c[K];
+            int32_t dp[K];
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                if (NOFLUX) {
+                    c[k] = zmove_noflux(A[k], T[k], dp[k]);
+                } else {
+                    c[k] = zmove_entry<WRAPFIX>(A[k], T[k]);
+                    dp[k] = (int32_t)T[k].dp;
+                }
+                zchunk_fill_buf(zrsrc, c[k], QB[k], QV[k]);
+            });
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                zdirection<WRAPFIX, 2, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys);
+            });
+            static_for<K>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                // two short divergent paths instead of selects
+                if (__builtin_expect(zchunk_wall<128>(c[k], QV[k]) != 0u, 0)) {
+                    H[k] += 1u;
+                } else {
+                    A[k] = c[k];
+                    P[k] += dp[k];
+                }
+            });
         };
         while (left) {
             const uint32_t ns = left < kZFlush - since ? left : kZFlush - since;
@@ -175,39 +208,48 @@ __global__ __launch_bounds__(kPfgseThreads) void zpfgse_kernel(const WalkArgs a,
                 flush();
         }
         flush();
-        hits += h;
-        static_for<MAXSETS>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            if (q < nsets)
-                e[q] *= pow(p.n_sets > 0 ? p.set_delta[q] : own_delta, (double)h);
+        double dX[K], dY[K], dZ[K];
+        static_for<K>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            hits[k] += H[k];
+            static_for<MAXSETS>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if (q < nsets)
+                    e[k][q] *= pow(p.n_sets > 0 ? p.set_delta[q] : own_delta[k], (double)H[k]);
+            });
+            dX[k] = (double)(X[k] - x0[k]);
+            dY[k] = (double)(Y[k] - y0[k]);
+            dZ[k] = (double)(Z[k] - z0[k]);
         });
-        const double dX = (double)(X - x0), dY = (double)(Y - y0), dZ = (double)(Z - z0);
-        pfgse_measure<ROT, MAXSETS>(p, i, dX, dY, dZ, e, nsets, active, lds);
+        pfgse_measure_k<ROT, MAXSETS, K>(p, i, dX, dY, dZ, e, nsets, lds);
         if (p.msd_rows)
-            pfgse_msd(p, i, dX, dY, dZ, active, lds);
+            pfgse_msd_k<K>(p, i, dX, dY, dZ, active, lds);
     }
-    s = xorshift64_inverse(s);
-    if (active) {
-        a.x[id] = X;
-        a.y[id] = Y;
-        a.z[id] = Z;
-        a.seed[id] = s;
-        if (a.energy)
-            a.energy[id] = e[0];
-        if (a.hits)
-            a.hits[id] = hits;
-    }
+    static_for<K>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        S[k] = xorshift64_inverse(S[k]);
+        if (active[k]) {
+            a.x[id[k]] = X[k];
+            a.y[id[k]] = Y[k];
+            a.z[id[k]] = Z[k];
+            a.seed[id[k]] = S[k];
+            if (a.energy)
+                a.energy[id[k]] = e[k][0];
+            if (a.hits)
+                a.hits[id[k]] = hits[k];
+        }
+    });
 }
 
 template <bool WRAPFIX, bool ROT, bool NOFLUX>
 static void launch_zpfgse_sets(const WalkArgs &a, const PfgseArgs &p, dim3 g, dim3 t, hipStream_t st)
 {
     if (p.n_sets <= 1)
-        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, 1>), g, t, 0, st, a, p);
+        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, 1, kPfgseK>), g, t, 0, st, a, p);
     else if (p.n_sets <= 2)
-        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, 2>), g, t, 0, st, a, p);
+        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, 2, kPfgseK>), g, t, 0, st, a, p);
     else
-        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, kMaxSets>), g, t, 0, st, a, p);
+        hipLaunchKernelGGL((zpfgse_kernel<WRAPFIX, ROT, NOFLUX, kMaxSets, kPfgseK>), g, t, 0, st, a, p);
 }
 
 hipError_t launch_zpfgse(const WalkArgs &a, const PfgseArgs &p, hipStream_t st)
