@@ -291,10 +291,13 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
                    valu_busy_measured=prof.get("valu_busy"), counters=os.path.relpath(COUNTERS_FILE, ROOT))
         if prof.get("sclk_GHz_measured"):
             # the same ceiling at the shader clock the counters saw during the
-            # walk (sustained load runs below the 2.4 GHz peak)
+            # walk (sustained load runs below the 2.4 GHz peak); the clock of
+            # this run's box can differ from the profiled one by a few percent
             f = float(prof["sclk_GHz_measured"]) * 1e9
             sec["valu_at_measured_clock"] = {"clock_GHz": f / 1e9, "peak": peak * f / CLOCK_HZ,
-                                             "frac": steps_per_s / (peak * f / CLOCK_HZ)}
+                                             "frac": steps_per_s / (peak * f / CLOCK_HZ),
+                                             "basis": "GRBM_COUNT per XCD over the walk in the committed "
+                                                      "counter run (profiles/walk_kernel_cpmg512.md)"}
         # HBM bytes per launch scale with the launch's walker-steps
         scale = steps_per_launch / float(prof["walker_steps_per_launch"])
         hbm = float(prof["hbm_bytes_per_launch"]) * scale
@@ -578,8 +581,10 @@ def run_fit(args):
     labels = R.voronoi_labels(solid, 64, 8, args.seed + 1)
     seeds, x, y, z = R.seed_and_place(solid, args.seed + 2, N)
     ctx = R.Context(0)
-    if args.launch:
-        ctx.set_launch(*(int(v) for v in args.launch.split(",")))
+    # 1024-thread blocks: the labelled forward 17.4 ms against 18.1 ms at
+    # 256 x 1 and 21.8 ms at 256 x 2 walkers per lane, the 64-individual
+    # scoring 0.176 s against 0.188 s (measured on this workload)
+    ctx.set_launch(*((int(v) for v in args.launch.split(",")) if args.launch else (1024, 1)))
     img = R.Image.from_solid(ctx, solid, labels=labels, n_labels=8)
     model = CpmgModel(ctx, img, x, y, z, seeds, echoes=E, steps_per_echo=spe, voxel=voxel, D0=D0, ckpt_every=500)
     rho_true = rng.uniform(5.0, 40.0, 8)
@@ -606,6 +611,7 @@ def run_fit(args):
     ga = genetic_fit(model, target, bounds_lo=np.concatenate([np.full(8, 1.0), [500.0]]),
                      bounds_hi=np.concatenate([np.full(8, 60.0), [5000.0]]), truth=truth, tol=1e-3, population=64,
                      time_limit=args.ga_time_limit, log=log)
+    launch = dict(zip(("threads_per_block", "walkers_per_lane"), ctx.get_tuning()[:2]))
     img.close()
     ctx.close()
 
@@ -618,7 +624,8 @@ def run_fit(args):
             "value": g["seconds"], "unit": "s", "higher_is_better": False, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"{S}^3 sphere pack (r 6-12, porosity {phi:.3f}), 64-site Voronoi labels 0-7, "
                                    f"{N} walkers, {E} echoes x {spe} steps, mirror, noise 0.5%, checkpoint every 500",
-                       "forward_s": fwd_s, "population64_scoring_s": batch64_s},
+                       "forward_s": fwd_s, "population64_scoring_s": batch64_s,
+                       "launch": launch},
             "gradient": {"converged": g["converged"], "seconds": g["seconds"], "evals": g["evals"],
                          "rms_truth": g["rms_truth"], "rho": np.asarray(g["rho"]).tolist(), "t2b": g["t2b"],
                          "recovery": recovery(g["rho"], g["t2b"])},
