@@ -351,6 +351,8 @@ def wl_small(args):
     seeds, x0, y0, z0 = R.seed_and_place(solid, args.seed, N)
     delta = R.penalty(rho, voxel, D0)
     ctx = R.Context(0)
+    if args.unfolded >= 0:
+        ctx.set_unfolded(args.unfolded)
     img = R.Image.from_solid(ctx, solid)
     dev = torch.device("cuda:0")
     X0, Y0, Z0 = (torch.from_numpy(a).to(dev) for a in (x0, y0, z0))
@@ -364,6 +366,8 @@ def wl_small(args):
         tpb, k, ms = ctx.autotune(img, X0, Y0, Z0, S0, steps_per_echo=spe, bc="mirror", trial_echoes=E, delta=delta)
         log(f"autotune: threads {tpb} x walkers/lane {k} ({ms:.3f} ms per {E} echoes)")
     tpb, k = ctx.get_launch()
+    form = ctx.get_tuning()[2]
+    log(f"image form {form}")
 
     def run():
         X.copy_(X0), Y.copy_(Y0), Z.copy_(Z0), SD.copy_(S0), EN.fill_(1.0)
@@ -407,7 +411,7 @@ def wl_small(args):
            "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"CPMG {S}^3 sphere pack (r 8, porosity {phi:.3f}), {N} walkers, {E} echoes x "
                                   f"{spe} steps, mirror, voxel {voxel} um, rho {rho} um/s, T2b {t2b} ms",
-                      "launch": f"{tpb}x{k}"},
+                      "launch": f"{tpb}x{k}", "image_form": form},
            "laplace": {"bins": 128, "lambda": lam, "ms": lap_ms, "t2_peak_ms": t2_peak},
            "cpu_baseline": {"value": steps / secsN, "unit": "walker-steps/s", "cores": threads, "kind": "port",
                             "sample": "all 10,000 walkers x 2000 echoes (reference loop order, digit-sum mod6)",
@@ -438,6 +442,8 @@ def wl_pfgse(args):
     seeds, x0, y0, z0 = R.seed_and_place(solid, args.seed, N)
     set_delta = np.array([1.0, R.penalty(10.0, voxel, D0)])
     ctx = R.Context(0)
+    if args.unfolded >= 0:
+        ctx.set_unfolded(args.unfolded)
     img = R.Image.from_solid(ctx, solid)
     dev = torch.device("cuda:0")
     X0, Y0, Z0 = (torch.from_numpy(a).to(dev) for a in (x0, y0, z0))

@@ -84,7 +84,9 @@ class Context:
         per-step fold), 2 unfolded + per-walker 16-byte chunk cache, 3 and 4
         the chunk tested by one shift and the next direction drawn a step
         ahead, through the 4 KiB byte-key table (3, default) or the 64 KiB
-        wide direction table (4)."""
+        wide direction table (4); 5 as 3, with CPMG of one walker per lane
+        loading both possible next chunks a step ahead (latency-bound runs
+        of few walkers)."""
         check(load().rwnmr_ctx_set_unfolded(self.handle, int(mode)), self.handle)
 
     def set_timing(self, enable: bool = True):

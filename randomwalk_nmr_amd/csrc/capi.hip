@@ -551,6 +551,9 @@ bool k_rows_arithmetic(const double *k, uint64_t nkv, uint32_t n_k, uint32_t n_r
 // Upper bound on the partial-sum scratch of one walk launch; longer CPMG runs
 // are split into several launches of fewer echoes.
 constexpr uint64_t kMaxPartialBytes = 1ull << 30;
+// walker counts up to one wave per SIMD (1024 SIMDs x 64 lanes): the
+// autotuner also times the speculative walk (form 5) for these
+constexpr uint64_t kSpecMaxWalkers = 1024ull * 64ull;
 
 // Walk in position order: every `resort_every` echoes the walkers are sorted
 // by the Morton code of their image block and gathered into context arrays,
@@ -921,8 +924,8 @@ rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *c, int32_t enable)
 {
     if (!c)
         return RWNMR_ERR_INVALID;
-    if (enable < 0 || enable > 4)
-        return fail(c, RWNMR_ERR_INVALID, "set_unfolded: mode must be 0 .. 4");
+    if (enable < 0 || enable > 5)
+        return fail(c, RWNMR_ERR_INVALID, "set_unfolded: mode must be 0 .. 5");
     c->unfolded = enable;
     return RWNMR_OK;
 }
@@ -1290,6 +1293,21 @@ rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *c, const rwnmr_image *img, const rwnm
         if (ms < best_t) {
             best_t = ms;
             best_form = 0;
+        }
+        c->unfolded = best_form;
+    }
+    // runs of at most one wave per SIMD are latency-bound: the speculative
+    // two-candidate walk (form 5, one walker per lane) against the rest
+    if (saved_unfolded >= 3 && c->shape.k == 1 && n <= kSpecMaxWalkers) {
+        c->unfolded = 5;
+        double ms = 0.0;
+        if ((s = trial(ms, false)) != RWNMR_OK) {
+            restore();
+            return s;
+        }
+        if (ms < best_t) {
+            best_t = ms;
+            best_form = 5;
         }
         c->unfolded = best_form;
     }

@@ -709,4 +709,139 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     });
 }
 
+// Speculative CPMG tile for latency-bound runs (few waves per SIMD, where a
+// step waits for its chunk load with nothing else to issue): one walker per
+// lane, no chunk cache, directions drawn two steps ahead.  At the start of
+// step t both possible candidates of step t + 1 are formed, from the current
+// position (if step t hits a wall) and from step t's candidate (if it
+// moves), and both of their 16-byte chunks are loaded; then the direction of
+// step t + 2 is drawn, and only then does step t wait for its own chunk,
+// loaded at the start of step t - 1 (the empty asm holds that wait after the
+// draw).  Two chunk loads per walker-step (more VALU and L2 requests than
+// the chunk-cache walk), so it pays only when the walk is latency-bound.
+// Same walk, bit for bit, as zwalk_tile (EMODE 1).
+template <bool WRAPFIX, bool NOFLUX, bool MFLAGS>
+__device__ __forceinline__ void zwalk_tile_spec(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
+                                                double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
+                                                const uint8_t *keys)
+{
+    const zrsrc_t zrsrc = zchunk_rsrc(a.zwords, a.zl);
+    const uint64_t id = tile * blockDim.x + threadIdx.x;
+    const uint32_t echoes = a.echoes, spe = a.steps_per_echo;
+    bool act = id < a.n;
+    const uint64_t j = act ? id : 0;
+    int32_t X = act ? a.x[j] : 0, Y = act ? a.y[j] : 0, Z = act ? a.z[j] : 0;
+    if (NOFLUX && act && !((unsigned)X < (unsigned)a.W && (unsigned)Y < (unsigned)a.H && (unsigned)Z < (unsigned)a.D)) {
+        act = false;   // a noflux walker must start inside the image
+        atomicOr(a.error_flag, 1u);
+        X = Y = Z = 0;
+    }
+    uint64_t S = act ? a.seed[j] : 1ull;
+    double E = act ? (a.energy ? a.energy[j] : 1.0) : 0.0;   // idle lanes add 0 to echo sums
+    uint32_t Hc = (act && a.hits) ? a.hits[j] : 0u;
+    const double DL = (act && a.penalty) ? a.penalty[j] : a.delta_uniform;
+    uint32_t A, FO = 0u;
+    if (MFLAGS) {
+        const int32_t w = (int32_t)(a.zl.period[0] >> 1), h = (int32_t)(a.zl.period[1] >> 1),
+                      d = (int32_t)(a.zl.period[2] >> 1);
+        A = zaddress(fold_mirror(X, w), fold_mirror(Y, h), fold_mirror(Z, d), a.zl);
+        FO = kZFlagStride * ((uint32_t)mirror_flag(X, w) | ((uint32_t)mirror_flag(Y, h) << 1) |
+                             ((uint32_t)mirror_flag(Z, d) << 2));
+    } else {
+        A = zaddress(X, Y, Z, a.zl);
+    }
+    int32_t P = 0;
+    const char *tab_bytes = reinterpret_cast<const char *>(tab);
+    const char *fix_bytes = reinterpret_cast<const char *>(fix);
+    auto move = [&](uint32_t from, const ZEntry &t, int32_t &dp) -> uint32_t {
+        if (NOFLUX)
+            return zmove_noflux(from, t, dp);
+        dp = (int32_t)t.dp;
+        return zmove_entry<WRAPFIX>(from, t);
+    };
+    auto chunk = [&](uint32_t c) -> uint4 {
+        const zv4u v = zchunk_load(zrsrc, c >> 7);
+        return make_uint4(v.x, v.y, v.z, v.w);
+    };
+    // the first step's candidate C and its chunk (in both slots), and the
+    // second step's direction T
+    ZEntry T;
+    zdirection<WRAPFIX, 2, NOFLUX>(S, tab_bytes, fix_bytes, T, keys, FO);
+    int32_t DP;
+    uint32_t C = move(A, T, DP);
+    uint4 QS = chunk(C), QM = chunk(C);
+    bool stay = true;   // which of QS / QM holds the chunk of C
+    zdirection<WRAPFIX, 2, NOFLUX>(S, tab_bytes, fix_bytes, T, keys, FO);
+    auto step = [&]() {
+        int32_t dps, dpm;
+        const uint32_t cs = move(A, T, dps);   // next candidate if this step hits
+        const uint32_t cm = move(C, T, dpm);   // next candidate if this step moves
+        const uint4 qs = chunk(cs);
+        const uint4 qm = chunk(cm);
+        ZEntry Tn;
+        zdirection<WRAPFIX, 2, NOFLUX>(S, tab_bytes, fix_bytes, Tn, keys, FO);
+        // this step's chunks were loaded a step ago: wait for them here,
+        // after the loads above and the draw, not before
+        asm volatile("" : "+v"(QS.x), "+v"(QS.y), "+v"(QS.z), "+v"(QS.w), "+v"(QM.x), "+v"(QM.y), "+v"(QM.z),
+                     "+v"(QM.w)::"memory");
+        const uint4 q = stay ? QS : QM;
+        const bool wall = zchunk_wall<128>(C, q) != 0u;
+        if (__builtin_expect(wall, 0)) {
+            Hc += 1u;
+            E *= DL;
+        } else {
+            A = C;
+            P += DP;
+        }
+        C = wall ? cs : cm;
+        DP = wall ? dps : dpm;
+        QS = qs;
+        QM = qm;
+        stay = wall;
+        T = Tn;
+    };
+    uint32_t since = 0;
+    auto flush = [&]() {
+        int32_t dx, dy, dz;
+        zunpack(P, dx, dy, dz);
+        X += dx;
+        Y += dy;
+        Z += dz;
+        P = 0;
+    };
+    auto walk_echo = [&]() {
+        uint32_t left = spe;
+        while (left) {
+            const uint32_t ns = left < kZFlush - since ? left : kZFlush - since;
+            uint32_t i = 0;
+            for (; i + 2 <= ns; i += 2) {
+                step();
+                step();
+            }
+            if (i < ns)
+                step();
+            since += ns;
+            left -= ns;
+            if (since == kZFlush) {
+                flush();
+                since = 0;
+            }
+        }
+    };
+    run_echoes<true>(echoes, lds, part, part_stride, walk_echo, [&]() { return E; });
+    flush();
+    // the directions were drawn two steps ahead
+    S = xorshift64_inverse(xorshift64_inverse(S));
+    if (act) {
+        a.x[id] = X;
+        a.y[id] = Y;
+        a.z[id] = Z;
+        a.seed[id] = S;
+        if (a.energy)
+            a.energy[id] = E;
+        if (a.hits)
+            a.hits[id] = Hc;
+    }
+}
+
 }  // namespace rwnmr

@@ -44,6 +44,30 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
                                                               EMODE == 2 && K == 1 ? zlh : nullptr);
 }
 
+// Speculative CPMG walk (zwalk_tile_spec): one walker per lane, for runs
+// too small to hide the chunk-load latency of a step.
+template <bool WRAPFIX, bool NOFLUX, bool MFLAGS>
+__global__ __launch_bounds__(1024) void zwalk_spec_kernel(const WalkArgs a)
+{
+    __shared__ double lds[kEchoChunk][kEchoCols];
+    __shared__ uint4 tab[MFLAGS ? 8 * kZFlagStride / 16 : (NOFLUX ? 12 : 6)];
+    __shared__ uint2 fix[MFLAGS ? 8 * kZFlagStride / 16 : 6];
+    __shared__ uint8_t keys[kZKeyEntries];
+    const uint64_t tile = tile_of_block(a);
+    if (a.tile_map && tile >= a.n_tiles)
+        return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
+    if (NOFLUX)
+        zbuild_table_noflux(a.zl, tab);
+    else if (MFLAGS)
+        zbuild_table_flags(a.zl, tab, fix);
+    else
+        zbuild_table(a.zl, tab, fix);
+    zbuild_key_table(keys, NOFLUX ? 32u : 16u);
+    __syncthreads();
+    zwalk_tile_spec<WRAPFIX, NOFLUX, MFLAGS>(a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride,
+                                             lds, tab, fix, keys);
+}
+
 // Label-set CPMG: 8 sets of label deltas evaluated on one trajectory (one
 // walker per lane; zwalk_tile with NS = 8).  Unused sets (a.n_label_sets <
 // 8) carry deltas of 1.
@@ -396,9 +420,32 @@ static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t bl
 // 128-bit chunk with the shift test and the direction drawn one step ahead,
 // through the byte-key table (2) or, for power-of-two periods, the wide
 // direction table (3; the 6-entry table otherwise).
+template <bool WRAPFIX, bool NOFLUX, bool MFLAGS>
+static hipError_t launch_z_spec(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
+{
+    hipLaunchKernelGGL((zwalk_spec_kernel<WRAPFIX, NOFLUX, MFLAGS>), dim3(blocks), dim3(ls.threads), 0, st, a);
+    return hipGetLastError();
+}
+
 template <int EMODE>
 static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
+    // a.zcache 4: the speculative CPMG walk (one walker per lane); other
+    // modes and lane shapes take the pipelined chunk-cache walk
+    if (a.zcache == 4 && EMODE == 1 && ls.k == 1) {
+        if (a.znoflux)
+            return launch_z_spec<false, true, false>(a, ls, blocks, st);
+        if (a.zmflags)
+            return a.zl.wrapfix ? launch_z_spec<true, false, true>(a, ls, blocks, st)
+                                : launch_z_spec<false, false, true>(a, ls, blocks, st);
+        return a.zl.wrapfix ? launch_z_spec<true, false, false>(a, ls, blocks, st)
+                            : launch_z_spec<false, false, false>(a, ls, blocks, st);
+    }
+    if (a.zcache == 4) {
+        WalkArgs b = a;
+        b.zcache = 2;
+        return launch_z_mode<EMODE>(b, ls, blocks, st);
+    }
     if (a.znoflux)   // noflux on the periodic image: pipelined byte-key walk only
         return a.zcache >= 2 ? launch_z<EMODE, false, 128, 2, true>(a, ls, blocks, st) : hipErrorInvalidValue;
     if (a.zmflags && a.zcache >= 2)   // mirror, walkers folded to the first octant with reflection flags

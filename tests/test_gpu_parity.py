@@ -81,13 +81,14 @@ def test_locality_order_and_brick_layout_match_oracle(ctx, bc, layout, shape):
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("dims", [(32, 16, 8), (16, 16, 16), (30, 21, 17), (9, 1, 5), (2, 3, 64)])
-@pytest.mark.parametrize("unfolded", [4, 3, 2, 1, 0])
+@pytest.mark.parametrize("unfolded", [5, 4, 3, 2, 1, 0])
 @pytest.mark.parametrize("shape", [(256, 1), (512, 2), (256, 4)])
 def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
     """Noflux / periodic / mirror CPMG and map walks on the unfolded image
     with and without the per-walker chunk cache (power-of-two periods: plain
     dilated wrap; others: wrap fix; noflux: face test and turn-around on the
-    periodic image, folded kernel when a dimension is 1) and on the folded
+    periodic image, folded kernel when a dimension is 1), the speculative
+    two-candidate CPMG walk (form 5, one walker per lane) and the folded
     kernel: walkers bit-exact, echo sums to 1e-12, for every launch shape."""
     print(f"[test] unfolded={unfolded} bc={bc} dims={dims} launch={shape}", flush=True)
     W, H, D = dims
@@ -120,7 +121,7 @@ def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("spe", [1, 600])
-@pytest.mark.parametrize("unfolded", [4, 3, 2])
+@pytest.mark.parametrize("unfolded", [5, 4, 3, 2])
 def test_unfolded_long_echoes_and_locality_match_oracle(ctx, bc, spe, unfolded):
     """Steps per echo above the packed-displacement flush interval (511) and
     the Z-order re-sort of the unfolded walk on a power-of-two image."""
