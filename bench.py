@@ -612,6 +612,11 @@ def run_fit(args):
     model.decays(pop[:, :-1], pop[:, -1])
     torch.cuda.synchronize()
     batch64_s = time.perf_counter() - t0
+    # one untimed forward + backward, so both fits start with every kernel
+    # loaded (the GA's label-set walk ran above; HIP loads code on first use)
+    th = torch.tensor(np.log(np.full(9, 20.0)), requires_grad=True)
+    torch.mean(model.decay(torch.exp(th[:-1]), torch.exp(th[-1])).cpu() ** 2).backward()
+    torch.cuda.synchronize()
     g = gradient_fit(model, target, rho0=np.full(8, 20.0), t2b0=2000.0, truth=truth, tol=1e-3, max_evals=60,
                      log=log)
     ga = genetic_fit(model, target, bounds_lo=np.concatenate([np.full(8, 1.0), [500.0]]),

@@ -250,7 +250,7 @@ def gradient_fit(model: CpmgModel, target, *, rho0, t2b0, truth=None, tol=1e-3, 
         hist.append(dict(t=time.perf_counter() - t0, loss=float(loss.item()), rms_truth=err,
                          rho=torch.exp(th[:-1]).detach().numpy().copy(), t2b=float(t2b.item())))
         if log:
-            log(f"fit eval {len(hist)}: rms vs truth {err:.3e} loss {loss.item():.3e}")
+            log(f"fit eval {len(hist)} at {hist[-1]['t']:.3f} s: rms vs truth {err:.3e} loss {loss.item():.3e}")
         if err <= tol and not state["done"]:
             state["done"] = True
             state["best"] = hist[-1]
