@@ -496,7 +496,7 @@ __device__ __forceinline__ void pfgse_measure_k(const PfgseArgs &p, uint32_t i, 
             // the 8 cosines of walker w (ROT: the recurrence advanced 8 k)
             // handed to use(j, cos) in k order
             double cs[ROT ? 1 : K][8];
-            if (!ROT)
+            if constexpr (!ROT)
                 static_for<K>([&](auto kc_) {
                     constexpr int w = decltype(kc_)::value;
 #pragma unroll
@@ -517,7 +517,7 @@ __device__ __forceinline__ void pfgse_measure_k(const PfgseArgs &p, uint32_t i, 
             });
             auto cosines = [&](auto wc, auto &&use) {
                 constexpr int w = decltype(wc)::value;
-                if (ROT) {
+                if constexpr (ROT) {
                     // every set pair runs the 8 recurrence steps from the
                     // group's start (the cosines are not kept across pairs)
                     double cc = c0[w], cp = cp0[w];
@@ -542,13 +542,16 @@ __device__ __forceinline__ void pfgse_measure_k(const PfgseArgs &p, uint32_t i, 
             // (lanes 8j .. 8j + 7)
             static_for<(MAXSETS + 1) / 2>([&](auto pc) {
                 constexpr int q0 = 2 * decltype(pc)::value;
-                if (q0 + 1 < nsets) {
+                // the pair's second set (in bounds also where MAXSETS is odd
+                // and the pair branch cannot run)
+                constexpr int q1 = q0 + 1 < MAXSETS ? q0 + 1 : q0;
+                if (q0 + 1 < MAXSETS && q0 + 1 < nsets) {
                     double t[16];
                     static_for<K>([&](auto wc) {
                         constexpr int w = decltype(wc)::value;
                         cosines(wc, [&](int j, double cv) {
                             t[j] = w == 0 ? cv * e[w][q0] : fma(cv, e[w][q0], t[j]);
-                            t[8 + j] = w == 0 ? cv * e[w][q0 + 1] : fma(cv, e[w][q0 + 1], t[8 + j]);
+                            t[8 + j] = w == 0 ? cv * e[w][q1] : fma(cv, e[w][q1], t[8 + j]);
                         });
                     });
                     const double v = wave_reduce_scatter<16>(t);

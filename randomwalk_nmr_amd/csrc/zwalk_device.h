@@ -97,6 +97,49 @@ __device__ __forceinline__ void zbuild_key_table(uint8_t *keys, uint32_t bytes =
         keys[e] = (uint8_t)(bytes * (2u * ((e & 2047u) % 3u) + (e >> 11)));
 }
 
+// Flag-keyed byte tables (TAB 3; mirror walks with reflection flags): the
+// walker's flags select one of 8 byte-key tables (key + flags x 4096, 32 KiB
+// of LDS), whose bytes point straight at the direction entry for those
+// flags, so the step adds no per-walker table offset (the flags enter as
+// the accumulator of the byte sum, with the parity).  Twelve entries serve
+// every flag combination: reflecting an axis flips the dilated addend and
+// the wrap fix-up of its two directions but not their packed displacement
+// (the unbounded move), so entry 2 idx + f holds direction idx with its axis
+// reflected when f is 1 (16 bytes each; wrap fix-up entries 8 bytes).
+constexpr uint32_t kZFlagKeyEntries = 8u * kZKeyEntries;
+constexpr uint32_t kZFlippedEntries = 12u;
+
+// Threads 0..11 write; the caller syncs.
+__device__ __forceinline__ void zbuild_table_flipped(const ZLayout &zl, uint4 *tab, uint2 *fix)
+{
+    if (threadIdx.x < kZFlippedEntries) {
+        const uint32_t idx = threadIdx.x >> 1, f = threadIdx.x & 1u;
+        const uint32_t ax = (uint32_t)move_axis(zdir_residue(idx));
+        const ZDir d = zdir_make(zl, idx, f << ax);
+        tab[threadIdx.x] = make_uint4(d.m, d.add, (uint32_t)d.dp, d.nm);
+        fix[threadIdx.x] = make_uint2(d.wrapv, d.fixv);
+    }
+}
+
+// The byte of key e (flags e >> 12, parity (e >> 11) & 1, byte sum e & 2047)
+// is the offset 16 (2 idx + flip) of its entry in zbuild_table_flipped, with
+// idx = 2 (sb % 3) + parity as in zbuild_key_table.  All threads of the block
+// write 4 keys per word; the caller syncs.
+__device__ __forceinline__ void zbuild_flag_key_table(uint32_t *keys)
+{
+    for (uint32_t w = threadIdx.x; w < kZFlagKeyEntries / 4u; w += blockDim.x) {
+        uint32_t v = 0u;
+        for (uint32_t b = 0; b < 4u; ++b) {
+            const uint32_t e = 4u * w + b;
+            const uint32_t flags = e >> 12, key = e & 4095u;
+            const uint32_t idx = 2u * ((key & 2047u) % 3u) + (key >> 11);
+            const uint32_t ax = (uint32_t)move_axis(zdir_residue(idx));
+            v |= (16u * (2u * idx + ((flags >> ax) & 1u))) << (8u * b);
+        }
+        keys[w] = v;
+    }
+}
+
 // Noflux direction table of layout zl (periodic image): two 16-byte halves
 // per direction, {m, add, dp, tgt} and {add2, dp2, mp, mm} (ZFlip).
 // Threads 0..5 write; the caller syncs.
@@ -113,13 +156,16 @@ __device__ __forceinline__ void zbuild_table_noflux(const ZLayout &zl, uint4 *ta
 // the table entry (16 bytes per entry in tab_bytes, 8 in fix_bytes) of
 // state % 6.  TAB 0: the 6-entry table, s % 3 by arithmetic; 1: tab_bytes is
 // the wide table (zbuild_table_wide); 2: the 6-entry table reached through
-// the byte-key table `keys` (zbuild_key_table).
+// the byte-key table `keys` (zbuild_key_table); 3: the 12-entry table of
+// zbuild_table_flipped reached through the flag-keyed tables `keys`
+// (zbuild_flag_key_table), fo = flags x 4096.
 struct ZEntry {
     uint32_t m, add, dp, fx, fy;      // axis mask, dilated addend, packed displacement, wrap fix-up
     uint32_t tgt, add2, dp2, mp, mm;  // noflux: face test and the turned-around move (ZFlip)
 };
 
-// fo: the walker's table offset (mirror walks with reflection flags), else 0.
+// fo: the walker's table offset (mirror walks with reflection flags), else 0;
+// TAB 3: its key offset.
 template <bool WRAPFIX, int TAB = 0, bool NOFLUX = false>
 __device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, const char *fix_bytes, ZEntry &e,
                                            const uint8_t *keys = nullptr, uint32_t fo = 0u)
@@ -129,7 +175,11 @@ __device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, c
     s = xorshift64(s);
     const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
     uint32_t off;
-    if (TAB == 2) {
+    if (TAB == 3) {
+        // the flags' key table: key offset and parity in one and-or
+        const uint32_t key = __builtin_amdgcn_sad_u8(lo, 0u, __builtin_amdgcn_sad_u8(hi, 0u, ((lo << 11) & 0x800u) | fo));
+        off = keys[key];
+    } else if (TAB == 2) {
         // key = byte sum + (state & 1) * 2048, the parity entering as the
         // accumulator of the first byte sum
         const uint32_t key = __builtin_amdgcn_sad_u8(lo, 0u, __builtin_amdgcn_sad_u8(hi, 0u, (lo << 11) & 0x800u));
@@ -145,7 +195,8 @@ __device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, c
         const uint32_t q = __umul24(sb, 2731u) >> 13;
         off = (uint32_t)__mul24((int)q, -96) + ((sb << 5) | ((lo & 1u) << 4));
     }
-    off += fo;
+    if (TAB != 3)
+        off += fo;
     const uint4 t = *reinterpret_cast<const uint4 *>(tab_bytes + off);
     e.m = t.x;
     e.add = t.y;
@@ -378,7 +429,8 @@ __device__ __forceinline__ uint32_t zlabel_at(const WalkArgs &a, uint32_t c)
 // EMODE 2 with K == 1 keeps the per-label hit counts in LDS (lh: blockDim
 // rows of kLhStride entries; one LDS add per hit instead of a compare and
 // add per label), else in registers.
-// MFLAGS: mirror walk with reflection flags (tab / fix: zbuild_table_flags).
+// MFLAGS: mirror walk with reflection flags (tab / fix: zbuild_table_flags,
+// or zbuild_table_flipped with TAB 3).
 // NS == 8 (EMODE 2, K == 1, pipelined chunk cache): label-set walk, each
 // walker carrying 8 energies, one per set of label deltas (ldelta: 8 rows of
 // kMaxLabels, set-major), all starting from the walker's energy; a hit
@@ -396,6 +448,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
 {
     constexpr bool WIDE = TAB == 1;
     static_assert(!NOFLUX || CACHE >= 64, "noflux: pipelined chunk-cache walk only");
+    static_assert(TAB != 3 || (MFLAGS && CACHE >= 64), "flag-keyed tables: pipelined mirror walk only");
     static_assert(NS == 1 || (NS == 8 && EMODE == 2 && K == 1 && CACHE >= 64), "label sets: 8 sets, K == 1");
     // label-set walks count no per-label hits (a.label_hits is null)
     constexpr bool LH_LDS = EMODE == 2 && K == 1 && NS == 1;
@@ -448,8 +501,9 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             const int32_t w = (int32_t)(a.zl.period[0] >> 1), h = (int32_t)(a.zl.period[1] >> 1),
                           d = (int32_t)(a.zl.period[2] >> 1);
             A[k] = zaddress(fold_mirror(X[k], w), fold_mirror(Y[k], h), fold_mirror(Z[k], d), a.zl);
-            FO[k] = kZFlagStride * ((uint32_t)mirror_flag(X[k], w) | ((uint32_t)mirror_flag(Y[k], h) << 1) |
-                                    ((uint32_t)mirror_flag(Z[k], d) << 2));
+            FO[k] = (TAB == 3 ? kZKeyEntries : kZFlagStride) *
+                    ((uint32_t)mirror_flag(X[k], w) | ((uint32_t)mirror_flag(Y[k], h) << 1) |
+                     ((uint32_t)mirror_flag(Z[k], d) << 2));
         } else {
             A[k] = zaddress(X[k], Y[k], Z[k], a.zl);
             FO[k] = 0u;

@@ -15,23 +15,29 @@ template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = fals
 __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kEchoCols];
-    __shared__ uint4 tab[TAB == 1 ? kZWideEntries : (MFLAGS ? 8 * kZFlagStride / 16 : (NOFLUX ? 12 : 6))];
+    __shared__ uint4 tab[TAB == 1 ? kZWideEntries
+                                  : (TAB == 3 ? kZFlippedEntries : (MFLAGS ? 8 * kZFlagStride / 16 : (NOFLUX ? 12 : 6)))];
     __shared__ uint4 small[6];
-    __shared__ uint2 fix[MFLAGS ? 8 * kZFlagStride / 16 : 6];
-    __shared__ uint8_t keys[TAB == 2 ? kZKeyEntries : 4];
+    __shared__ uint2 fix[TAB == 3 ? kZFlippedEntries : (MFLAGS ? 8 * kZFlagStride / 16 : 6)];
+    __shared__ uint32_t keys[TAB == 3 ? kZFlagKeyEntries / 4 : (TAB == 2 ? kZKeyEntries / 4 : 1)];
     __shared__ double ldelta[kMaxLabels];
     extern __shared__ uint32_t zlh[];   // EMODE 2, K == 1: per-label hit counts (blockDim x kLhStride)
     const uint64_t tile = tile_of_block(a);
     if (a.tile_map && tile >= a.n_tiles)
         return;  // spare workgroup of the XCD-striped grid (whole block leaves together)
+    uint8_t *key_bytes = reinterpret_cast<uint8_t *>(keys);
     if (NOFLUX)
         zbuild_table_noflux(a.zl, tab);
+    else if (TAB == 3)
+        zbuild_table_flipped(a.zl, tab, fix);
     else if (MFLAGS)
         zbuild_table_flags(a.zl, tab, fix);
     else
         zbuild_table(a.zl, TAB == 1 ? small : tab, fix);
+    if (TAB == 3)
+        zbuild_flag_key_table(keys);
     if (TAB == 2)
-        zbuild_key_table(keys, NOFLUX ? 32u : 16u);
+        zbuild_key_table(key_bytes, NOFLUX ? 32u : 16u);
     if (EMODE == 2 && threadIdx.x < kMaxLabels)
         ldelta[threadIdx.x] = (int)threadIdx.x < a.n_labels ? a.label_delta[threadIdx.x] : 1.0;
     __syncthreads();
@@ -40,7 +46,7 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
         __syncthreads();
     }
     zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS>(a, tile, a.partials ? a.partials + tile : nullptr,
-                                                              a.partial_stride, lds, tab, fix, ldelta, keys,
+                                                              a.partial_stride, lds, tab, fix, ldelta, key_bytes,
                                                               EMODE == 2 && K == 1 ? zlh : nullptr);
 }
 
@@ -448,9 +454,15 @@ static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32
     }
     if (a.znoflux)   // noflux on the periodic image: pipelined byte-key walk only
         return a.zcache >= 2 ? launch_z<EMODE, false, 128, 2, true>(a, ls, blocks, st) : hipErrorInvalidValue;
-    if (a.zmflags && a.zcache >= 2)   // mirror, walkers folded to the first octant with reflection flags
+    if (a.zmflags && a.zcache >= 2) {   // mirror, walkers folded to the first octant with reflection flags
+        // CPMG and map walks of blocks of 512 threads or more read the
+        // flag-keyed tables (32 KiB of LDS, still 8 waves per SIMD there)
+        if (EMODE < 2 && ls.threads >= 512)
+            return a.zl.wrapfix ? launch_z<EMODE, true, 128, 3, false, true>(a, ls, blocks, st)
+                                : launch_z<EMODE, false, 128, 3, false, true>(a, ls, blocks, st);
         return a.zl.wrapfix ? launch_z<EMODE, true, 128, 2, false, true>(a, ls, blocks, st)
                             : launch_z<EMODE, false, 128, 2, false, true>(a, ls, blocks, st);
+    }
     if (a.zl.wrapfix) {
         switch (a.zcache) {
         case 0: return launch_z<EMODE, true, 0, 0>(a, ls, blocks, st);

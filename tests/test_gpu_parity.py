@@ -142,6 +142,49 @@ def test_unfolded_long_echoes_and_locality_match_oracle(ctx, bc, spe, unfolded):
     _assert_echoes(g["echo_sums"], o["echo_sums"])
 
 
+def _mirror_images(solid, x, y, z, seed):
+    """The walkers moved into random mirror images (-2 .. 2 per axis) of
+    their start voxels: an odd image holds the reflection W - 1 - x, so every
+    start folds back to the same pore voxel and carries nonzero reflection
+    flags."""
+    D, H, W = solid.shape
+    rng = np.random.default_rng(seed)
+    out = []
+    for c, L in ((x, W), (y, H), (z, D)):
+        m = rng.integers(-2, 3, len(c))
+        out.append((m * L + np.where(m % 2 == 0, c, L - 1 - c)).astype(np.int32))
+    return out
+
+
+@pytest.mark.parametrize("dims", [(32, 16, 8), (30, 21, 17)])
+@pytest.mark.parametrize("shape", [(256, 1), (512, 1), (1024, 1), (512, 2)])
+def test_mirror_reflection_flags_match_oracle(ctx, dims, shape):
+    """Mirror CPMG and map walks from starts in every mirror image, so each
+    launch folds its walkers with all 8 reflection-flag combinations, for
+    the launch shapes of both direction-table forms (blocks of 512 threads
+    or more read the flag-keyed tables) and periods with and without the
+    wrap fix-up; a locality re-sort every 5 echoes makes later launches
+    start from the walkers' moved positions."""
+    print(f"[test] mirror flags dims={dims} launch={shape}", flush=True)
+    W, H, D = dims
+    solid, seeds, x, y, z, pen = _case(W, H, D, 3.5, 0.35, 3001, 41)
+    x, y, z = _mirror_images(solid, x, y, z, 43)
+    ctx.set_launch(*shape)
+    ctx.set_locality(True, resort_every=5, min_walkers=0)
+    try:
+        img = R.Image.from_solid(ctx, solid)
+        g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=23, steps_per_echo=7, bc="mirror", penalty=pen)
+        m = R.map_run(ctx, img, x, y, z, seeds, steps=23 * 7, bc="mirror")
+    finally:
+        ctx.set_locality(True, resort_every=1000, min_walkers=65536)
+        ctx.set_launch(256, 1)
+    o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=23, steps_per_echo=7, bc="mirror", penalty=pen)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(m[k], o[k]), k
+
+
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 def test_map_matches_oracle(ctx, bc):
     print(f"[test] map bc={bc}", flush=True)
