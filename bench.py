@@ -246,7 +246,8 @@ def main():
                               "image_form": {3: "unfolded + register chunk cache", 0: "folded bitBlock bricks"}.get(
                                   form, f"mode {form}"),
                               "echoes_between_resorts": resort}},
-        "roofline": roofline(args, prof, probe, kernel_steps_per_s, steps_per_launch, avg_kernel_s, launches, k),
+        "roofline": roofline(args, prof, probe, kernel_steps_per_s, steps_per_launch, avg_kernel_s, launches, k,
+                             tpb),
         "cpu_baseline": cpu,
         "checks": {"decay_monotone": decay_ok, "subsample_bit_exact_vs_cpu": parity},
     }
@@ -258,16 +259,19 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def walk_kernel_label(bc, k):
-    form = {"mirror": "unfolded mirror image, walkers folded into its first octant with reflection-flag direction "
-                      "tables at every launch",
+def walk_kernel_label(bc, k, threads):
+    form = {"mirror": "unfolded mirror image, walkers folded into its first octant with their reflection flags "
+                      "at every launch",
             "periodic": "unfolded periodic image",
             "noflux": "periodic unfolded image with the noflux face test and turn-around"}[bc]
+    # mirror walks of blocks of 512 threads or more read flag-keyed byte tables
+    table = ("flag-keyed byte-key tables" if bc == "mirror" and threads >= 512 else
+             "byte-key direction table" + (" per reflection-flag combination" if bc == "mirror" else ""))
     return (f"zwalk_kernel<CPMG, {k} walker(s) per lane, 16-byte register chunk cache by indexed buffer loads, "
-            f"byte-key direction table, direction drawn a step ahead> on the {form}")
+            f"{table}, direction drawn a step ahead> on the {form}")
 
 
-def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, launches, k):
+def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, launches, k, threads):
     """Roofline of the dominant kernel.  The binding roof is VALU issue: the
     walk keeps each walker's 16-byte image chunk in registers and the L2
     serves nearly every reload, so HBM is far from busy; its ceiling in
@@ -277,7 +281,7 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
     gathers), the measured HBM bytes per launch, and the walk's L2 request
     rate against the random-gather probe's."""
     out = {"bound": "valu", "achieved": steps_per_s, "peak": None, "unit": "walker-steps/s", "frac": None,
-           "traffic": None, "kernel": walk_kernel_label(args.bc, k), "avg_kernel_ms": avg_kernel_s * 1e3,
+           "traffic": None, "kernel": walk_kernel_label(args.bc, k, threads), "avg_kernel_ms": avg_kernel_s * 1e3,
            "launches": launches, "walker_steps_per_launch": steps_per_launch,
            "peak_basis": f"{SIMDS} SIMDs x {CLOCK_HZ / 1e9} GHz / {CYCLES_PER_VALU} cycles per wave64 VALU "
                          "instruction x 64 lanes / VALU instructions per wave-step"}
@@ -289,15 +293,20 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
         peak = VALU_WAVE_INSTR_PER_S * 64.0 / vps
         out.update(peak=peak, frac=steps_per_s / peak, valu_per_wave_step=vps,
                    valu_busy_measured=prof.get("valu_busy"), counters=os.path.relpath(COUNTERS_FILE, ROOT))
-        if prof.get("sclk_GHz_measured"):
-            # the same ceiling at the shader clock the counters saw during the
-            # walk (sustained load runs below the 2.4 GHz peak); the clock of
-            # this run's box can differ from the profiled one by a few percent
-            f = float(prof["sclk_GHz_measured"]) * 1e9
+        if prof.get("shader_cycles_per_launch"):
+            # the same ceiling at the shader clock this run ran at: a launch
+            # takes a fixed number of shader cycles (the committed counter
+            # runs: constant across launches of different durations), so
+            # this run's clock is those cycles over its own launch time;
+            # sustained load runs below the 2.4 GHz peak
+            cyc = float(prof["shader_cycles_per_launch"]) * steps_per_launch / float(prof["walker_steps_per_launch"])
+            f = cyc / avg_kernel_s
             sec["valu_at_measured_clock"] = {"clock_GHz": f / 1e9, "peak": peak * f / CLOCK_HZ,
                                              "frac": steps_per_s / (peak * f / CLOCK_HZ),
-                                             "basis": "GRBM_COUNT per XCD over the walk in the committed "
-                                                      "counter run (profiles/walk_kernel_cpmg512.md)"}
+                                             "counter_run_clock_GHz": prof.get("sclk_GHz_measured"),
+                                             "basis": "shader cycles per launch (GRBM_COUNT per XCD, committed "
+                                                      "counter run, profiles/walk_kernel_cpmg512.md) / this run's "
+                                                      "average launch time"}
         # HBM bytes per launch scale with the launch's walker-steps
         scale = steps_per_launch / float(prof["walker_steps_per_launch"])
         hbm = float(prof["hbm_bytes_per_launch"]) * scale
