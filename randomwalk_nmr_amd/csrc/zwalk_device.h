@@ -3,10 +3,10 @@
 // persistent kernel.  Layout and move arithmetic: walk_common.h, "Unfolded
 // image with dilated voxel addresses".
 //
-// Per step and walker, all in VGPRs: xorshift64; s % 3 from the byte sum of
-// the state (two v_sad_u8) and a 24-bit multiply; s % 2 from the low bit;
-// the pair indexes a 6-entry direction table in LDS (axis mask, dilated
-// addend, packed displacement); the dilated add gives the candidate voxel
+// Per step and walker, all in VGPRs: xorshift64; a key congruent to the
+// state mod 6 from two byte dot products (zdot_key) indexes a byte table in
+// LDS that points at the direction entry (axis mask, dilated addend, packed
+// displacement) of s % 6; the dilated add gives the candidate voxel
 // address with the periodic wrap included; then the wall bit: with the
 // register chunk cache (CACHE != 0) each walker keeps the 16-byte chunk
 // (8 x 4 x 4 voxels) of its last lookup and reads memory only when the
@@ -83,25 +83,38 @@ __device__ __forceinline__ void zbuild_table_wide(const uint4 *small, uint4 *tab
     }
 }
 
-// Byte-key table: for key = (state & 1) * 2048 + byte sum of the state, the
-// byte offset (2 * (sb % 3) + (state & 1)) * 16 of the direction entry in
-// the 6-entry table (4 KiB of LDS).  All threads of the block write; the
-// caller syncs.
-constexpr uint32_t kZKeyEntries = 4096;
+// Direction key of the 64-bit state s with bytes b0 .. b7 (b0 lowest): 256
+// = 4 (mod 6) and 4^j = 4 (mod 6) for j >= 1, so s = b0 + 4 (b1 + ... + b7)
+// (mod 6), and two byte dot products give that sum, key = b0 + 4 (b1 + ... +
+// b7) + acc <= 7395 + acc, congruent to s mod 6 (2 VALU: v_dot4_u32_u8 on
+// the high word with weights 4 4 4 4, then on the low word with 4 4 4 1).
+constexpr uint32_t kZKeyMax = 255u * 29u;   // b0 + 4 x 7 bytes
+
+__device__ __forceinline__ uint32_t zdot_key(uint32_t lo, uint32_t hi, uint32_t acc)
+{
+    return __builtin_amdgcn_udot4(lo, 0x04040401u, __builtin_amdgcn_udot4(hi, 0x04040404u, acc, false), false);
+}
+
+// Byte-key table: for each key k <= kZKeyMax the byte offset
+// (2 (k % 3) + (k % 2)) x bytes of the direction entry of s % 6 = k % 6 in
+// the 6-entry table (the entry index 2 (s % 3) + (s % 2) of zdir_make);
+// 7424 bytes of LDS.  All threads of the block write; the caller syncs.
+constexpr uint32_t kZKeyEntries = 7424;   // kZKeyMax + 1 rounded up to 64 bytes
+static_assert(kZKeyEntries > kZKeyMax && kZKeyEntries % 64u == 0u, "key table size");
 
 // bytes: the entry size of the direction table the offsets point into (16,
 // or 32 for the noflux table).
 __device__ __forceinline__ void zbuild_key_table(uint8_t *keys, uint32_t bytes = 16u)
 {
     for (uint32_t e = threadIdx.x; e < kZKeyEntries; e += blockDim.x)
-        keys[e] = (uint8_t)(bytes * (2u * ((e & 2047u) % 3u) + (e >> 11)));
+        keys[e] = (uint8_t)(bytes * (2u * (e % 3u) + (e & 1u)));
 }
 
 // Flag-keyed byte tables (TAB 3; mirror walks with reflection flags): the
-// walker's flags select one of 8 byte-key tables (key + flags x 4096, 32 KiB
-// of LDS), whose bytes point straight at the direction entry for those
-// flags, so the step adds no per-walker table offset (the flags enter as
-// the accumulator of the byte sum, with the parity).  Twelve entries serve
+// walker's flags select one of 8 byte-key tables (key + flags x
+// kZKeyEntries, 58 KiB of LDS), whose bytes point straight at the direction
+// entry for those flags, so the step adds no per-walker table offset (the
+// flags' table offset enters as the accumulator of the key).  Twelve entries serve
 // every flag combination: reflecting an axis flips the dilated addend and
 // the wrap fix-up of its two directions but not their packed displacement
 // (the unbounded move), so entry 2 idx + f holds direction idx with its axis
@@ -121,18 +134,19 @@ __device__ __forceinline__ void zbuild_table_flipped(const ZLayout &zl, uint4 *t
     }
 }
 
-// The byte of key e (flags e >> 12, parity (e >> 11) & 1, byte sum e & 2047)
-// is the offset 16 (2 idx + flip) of its entry in zbuild_table_flipped, with
-// idx = 2 (sb % 3) + parity as in zbuild_key_table.  All threads of the block
-// write 4 keys per word; the caller syncs.
+// The byte of entry e (flags e / kZKeyEntries, key k = e % kZKeyEntries) is
+// the offset 16 (2 idx + flip) of its entry in zbuild_table_flipped, with
+// idx = 2 (k % 3) + (k % 2) as in zbuild_key_table.  All threads of the
+// block write 4 keys per word (a word never straddles two flag tables);
+// the caller syncs.
 __device__ __forceinline__ void zbuild_flag_key_table(uint32_t *keys)
 {
     for (uint32_t w = threadIdx.x; w < kZFlagKeyEntries / 4u; w += blockDim.x) {
         uint32_t v = 0u;
         for (uint32_t b = 0; b < 4u; ++b) {
             const uint32_t e = 4u * w + b;
-            const uint32_t flags = e >> 12, key = e & 4095u;
-            const uint32_t idx = 2u * ((key & 2047u) % 3u) + (key >> 11);
+            const uint32_t flags = e / kZKeyEntries, key = e % kZKeyEntries;
+            const uint32_t idx = 2u * (key % 3u) + (key & 1u);
             const uint32_t ax = (uint32_t)move_axis(zdir_residue(idx));
             v |= (16u * (2u * idx + ((flags >> ax) & 1u))) << (8u * b);
         }
@@ -176,14 +190,10 @@ __device__ __forceinline__ void zdirection(uint64_t &s, const char *tab_bytes, c
     const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
     uint32_t off;
     if (TAB == 3) {
-        // the flags' key table: key offset and parity in one and-or
-        const uint32_t key = __builtin_amdgcn_sad_u8(lo, 0u, __builtin_amdgcn_sad_u8(hi, 0u, ((lo << 11) & 0x800u) | fo));
-        off = keys[key];
+        // the flags' key table: its offset is the key's accumulator
+        off = keys[zdot_key(lo, hi, fo)];
     } else if (TAB == 2) {
-        // key = byte sum + (state & 1) * 2048, the parity entering as the
-        // accumulator of the first byte sum
-        const uint32_t key = __builtin_amdgcn_sad_u8(lo, 0u, __builtin_amdgcn_sad_u8(hi, 0u, (lo << 11) & 0x800u));
-        off = keys[key];
+        off = keys[zdot_key(lo, hi, 0u)];
     } else if (TAB == 1) {
         // byte sum sb <= 2040 == state (mod 3)
         const uint32_t sb = __builtin_amdgcn_sad_u8(hi, 0u, __builtin_amdgcn_sad_u8(lo, 0u, 0u));
@@ -625,6 +635,9 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 constexpr int k = decltype(kc)::value;
                 zdirection<WRAPFIX, TAB, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys, FO[k]);
             });
+            // keep the direction's table reads ahead of the wall test, so
+            // their latency overlaps the chunk load's
+            __builtin_amdgcn_sched_barrier(0);
             // the previous step's hit (its label load overlapped this step's chunk load)
             settle_labels();
             static_for<K>([&](auto kc) {

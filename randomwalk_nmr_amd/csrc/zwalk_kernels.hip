@@ -455,9 +455,9 @@ static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32
     if (a.znoflux)   // noflux on the periodic image: pipelined byte-key walk only
         return a.zcache >= 2 ? launch_z<EMODE, false, 128, 2, true>(a, ls, blocks, st) : hipErrorInvalidValue;
     if (a.zmflags && a.zcache >= 2) {   // mirror, walkers folded to the first octant with reflection flags
-        // CPMG and map walks of blocks of 512 threads or more read the
-        // flag-keyed tables (32 KiB of LDS, still 8 waves per SIMD there)
-        if (EMODE < 2 && ls.threads >= 512)
+        // CPMG and map walks of 1024-thread blocks read the flag-keyed
+        // tables (58 KiB of LDS: two blocks per CU, 8 waves per SIMD)
+        if (EMODE < 2 && ls.threads >= 1024)
             return a.zl.wrapfix ? launch_z<EMODE, true, 128, 3, false, true>(a, ls, blocks, st)
                                 : launch_z<EMODE, false, 128, 3, false, true>(a, ls, blocks, st);
         return a.zl.wrapfix ? launch_z<EMODE, true, 128, 2, false, true>(a, ls, blocks, st)
