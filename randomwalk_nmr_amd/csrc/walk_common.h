@@ -295,6 +295,24 @@ RW_HD uint32_t zdir_residue(uint32_t idx)
     return (0x521430u >> (4u * idx)) & 0xFu;   // idx 0..5 -> 0, 3, 4, 1, 2, 5
 }
 
+// Direction key of a 64-bit state s with bytes b0 .. b7 (b0 lowest): 256 = 4
+// (mod 6) and 4^j = 4 (mod 6) for j >= 1, so s = b0 + 4 (b1 + ... + b7) (mod
+// 6).  The kernels form key = b0 + 4 (b1 + ... + b7) + acc with two byte dot
+// products (v_dot4_u32_u8) of the state's words with these weights; key - acc
+// <= kZKeyMax, and the table index of a key k is zkey_index(k).
+constexpr uint32_t kZKeyWeightsLo = 0x04040401u, kZKeyWeightsHi = 0x04040404u;
+constexpr uint32_t kZKeyMax = 255u * 29u;   // b0 + 4 x 7 bytes
+
+// Byte dot product of v_dot4_u32_u8 (the host restatement the tests use).
+RW_HD uint32_t zdot4(uint32_t a, uint32_t w, uint32_t acc)
+{
+    for (int i = 0; i < 4; ++i)
+        acc += ((a >> (8 * i)) & 0xFFu) * ((w >> (8 * i)) & 0xFFu);
+    return acc;
+}
+
+RW_HD uint32_t zkey_index(uint32_t k) { return 2u * (k % 3u) + (k & 1u); }
+
 RW_HD ZFlip zdir_noflux(const ZLayout &L, uint32_t idx)
 {
     const uint32_t r = zdir_residue(idx);

@@ -83,22 +83,19 @@ __device__ __forceinline__ void zbuild_table_wide(const uint4 *small, uint4 *tab
     }
 }
 
-// Direction key of the 64-bit state s with bytes b0 .. b7 (b0 lowest): 256
-// = 4 (mod 6) and 4^j = 4 (mod 6) for j >= 1, so s = b0 + 4 (b1 + ... + b7)
-// (mod 6), and two byte dot products give that sum, key = b0 + 4 (b1 + ... +
-// b7) + acc <= 7395 + acc, congruent to s mod 6 (2 VALU: v_dot4_u32_u8 on
-// the high word with weights 4 4 4 4, then on the low word with 4 4 4 1).
-constexpr uint32_t kZKeyMax = 255u * 29u;   // b0 + 4 x 7 bytes
-
+// Direction key of the state (walk_common.h: key = b0 + 4 (b1 + ... + b7) +
+// acc, congruent to s + acc mod 6): 2 VALU, v_dot4_u32_u8 on the high word
+// with weights 4 4 4 4, then on the low word with 4 4 4 1.
 __device__ __forceinline__ uint32_t zdot_key(uint32_t lo, uint32_t hi, uint32_t acc)
 {
-    return __builtin_amdgcn_udot4(lo, 0x04040401u, __builtin_amdgcn_udot4(hi, 0x04040404u, acc, false), false);
+    return __builtin_amdgcn_udot4(lo, kZKeyWeightsLo, __builtin_amdgcn_udot4(hi, kZKeyWeightsHi, acc, false),
+                                  false);
 }
 
 // Byte-key table: for each key k <= kZKeyMax the byte offset
-// (2 (k % 3) + (k % 2)) x bytes of the direction entry of s % 6 = k % 6 in
-// the 6-entry table (the entry index 2 (s % 3) + (s % 2) of zdir_make);
-// 7424 bytes of LDS.  All threads of the block write; the caller syncs.
+// zkey_index(k) x bytes of the direction entry of s % 6 = k % 6 in the
+// 6-entry table (the entry index 2 (s % 3) + (s % 2) of zdir_make); 7424
+// bytes of LDS.  All threads of the block write; the caller syncs.
 constexpr uint32_t kZKeyEntries = 7424;   // kZKeyMax + 1 rounded up to 64 bytes
 static_assert(kZKeyEntries > kZKeyMax && kZKeyEntries % 64u == 0u, "key table size");
 
@@ -107,7 +104,7 @@ static_assert(kZKeyEntries > kZKeyMax && kZKeyEntries % 64u == 0u, "key table si
 __device__ __forceinline__ void zbuild_key_table(uint8_t *keys, uint32_t bytes = 16u)
 {
     for (uint32_t e = threadIdx.x; e < kZKeyEntries; e += blockDim.x)
-        keys[e] = (uint8_t)(bytes * (2u * (e % 3u) + (e & 1u)));
+        keys[e] = (uint8_t)(bytes * zkey_index(e));
 }
 
 // Flag-keyed byte tables (TAB 3; mirror walks with reflection flags): the
@@ -146,7 +143,7 @@ __device__ __forceinline__ void zbuild_flag_key_table(uint32_t *keys)
         for (uint32_t b = 0; b < 4u; ++b) {
             const uint32_t e = 4u * w + b;
             const uint32_t flags = e / kZKeyEntries, key = e % kZKeyEntries;
-            const uint32_t idx = 2u * (key % 3u) + (key & 1u);
+            const uint32_t idx = zkey_index(key);
             const uint32_t ax = (uint32_t)move_axis(zdir_residue(idx));
             v |= (16u * (2u * idx + ((flags >> ax) & 1u))) << (8u * b);
         }

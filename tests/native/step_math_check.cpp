@@ -47,6 +47,38 @@ static void check_unfolded()
         EXPECT(zdir_residue(zdir_index_from_sum(zbyte_sum(v), (uint32_t)v)) == v % 6u, "residue small\n");
     for (uint64_t v : {~0ull, 1ull << 63, 0xFFFFFFFFull, 0x8080808080808080ull})
         EXPECT(zdir_residue(zdir_index_from_sum(zbyte_sum(v), (uint32_t)v)) == v % 6u, "residue edge\n");
+    // the dot-product key: congruent to s mod 6, at most kZKeyMax, and its
+    // table index names the entry of residue s % 6
+    auto dot_key = [](uint64_t v) {
+        return zdot4((uint32_t)v, kZKeyWeightsLo, zdot4((uint32_t)(v >> 32), kZKeyWeightsHi, 0u));
+    };
+    s = 0x2545F4914F6CDD1Dull;
+    uint32_t kmax = 0;
+    for (int i = 0; i < 5000000; ++i) {
+        s = xorshift64(s);
+        const uint32_t k = dot_key(s);
+        kmax = k > kmax ? k : kmax;
+        EXPECT(k % 6u == s % 6u, "dot key %llu\n", (unsigned long long)s);
+    }
+    for (uint64_t v = 0; v < 100000; ++v)
+        EXPECT(dot_key(v) % 6u == v % 6u, "dot key small\n");
+    for (uint64_t v : {~0ull, ~0ull - 1, 1ull << 63, 0xFFFFFFFFull, 0xFF00000000000000ull, 0x8080808080808080ull,
+                       0x0101010101010101ull, 0xFFull}) {
+        EXPECT(dot_key(v) % 6u == v % 6u, "dot key edge %llx\n", (unsigned long long)v);
+        EXPECT(dot_key(v) <= kZKeyMax, "dot key range %llx\n", (unsigned long long)v);
+    }
+    EXPECT(dot_key(~0ull) == kZKeyMax, "dot key max\n");
+    EXPECT(kmax <= kZKeyMax, "dot key stream max %u\n", kmax);
+    for (uint32_t k = 0; k <= kZKeyMax; ++k)
+        EXPECT(zdir_residue(zkey_index(k)) == k % 6u, "key index %u\n", k);
+    // zdot4 is the byte dot product
+    for (uint32_t a : {0u, 0xFFFFFFFFu, 0x01020304u, 0x80FF7F01u})
+        for (uint32_t w : {0u, 0xFFFFFFFFu, kZKeyWeightsLo, kZKeyWeightsHi}) {
+            uint32_t ref = 7u;
+            for (int i = 0; i < 4; ++i)
+                ref += ((a >> (8 * i)) & 255u) * ((w >> (8 * i)) & 255u);
+            EXPECT(zdot4(a, w, 7u) == ref, "zdot4\n");
+        }
     // unfold == the reference folds, for any unbounded coordinate
     for (int d = 1; d <= 13; ++d)
         for (int g = -100; g <= 100; ++g) {
