@@ -54,6 +54,63 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+class ClockSampler:
+    """The GPU's GFX clock over a timed region, read from the system
+    management library (amdsmi; no HIP calls) by a thread every `period`
+    seconds; samples taken while the GFX engine is less than 50% busy are
+    dropped.  `result()` is None where amdsmi is missing or fails."""
+
+    def __init__(self, period=0.01):
+        self.period, self.samples, self.error = period, [], None
+        self._stop, self._thread, self._h = None, None, None
+        try:
+            import amdsmi
+            amdsmi.amdsmi_init()
+            self._smi = amdsmi
+            self._h = amdsmi.amdsmi_get_processor_handles()[0]   # the box's one GPU
+        except Exception as e:  # noqa: BLE001 - the clock is a secondary figure
+            self.error = str(e)
+
+    def _run(self):
+        while not self._stop.is_set():
+            try:
+                c = self._smi.amdsmi_get_clock_info(self._h, self._smi.AmdSmiClkType.GFX)
+                a = self._smi.amdsmi_get_gpu_activity(self._h)
+                if isinstance(c.get("clk"), (int, float)) and isinstance(a.get("gfx_activity"), (int, float)) \
+                        and a["gfx_activity"] >= 50:
+                    self.samples.append(float(c["clk"]))
+            except Exception as e:  # noqa: BLE001
+                self.error = str(e)
+                return
+            self._stop.wait(self.period)
+
+    def __enter__(self):
+        if self._h is not None:
+            import threading
+            self._stop = threading.Event()
+            self._thread = threading.Thread(target=self._run, daemon=True)
+            self._thread.start()
+        return self
+
+    def __exit__(self, *exc):
+        if self._thread is not None:
+            self._stop.set()
+            self._thread.join()
+        if self._h is not None:
+            try:
+                self._smi.amdsmi_shut_down()
+            except Exception:  # noqa: BLE001
+                pass
+        return False
+
+    def result(self):
+        if not self.samples:
+            return None
+        v = sorted(self.samples)
+        return {"median_MHz": v[len(v) // 2], "min_MHz": v[0], "max_MHz": v[-1], "samples": len(v),
+                "source": "amdsmi GFX clock, sampled every 10 ms over the timed steps (GFX activity >= 50%)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,13 +212,18 @@ def main():
     ctx.get_timing(reset=True)
     torch.cuda.synchronize()
     ctx.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        one_step()
-        log(f"step {i + 1}/{args.steps} done")
-    torch.cuda.synchronize()
-    ctx.synchronize()
-    t1 = time.perf_counter()
+    sampler = ClockSampler()
+    with sampler:
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            one_step()
+            log(f"step {i + 1}/{args.steps} done")
+        torch.cuda.synchronize()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+    clock = sampler.result()
+    if clock:
+        log(f"GFX clock over the timed steps: median {clock['median_MHz']:.0f} MHz ({clock['samples']} samples)")
     kern_ms, launches = ctx.get_timing(reset=True)
     ctx.set_timing(False)
 
@@ -173,12 +235,15 @@ def main():
     kernel_steps_per_s = steps_per_launch / avg_kernel_s
     # VALU per wave-step, L2 requests and HBM bytes per launch come from the committed
     # rocprofv3 counters of this same command; they apply when the profile was taken with the
-    # same boundary and walkers per lane (the kernel's code; threads per block do not change it)
+    # same kernel code: boundary, walkers per lane, image form, and for mirror walks the
+    # direction-table form (flag-keyed tables with 1024-thread blocks)
     prof = None
     if os.path.exists(COUNTERS_FILE):
         with open(COUNTERS_FILE) as f:
             prof = json.load(f)
-        if prof.get("bc") != args.bc or int(prof.get("walkers_per_lane", 0)) != k or form != 3:
+        prof_tpb = int(str(prof.get("launch", "0,0")).split(",")[0])
+        if prof.get("bc") != args.bc or int(prof.get("walkers_per_lane", 0)) != k or form != 3 or \
+                (args.bc == "mirror" and (prof_tpb >= 1024) != (tpb >= 1024)):
             prof = None
     # gather probe: random 16-byte loads from tables held in L2, in the MALL and in HBM
     probe = {}
@@ -247,7 +312,7 @@ def main():
                                   form, f"mode {form}"),
                               "echoes_between_resorts": resort}},
         "roofline": roofline(args, prof, probe, kernel_steps_per_s, steps_per_launch, avg_kernel_s, launches, k,
-                             tpb),
+                             tpb, clock),
         "cpu_baseline": cpu,
         "checks": {"decay_monotone": decay_ok, "subsample_bit_exact_vs_cpu": parity},
     }
@@ -271,7 +336,7 @@ def walk_kernel_label(bc, k, threads):
             f"{table}, direction drawn a step ahead> on the {form}")
 
 
-def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, launches, k, threads):
+def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, launches, k, threads, clock=None):
     """Roofline of the dominant kernel.  The binding roof is VALU issue: the
     walk keeps each walker's 16-byte image chunk in registers and the L2
     serves nearly every reload, so HBM is far from busy; its ceiling in
@@ -293,20 +358,13 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
         peak = VALU_WAVE_INSTR_PER_S * 64.0 / vps
         out.update(peak=peak, frac=steps_per_s / peak, valu_per_wave_step=vps,
                    valu_busy_measured=prof.get("valu_busy"), counters=os.path.relpath(COUNTERS_FILE, ROOT))
-        if prof.get("shader_cycles_per_launch"):
-            # the same ceiling at the shader clock this run ran at: a launch
-            # takes a fixed number of shader cycles (the committed counter
-            # runs: constant across launches of different durations), so
-            # this run's clock is those cycles over its own launch time;
-            # sustained load runs below the 2.4 GHz peak
-            cyc = float(prof["shader_cycles_per_launch"]) * steps_per_launch / float(prof["walker_steps_per_launch"])
-            f = cyc / avg_kernel_s
+        if clock:
+            # the same ceiling at the GFX clock this run ran at (sustained
+            # load runs below the 2.4 GHz peak), sampled over its timed steps
+            f = clock["median_MHz"] * 1e6
             sec["valu_at_measured_clock"] = {"clock_GHz": f / 1e9, "peak": peak * f / CLOCK_HZ,
-                                             "frac": steps_per_s / (peak * f / CLOCK_HZ),
-                                             "counter_run_clock_GHz": prof.get("sclk_GHz_measured"),
-                                             "basis": "shader cycles per launch (GRBM_COUNT per XCD, committed "
-                                                      "counter run, profiles/walk_kernel_cpmg512.md) / this run's "
-                                                      "average launch time"}
+                                             "frac": steps_per_s / (peak * f / CLOCK_HZ), "clock": clock,
+                                             "counter_run_valu_busy": prof.get("valu_busy")}
         # HBM bytes per launch scale with the launch's walker-steps
         scale = steps_per_launch / float(prof["walker_steps_per_launch"])
         hbm = float(prof["hbm_bytes_per_launch"]) * scale

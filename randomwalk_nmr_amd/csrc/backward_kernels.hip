@@ -166,6 +166,9 @@ __global__ __launch_bounds__(kZBackThreads) void zbackward_kernel(const WalkArgs
         const uint32_t c = zmove_entry<WRAPFIX>(A, T);
         zchunk_fill_buf(zrsrc, c, QB, QV);
         zdirection<WRAPFIX, 2>(s, tab_bytes, fix_bytes, T, keys);
+        // the table reads stay ahead of the wall test, so their latency
+        // overlaps the chunk load's
+        __builtin_amdgcn_sched_barrier(0);
         settle();
         if (__builtin_expect(zchunk_wall<128>(c, QV) != 0u, 0)) {
             lab = zlabel_at(a, c);

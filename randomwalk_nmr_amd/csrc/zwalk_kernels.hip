@@ -211,6 +211,9 @@ __global__ __launch_bounds__(kPfgseThreads, (K > 1 && ROT && MAXSETS <= 2 && !NO
                 constexpr int k = decltype(kc)::value;
                 zdirection<WRAPFIX, 2, NOFLUX>(S[k], tab_bytes, fix_bytes, T[k], keys);
             });
+            // the table reads stay ahead of the wall tests, so their latency
+            // overlaps the chunk loads'
+            __builtin_amdgcn_sched_barrier(0);
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
                 // two short divergent paths instead of selects
