@@ -1312,23 +1312,29 @@ rwnmr_status rwnmr_ctx_autotune(rwnmr_ctx *c, const rwnmr_image *img, const rwnm
         c->unfolded = best_form;
     }
     // 3. echoes per launch of locality-ordered runs (the re-sort interval),
-    // timed over the whole run with its re-sorts
+    // timed over the whole run with its re-sorts; the candidates differ by
+    // about 1%, so each is timed twice, in mirrored order (drifts of the
+    // clock over the phase cancel), and keeps its better time
     uint32_t best_resort = saved_resort;
     if (c->locality && n >= c->locality_min_walkers && trial_echoes >= 4) {
-        double best_whole = 1e300;
-        const uint32_t cands[] = {trial_echoes / 4, trial_echoes / 2, trial_echoes};
-        for (uint32_t r : cands) {
-            c->resort_every = std::max<uint32_t>(1u, r);
+        const uint32_t cands[3] = {trial_echoes / 4, trial_echoes / 2, trial_echoes};
+        double cand_ms[3] = {1e300, 1e300, 1e300};
+        for (int t = 0; t < 6; ++t) {
+            const int ci = t < 3 ? t : 5 - t;
+            c->resort_every = std::max<uint32_t>(1u, cands[ci]);
             double ms = 0.0;
             if ((s = trial(ms, true)) != RWNMR_OK) {
                 restore();
                 return s;
             }
-            if (ms < best_whole) {
-                best_whole = ms;
-                best_resort = c->resort_every;
-            }
+            cand_ms[ci] = std::min(cand_ms[ci], ms);
         }
+        double best_whole = 1e300;
+        for (int ci = 0; ci < 3; ++ci)
+            if (cand_ms[ci] < best_whole) {
+                best_whole = cand_ms[ci];
+                best_resort = std::max<uint32_t>(1u, cands[ci]);
+            }
     }
     c->timing = was_timing;
     c->kernel_ms = saved_ms;
