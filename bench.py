@@ -646,7 +646,7 @@ def run_fit(args):
     import numpy as np
     import torch
     import randomwalk_nmr_amd as R
-    from randomwalk_nmr_amd.fit import CpmgModel, genetic_fit, gradient_fit
+    from randomwalk_nmr_amd.fit import CpmgModel, genetic_fit, gradient_fit, lm_fit
     rng = np.random.default_rng(args.seed)
     voxel, D0, N, E, S = 1.0, 2.5, 1_000_000, 4000, 256
     spe = int(round(0.2 / R.time_step(voxel, D0)))
@@ -679,13 +679,20 @@ def run_fit(args):
     model.decays(pop[:, :-1], pop[:, -1])
     torch.cuda.synchronize()
     batch64_s = time.perf_counter() - t0
-    # one untimed forward + backward, so both fits start with every kernel
-    # loaded (the GA's label-set walk ran above; HIP loads code on first use)
+    # one untimed forward + backward and one untimed Jacobian forward, so
+    # every fit starts with its kernels loaded (the GA's label-set walk ran
+    # above; HIP loads code on first use)
     th = torch.tensor(np.log(np.full(9, 20.0)), requires_grad=True)
     torch.mean(model.decay(torch.exp(th[:-1]), torch.exp(th[-1])).cpu() ** 2).backward()
+    model.decay_and_jacobian(np.full(8, 20.0), 2000.0)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model.decay_and_jacobian(rho_true, t2b_true)
+    torch.cuda.synchronize()
+    jac_s = time.perf_counter() - t0
     g = gradient_fit(model, target, rho0=np.full(8, 20.0), t2b0=2000.0, truth=truth, tol=1e-3, max_evals=60,
                      log=log)
+    lm = lm_fit(model, target, rho0=np.full(8, 20.0), t2b0=2000.0, truth=truth, tol=1e-3, max_evals=40, log=log)
     ga = genetic_fit(model, target, bounds_lo=np.concatenate([np.full(8, 1.0), [500.0]]),
                      bounds_hi=np.concatenate([np.full(8, 60.0), [5000.0]]), truth=truth, tol=1e-3, population=64,
                      time_limit=args.ga_time_limit, log=log)
@@ -698,15 +705,25 @@ def run_fit(args):
         return {"rho_rel_err": rel.tolist(), "rho_rel_err_max": float(rel.max()),
                 "rho_rel_err_mean": float(rel.mean()), "t2b_rel_err": abs(float(t2b) - t2b_true) / t2b_true}
 
-    return {"metric": "rho-fit GPU wall time to 1e-3 RMS decay error (gradient vs GA)",
-            "value": g["seconds"], "unit": "s", "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+    # the line's value: the faster of the two derivative fits that converged
+    fits = [f for f in (lm, g) if f["converged"]]
+    best = min(fits, key=lambda f: f["seconds"]) if fits else g
+    return {"metric": "rho-fit GPU wall time to 1e-3 RMS decay error (least squares with the walk's Jacobian, "
+                      "or gradient, vs GA)",
+            "value": best["seconds"], "unit": "s", "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+            "value_from": "lm" if best is lm else "gradient",
             "config": {"workload": f"{S}^3 sphere pack (r 6-12, porosity {phi:.3f}), 64-site Voronoi labels 0-7, "
                                    f"{N} walkers, {E} echoes x {spe} steps, mirror, noise 0.5%, checkpoint every 500",
-                       "forward_s": fwd_s, "population64_scoring_s": batch64_s,
+                       "forward_s": fwd_s, "forward_jacobian_s": jac_s, "population64_scoring_s": batch64_s,
                        "launch": launch},
+            "lm": {"converged": lm["converged"], "seconds": lm["seconds"], "evals": lm["evals"],
+                   "rms_truth": lm["rms_truth"], "rho": np.asarray(lm["rho"]).tolist(), "t2b": lm["t2b"],
+                   "recovery": recovery(lm["rho"], lm["t2b"]),
+                   "method": "Levenberg-Marquardt on (log rho_l, log T2b), Jacobian from one walk per evaluation"},
             "gradient": {"converged": g["converged"], "seconds": g["seconds"], "evals": g["evals"],
                          "rms_truth": g["rms_truth"], "rho": np.asarray(g["rho"]).tolist(), "t2b": g["t2b"],
-                         "recovery": recovery(g["rho"], g["t2b"])},
+                         "recovery": recovery(g["rho"], g["t2b"]),
+                         "method": "L-BFGS, gradient by checkpoint replay (forward + backward per evaluation)"},
             "ga": {"converged": ga["converged"], "seconds": ga["seconds"], "generations": ga["generations"],
                    "evals": ga["evals"], "resets": ga["resets"], "rms_truth": ga["rms_truth"],
                    "time_limit_s": args.ga_time_limit, "genotype": np.asarray(ga["genotype"]).tolist(),
@@ -715,7 +732,8 @@ def run_fit(args):
             "identifiability": "a decay within 1e-3 RMS does not determine 8 per-label relaxivities: different "
                                "(rho_l, T2b) give decays that agree to the noise level, so recovery errors are "
                                "reported beside the decay error",
-            "ga_over_gradient": (ga["seconds"] / g["seconds"]) if (g["converged"] and ga["converged"]) else None}
+            "ga_over_gradient": (ga["seconds"] / g["seconds"]) if (g["converged"] and ga["converged"]) else None,
+            "ga_over_lm": (ga["seconds"] / lm["seconds"]) if (lm["converged"] and ga["converged"]) else None}
 
 
 def wl_fit(args):

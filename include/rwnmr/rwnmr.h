@@ -308,6 +308,20 @@ rwnmr_status rwnmr_cpmg_backward(rwnmr_ctx *ctx, const rwnmr_image *img, const v
                                  uint32_t echoes, uint32_t steps_per_echo, int32_t bc, const double *label_delta,
                                  uint32_t every, const double *grad_echo, double *grad_delta);
 
+/* rwnmr_cpmg_forward_jacobian (no reference counterpart; for least-squares
+ * fits of the per-label relaxivities): the labelled CPMG run of
+ * rwnmr_cpmg_run_labels (shift 0) that also returns, in one pass, the
+ * Jacobian of the echo sums with respect to the log deltas,
+ * jac[l][n] = dM(n)/d log delta_l = sum_w h_wl(n) E_w(n) for l < n_labels
+ * and n = 0..echoes ([n_labels][echoes+1], host or device like the
+ * walkers).  label_hits (nullable, [n_labels][n] uint32, in/out) are the
+ * hits before the run, counted in h (NULL: zero).  Needs the unfolded
+ * labelled image (the default); one walker per lane with the context's
+ * threads per block. */
+rwnmr_status rwnmr_cpmg_forward_jacobian(rwnmr_ctx *ctx, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                         uint32_t steps_per_echo, int32_t bc, const double *label_delta,
+                                         uint32_t *label_hits, double *echo_sums, double *jac);
+
 /* PFGSE, reference GPU semantics (NMR_PFGSE::simulation_cuda,
  * B/NMR_Simulation/NMR_pfgse_cuda.cpp:536-1348): for each diffusion time i
  * in order, walk steps_inc[i] more steps (PFG_map_*, :34-339), multiply the

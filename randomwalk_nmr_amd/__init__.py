@@ -5,7 +5,7 @@ behind the C ABI in include/rwnmr/); this package is the Python host mirror.
 """
 from ._lib import RwnmrError, load  # noqa: F401
 from .core import (BC, Context, Image, bc_code, bitblock_dims, collision_histogram, cpmg_batch_run,  # noqa: F401
-                   CpmgGraph, cpmg_run_label_sets,
+                   CpmgGraph, cpmg_run_label_sets, cpmg_forward_jacobian,
                    cpmg_field_run, cpmg_histogram_simulate, cpmg_run, histogram_decay, histogram_list,
                    histogram_penalties, map_run,  # noqa: F401
                    pack_bitblocks, penalty, pfgse_run, pfgse_run_sampled, pfgse_sample_stats, seed_and_place,

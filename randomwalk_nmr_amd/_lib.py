@@ -148,6 +148,7 @@ SIGNATURES = {
                               U32, P, P, P]),
     "rwnmr_cpmg_batch_run": (I32, [P, ct.POINTER(CpmgInstance), U32, U32]),
     "rwnmr_cpmg_forward_ckpt": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, P, U32, P, P]),
+    "rwnmr_cpmg_forward_jacobian": (I32, [P, P, ct.POINTER(Walkers), U32, U32, I32, P, P, P, P]),
     "rwnmr_cpmg_backward": (I32, [P, P, P, U64, U32, U32, I32, P, U32, P, P]),
     "rwnmr_ckpt_bytes": (U64, [U64, U32, U32]),
     "rwnmr_probe_gather": (I32, [P, U64, U32, ct.POINTER(D), ct.POINTER(D)]),

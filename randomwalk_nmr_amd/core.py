@@ -348,6 +348,40 @@ def cpmg_run(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo,
     return dict(x=x, y=y, z=z, seed=seed, energy=energy, hits=hits, echo_sums=es, label_hits=lh)
 
 
+def cpmg_forward_jacobian(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo, label_delta,
+                          bc="mirror", energy=None, label_hits=None, echo_sums=None, jac=None, sync=True):
+    """Per-label CPMG that also returns the Jacobian of the echo sums with
+    respect to the log deltas (rwnmr_cpmg_forward_jacobian):
+    jac[l, n] = dM(n)/d log delta_l = sum_w h_wl(n) E_w(n).
+    Host inputs are copied and a dict (x, y, z, seed, energy, label_hits,
+    echo_sums [echoes+1], jac [n_labels, echoes+1]) returned; device inputs
+    are updated in place, echo_sums and jac must then be device tensors of
+    echoes+1 and n_labels * (echoes+1) doubles."""
+    dev = _is_dev(x)
+    n = len(x)
+    L = load()
+    if dev:
+        _torch_sync()
+        w = _walker_struct(x, y, z, seed, energy, None, None, 1.0, n, 1)
+        check(L.rwnmr_cpmg_forward_jacobian(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo,
+                                            bc_code(bc), _ptr(label_delta), _ptr(label_hits), _ptr(echo_sums),
+                                            _ptr(jac)), ctx.handle)
+        if sync:
+            ctx.synchronize()
+        return dict(x=x, y=y, z=z, seed=seed, energy=energy, label_hits=label_hits, echo_sums=echo_sums, jac=jac)
+    x, y, z = _host(x, np.int32), _host(y, np.int32), _host(z, np.int32)
+    seed = _host(seed, np.uint64)
+    energy = np.ones(n) if energy is None else _host(energy, np.float64, n)
+    lh = None if label_hits is None else _host(label_hits, np.uint32).reshape(-1, n)
+    ld = _host(label_delta, np.float64)
+    es = np.zeros(echoes + 1)
+    jc = np.zeros((img.n_labels, echoes + 1))
+    w = _walker_struct(x, y, z, seed, energy, None, None, 1.0, n, 0)
+    check(L.rwnmr_cpmg_forward_jacobian(ctx.handle, img.handle, ct.byref(w), echoes, steps_per_echo, bc_code(bc),
+                                        _ptr(ld), _ptr(lh), _ptr(es), _ptr(jc)), ctx.handle)
+    return dict(x=x, y=y, z=z, seed=seed, energy=energy, label_hits=lh, echo_sums=es, jac=jc)
+
+
 def cpmg_run_label_sets(ctx: Context, img: Image, x, y, z, seed, *, echoes, steps_per_echo, set_label_delta,
                         bc="mirror", energy=None, echo_sums=None, sync=True):
     """Several per-label relaxivity sets on one trajectory

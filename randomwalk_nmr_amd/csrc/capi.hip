@@ -113,6 +113,7 @@ struct rwnmr_ctx {
     DevBuf x, y, z, seed, energy, hits, penalty, x0, y0, z0;
     // per-call small arrays and reductions
     DevBuf partials, sums, label_delta, label_hits, steps_inc, kvec, set_delta;
+    DevBuf jac_partials, jac_rows;   // Jacobian forward: per-label partials and host-call rows
     // internal-field CPMG staging (host-pointer calls)
     DevBuf field, p_alive, phase;
     DevBuf msamples;   // PFGSE per-sample sums (host-pointer calls)
@@ -658,11 +659,13 @@ struct FieldRun {
 };
 
 // Shared CPMG driver: emode 1 (per-walker delta), 2 (per-label delta) or 3
-// (internal field, `fr`).
+// (internal field, `fr`).  jac (emode 2, nullable): also the Jacobian rows
+// jac[l][0..echoes] = sum_w h_wl(n) E_w(n) for l < n_labels (unfolded image,
+// one walker per lane, per-label hit counts kept across launches).
 rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
                          uint32_t steps_per_echo, int32_t bc, uint32_t shift, int emode,
                          const double *label_delta, uint32_t *label_hits, double *echo_sums,
-                         const FieldRun *fr = nullptr)
+                         const FieldRun *fr = nullptr, double *jac = nullptr)
 {
     rwnmr_status s;
     if (!c)
@@ -679,6 +682,8 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         if (shift != 0 && label_hits)
             return fail(c, RWNMR_ERR_INVALID, "per-label hit counts need shift == 0");
     }
+    if (jac && (emode != 2 || shift != 0 || !echo_sums))
+        return fail(c, RWNMR_ERR_INVALID, "Jacobian forward: per-label run with echo sums and shift == 0");
     WalkerView v;
     if ((s = clear_flag(c)) != RWNMR_OK)
         return s;
@@ -739,9 +744,23 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
                 a.label_hits = c->label_hits.as<uint32_t>();
             }
         }
+        if (jac && !a.label_hits) {
+            // the Jacobian needs each walker's hits since the start, kept
+            // across launches: start them at zero in context memory
+            const uint64_t cnt = (uint64_t)img->n_labels * w->n;
+            RW_HIP(c, c->label_hits.reserve(4 * cnt + 4));
+            a.label_hits = c->label_hits.as<uint32_t>();
+            hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, c->stream,
+                               a.label_hits, 0u, cnt);
+            RW_HIP(c, hipGetLastError());
+        }
     }
     attach_zimage(c, img, bc, emode, a, true, true);
-    const uint32_t blocks = rwnmr::walk_grid_blocks(w->n, c->shape, (int)shift);
+    if (jac && (!a.zwords || !a.zlabels))
+        return fail(c, RWNMR_ERR_INVALID, "Jacobian forward: needs the unfolded labelled image");
+    // the Jacobian forward holds one walker per lane
+    const LaunchShape shape = jac ? LaunchShape{c->shape.threads, 1u} : c->shape;
+    const uint32_t blocks = rwnmr::walk_grid_blocks(w->n, shape, (int)shift);
     a.partial_stride = blocks;
     double *sums = nullptr;
     if (echo_sums) {
@@ -755,11 +774,24 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
     // echoes per launch so that the partial buffer stays bounded
     uint64_t chunk = echoes ? echoes : 1;
     if (echo_sums && blocks) {
-        const uint64_t max_rows = std::max<uint64_t>(2, kMaxPartialBytes / (8ull * blocks));
+        // the Jacobian forward stages kMaxLabels more rows per echo
+        const uint64_t per_row = 8ull * blocks * (jac ? 1 + rwnmr::kMaxLabels : 1);
+        const uint64_t max_rows = std::max<uint64_t>(2, kMaxPartialBytes / per_row);
         chunk = std::min<uint64_t>(chunk, max_rows - 1);
     }
     if (echo_sums)
         RW_HIP(c, c->partials.reserve(sizeof(double) * (size_t)blocks * (size_t)(chunk + 1) + 8));
+    double *jrows = nullptr;
+    if (jac) {
+        RW_HIP(c, c->jac_partials.reserve(sizeof(double) * rwnmr::kMaxLabels * (size_t)blocks * (size_t)(chunk + 1) +
+                                          8));
+        if (w->on_device) {
+            jrows = jac;
+        } else {
+            RW_HIP(c, c->jac_rows.reserve(sizeof(double) * (size_t)img->n_labels * ((size_t)echoes + 1)));
+            jrows = c->jac_rows.as<double>();
+        }
+    }
     const bool local = emode < 2 && shift == 0 && c->locality && w->n >= c->locality_min_walkers;
     if (local) {
         a.steps_per_echo = steps_per_echo;
@@ -788,14 +820,22 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         const uint32_t ne = (uint32_t)std::min<uint64_t>(chunk, (uint64_t)echoes - done);
         a.echoes = ne;
         a.partials = echo_sums ? c->partials.as<double>() : nullptr;
+        a.jac_partials = jac ? c->jac_partials.as<double>() : nullptr;
         hipEvent_t t0 = timing_begin(c);
-        RW_HIP(c, launch_any_walk(a, bc, emode, c->shape, c->stream));
+        if (jac)
+            RW_HIP(c, rwnmr::launch_zwalk_jac(a, shape.threads, c->stream));
+        else
+            RW_HIP(c, launch_any_walk(a, bc, emode, c->shape, c->stream));
         timing_end(c, t0);
         if (echo_sums) {
             // rows 0..ne of this launch are echoes done..done+ne (row 0 repeats
             // the previous launch's last echo with identical bits)
             RW_HIP(c, rwnmr::launch_reduce_rows(a.partials, (uint64_t)ne + 1, blocks, sums + done, c->stream));
         }
+        if (jac)
+            for (int l = 0; l < img->n_labels; ++l)
+                RW_HIP(c, rwnmr::launch_reduce_rows(a.jac_partials + (uint64_t)l * (ne + 1) * blocks, (uint64_t)ne + 1,
+                                                    blocks, jrows + (uint64_t)l * (echoes + 1) + done, c->stream));
         done += ne;
     } while (done < echoes);
     }
@@ -803,6 +843,9 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         if (echo_sums)
             RW_HIP(c, hipMemcpyAsync(echo_sums, sums, sizeof(double) * ((size_t)echoes + 1), hipMemcpyDeviceToHost,
                                      c->stream));
+        if (jac)
+            RW_HIP(c, hipMemcpyAsync(jac, jrows, sizeof(double) * (size_t)img->n_labels * ((size_t)echoes + 1),
+                                     hipMemcpyDeviceToHost, c->stream));
         if (emode == 2 && label_hits)
             RW_HIP(c, hipMemcpyAsync(label_hits, a.label_hits, sizeof(uint32_t) * (size_t)img->n_labels * w->n,
                                      hipMemcpyDeviceToHost, c->stream));
@@ -883,7 +926,8 @@ void rwnmr_ctx_destroy(rwnmr_ctx *c)
                       &c->partials, &c->sums, &c->label_delta, &c->label_hits, &c->steps_inc, &c->kvec,
                       &c->set_delta, &c->flag, &c->batch_inst, &c->batch_tiles, &c->batch_rows,
                       &c->batch_counter, &c->lkeys, &c->lidx, &c->lsort_tmp, &c->field, &c->p_alive, &c->phase, &c->msamples,
-                      &c->lset_x, &c->lset_y, &c->lset_z, &c->lset_seed, &c->lset_energy, &c->lset_energies})
+                      &c->lset_x, &c->lset_y, &c->lset_z, &c->lset_seed, &c->lset_energy, &c->lset_energies,
+                      &c->jac_partials, &c->jac_rows})
         b->release();
     for (int i = 0; i < 8; ++i) {
         c->sortA[i].release();
@@ -1603,6 +1647,17 @@ rwnmr_status rwnmr_cpmg_run_labels(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_w
                                    uint32_t *label_hits, double *echo_sums)
 {
     return cpmg_common(c, img, w, echoes, steps_per_echo, bc, shift, 2, label_delta, label_hits, echo_sums);
+}
+
+rwnmr_status rwnmr_cpmg_forward_jacobian(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,
+                                         uint32_t steps_per_echo, int32_t bc, const double *label_delta,
+                                         uint32_t *label_hits, double *echo_sums, double *jac)
+{
+    if (!c)
+        return RWNMR_ERR_INVALID;
+    if (!jac || !echo_sums)
+        return fail(c, RWNMR_ERR_INVALID, "forward_jacobian: null echo_sums or jac");
+    return cpmg_common(c, img, w, echoes, steps_per_echo, bc, 0, 2, label_delta, label_hits, echo_sums, nullptr, jac);
 }
 
 rwnmr_status rwnmr_cpmg_run_label_sets(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w, uint32_t echoes,

@@ -51,6 +51,8 @@ struct WalkArgs {
     uint32_t zlf_shift, zlf_mul, zlf_shift2, zlf_mask;   // zoctant constants (zoctant_constants)
     uint32_t n_label_sets;           // label-set walks: label_delta holds [n_label_sets][n_labels]
     double *set_energy;              // label-set walks: nullable [8][n] energies carried between launches
+    double *jac_partials;            // Jacobian forward: [kMaxLabels][(echoes + 1)][partial_stride] block sums of
+                                     // h_l E (hits on label l up to the echo times the energy)
     // internal-field CPMG (emode 3): field[x + y W + z W H], phase step gammatau,
     // per-walker surviving fraction and phase (in/out)
     const double *field;

@@ -446,14 +446,22 @@ __device__ __forceinline__ uint32_t zlabel_at(const WalkArgs &a, uint32_t c)
 // Echo partials of set q go to part[q * (a.echoes + 1) * part_stride + row
 // * part_stride] (lds8: the staging of stage_echo_sets8); set 0's energy is
 // stored.
+// JAC (EMODE 2, K == 1, NS == 1): the Jacobian forward; besides the echo
+// sums, each echo's block sums of h_l E for the 8 labels (the walker's hits
+// on label l up to the echo, from its LDS row, times its energy: dE/d log
+// delta_l, since E is the sequential product of the hit labels' deltas) go
+// to a.jac_partials + tile, row stride part_stride, label stride (a.echoes +
+// 1) part_stride, through the 8-value reduce-scatter (lds8: the staging of
+// stage_echo_sets8).
 template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false, bool MFLAGS = false,
-          int NS = 1>
+          int NS = 1, bool JAC = false>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
                                            const double *ldelta = nullptr, const uint8_t *keys = nullptr,
                                            uint32_t *lh = nullptr, double *lds8 = nullptr)
 {
     constexpr bool WIDE = TAB == 1;
+    static_assert(!JAC || (EMODE == 2 && K == 1 && NS == 1 && CACHE >= 64), "Jacobian forward: labelled, K == 1");
     static_assert(!NOFLUX || CACHE >= 64, "noflux: pipelined chunk-cache walk only");
     static_assert(TAB != 3 || (MFLAGS && CACHE >= 64), "flag-keyed tables: pipelined mirror walk only");
     static_assert(NS == 1 || (NS == 8 && EMODE == 2 && K == 1 && CACHE >= 64), "label sets: 8 sets, K == 1");
@@ -728,6 +736,28 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             if (part)
                 stage_echo_sets8(lds8, echo, echoes, ES, part, part_stride, set_stride);
         }
+    } else if constexpr (JAC) {
+        // the labels' h_l E at echo 0 and after each echo (the pending hit
+        // settled), staged beside the grouped echo sums
+        double *jpart = part ? a.jac_partials + tile : nullptr;
+        uint32_t jecho = 0;
+        auto stage_jac = [&]() {
+            double v[8];
+#pragma unroll
+            for (int l = 0; l < 8; ++l)
+                v[l] = (double)my_lh[l] * E[0];
+            stage_echo_sets8(lds8, jecho, echoes, v, jpart, part_stride, set_stride);
+        };
+        static_assert(kMaxLabels == 8, "one reduce-scatter lane group per label");
+        if (jpart)
+            stage_jac();
+        auto walk_echo_jac = [&]() {
+            walk_echo();
+            ++jecho;
+            if (jpart)
+                stage_jac();
+        };
+        run_echoes<true>(echoes, lds, part, part_stride, walk_echo_jac, echo_value);
     } else {
         // one walker per lane: echoes reduced in groups of 4; several walkers
         // per lane stage each echo (the group buffer would push those

@@ -82,5 +82,10 @@ hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hip
 // a.label_delta; a.zlabels needed; one walker per lane, `threads` per
 // block); partials [set][echo rows][a.partial_stride] of 8 sets.
 hipError_t launch_zwalk_sets(const WalkArgs &a, uint32_t threads, hipStream_t st);
+// Jacobian forward (per-label CPMG that also returns the block sums of h_l E
+// per label and echo in a.jac_partials): a.zlabels, a.label_delta,
+// a.partials and a.jac_partials needed; one walker per lane, `threads` per
+// block.
+hipError_t launch_zwalk_jac(const WalkArgs &a, uint32_t threads, hipStream_t st);
 
 }  // namespace rwnmr

@@ -105,6 +105,39 @@ __global__ __launch_bounds__(1024) void zwalk_sets_kernel(const WalkArgs a)
                                                          lds8);
 }
 
+// Jacobian forward: the per-label CPMG walk (zwalk_tile EMODE 2, one walker
+// per lane, per-label hit counts in LDS) that also sums h_l E per label and
+// echo (zwalk_tile JAC).  Dynamic LDS: the 8-label echo staging (kEchoChunk
+// x 8 x waves doubles), then the hit-count rows (blockDim x kLhStride).
+template <bool WRAPFIX, bool NOFLUX, bool MFLAGS>
+__global__ __launch_bounds__(1024) void zwalk_jac_kernel(const WalkArgs a)
+{
+    extern __shared__ double zjdyn[];
+    __shared__ double lds[kEchoChunk][kEchoCols];
+    __shared__ uint4 tab[MFLAGS ? 8 * kZFlagStride / 16 : (NOFLUX ? 12 : 6)];
+    __shared__ uint2 fix[MFLAGS ? 8 * kZFlagStride / 16 : 6];
+    __shared__ uint8_t keys[kZKeyEntries];
+    __shared__ double ldelta[kMaxLabels];
+    const uint64_t tile = tile_of_block(a);
+    if (a.tile_map && tile >= a.n_tiles)
+        return;
+    double *lds8 = zjdyn;
+    uint32_t *zlh = reinterpret_cast<uint32_t *>(zjdyn + (size_t)kEchoChunk * 8 * (blockDim.x / kWave));
+    if (NOFLUX)
+        zbuild_table_noflux(a.zl, tab);
+    else if (MFLAGS)
+        zbuild_table_flags(a.zl, tab, fix);
+    else
+        zbuild_table(a.zl, tab, fix);
+    zbuild_key_table(keys, NOFLUX ? 32u : 16u);
+    if (threadIdx.x < kMaxLabels)
+        ldelta[threadIdx.x] = (int)threadIdx.x < a.n_labels ? a.label_delta[threadIdx.x] : 1.0;
+    __syncthreads();
+    zwalk_tile<2, 1, WRAPFIX, 128, 2, NOFLUX, MFLAGS, 1, true>(a, tile, a.partials ? a.partials + tile : nullptr,
+                                                               a.partial_stride, lds, tab, fix, ldelta, keys, zlh,
+                                                               lds8);
+}
+
 // PFGSE on the unfolded image: the walk between diffusion times counts hits
 // (register chunk cache, packed displacement flushed every kZFlush steps and
 // at every diffusion time), then pow(delta, hits) per energy set and the
@@ -509,6 +542,34 @@ hipError_t launch_zwalk_sets(const WalkArgs &a, uint32_t threads, hipStream_t st
         hipLaunchKernelGGL((zwalk_sets_kernel<true, false, false>), dim3(blocks), dim3(threads), lds8_bytes, st, a);
     } else {
         hipLaunchKernelGGL((zwalk_sets_kernel<false, false, false>), dim3(blocks), dim3(threads), lds8_bytes, st, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_zwalk_jac(const WalkArgs &a, uint32_t threads, hipStream_t st)
+{
+    if (a.n == 0)
+        return hipSuccess;
+    if (a.shift != 0 || !a.zwords || !a.zlabels || !a.label_delta || !a.partials || !a.jac_partials ||
+        a.n_labels <= 0 || a.n_labels > kMaxLabels || threads == 0 || threads > 1024 || threads % 64 != 0)
+        return hipErrorInvalidValue;
+    const uint64_t tiles = (a.n + threads - 1) / threads;
+    if (tiles > 0x7FFFFFF0ull || (a.tile_map && a.n_tiles != tiles))
+        return hipErrorInvalidValue;
+    const uint32_t blocks = a.tile_map ? (uint32_t)(((tiles + kXcds - 1) / kXcds) * kXcds) : (uint32_t)tiles;
+    const size_t dyn = (size_t)kEchoChunk * 8 * (threads / kWave) * sizeof(double) +
+                       (size_t)kLhStride * threads * sizeof(uint32_t);
+    if (a.znoflux) {
+        hipLaunchKernelGGL((zwalk_jac_kernel<false, true, false>), dim3(blocks), dim3(threads), dyn, st, a);
+    } else if (a.zmflags) {
+        if (a.zl.wrapfix)
+            hipLaunchKernelGGL((zwalk_jac_kernel<true, false, true>), dim3(blocks), dim3(threads), dyn, st, a);
+        else
+            hipLaunchKernelGGL((zwalk_jac_kernel<false, false, true>), dim3(blocks), dim3(threads), dyn, st, a);
+    } else if (a.zl.wrapfix) {
+        hipLaunchKernelGGL((zwalk_jac_kernel<true, false, false>), dim3(blocks), dim3(threads), dyn, st, a);
+    } else {
+        hipLaunchKernelGGL((zwalk_jac_kernel<false, false, false>), dim3(blocks), dim3(threads), dyn, st, a);
     }
     return hipGetLastError();
 }

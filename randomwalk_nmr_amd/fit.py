@@ -130,6 +130,44 @@ class CpmgModel:
         """S(t_n) as a differentiable torch function of rho[n_labels] and T2b."""
         return CpmgDecay.apply(rho, t2b, self)
 
+    def forward_jacobian(self, delta):
+        """Echo sums M(0..E) and J[l, n] = dM(n)/d log delta_l (device
+        float64, [E+1] and [n_labels, E+1]) from the start state, in one
+        walk (rwnmr_cpmg_forward_jacobian); M equals forward_sums(delta)."""
+        torch = self.torch
+        if not hasattr(self, "jac"):
+            self.jac = torch.zeros((self.n_labels, self.echoes + 1), dtype=torch.float64, device=self.dev)
+        self.delta_dev.copy_(torch.as_tensor(delta, dtype=torch.float64))
+        self.x.copy_(self.x0)
+        self.y.copy_(self.y0)
+        self.z.copy_(self.z0)
+        self.s.copy_(self.s0)
+        self.e.fill_(1.0)
+        torch.cuda.current_stream().synchronize()
+        w = core._walker_struct(self.x, self.y, self.z, self.s, self.e, None, None, 1.0, self.n, 1)
+        check(load().rwnmr_cpmg_forward_jacobian(self.ctx.handle, self.img.handle, ct.byref(w), self.echoes,
+                                                 self.spe, self.bc, core._ptr(self.delta_dev), None,
+                                                 core._ptr(self.sums), core._ptr(self.jac)), self.ctx.handle)
+        self.ctx.synchronize()
+        self.forwards += 1
+        return self.sums.clone(), self.jac.clone()
+
+    def decay_and_jacobian(self, rho, t2b):
+        """S(t_n) = exp(-t_n/T2b) M(n)/M(0) and its Jacobian with respect to
+        theta = (log rho_1 .. log rho_L, log T2b), numpy [E+1] and
+        [E+1, L+1]: dS/d log rho_l = exp(-t/T2b) J_l / M(0) x
+        (d delta/d rho) rho_l / delta_l (M(0), the start energies, does not
+        depend on rho), dS/d log T2b = S t / T2b."""
+        torch = self.torch
+        rho = np.asarray(rho, float)
+        delta = self.delta_of_rho(rho)
+        M, J = self.forward_jacobian(delta)
+        bulk = torch.exp(-self.times / float(t2b))
+        S = bulk * M / M[0]
+        dlog = torch.as_tensor(self.ddelta_drho * rho / delta, dtype=torch.float64, device=self.dev)
+        jac = torch.cat([(bulk / M[0])[:, None] * J.T * dlog[None, :], (S * self.times / float(t2b))[:, None]], 1)
+        return S.cpu().numpy(), jac.cpu().numpy()
+
     def population_sums(self, deltas):
         """Echo sums [P, E+1] (device float64) of P rows of label deltas, all
         on the trajectories of the start state (rwnmr_cpmg_run_label_sets:
@@ -266,6 +304,64 @@ def gradient_fit(model: CpmgModel, target, *, rho0, t2b0, truth=None, tol=1e-3, 
     best = state["best"] or min(hist, key=lambda h: h["rms_truth"])
     return dict(converged=state["done"], evals=len(hist), seconds=best["t"] if state["done"] else hist[-1]["t"],
                 rho=best["rho"], t2b=best["t2b"], rms_truth=best["rms_truth"], history=hist)
+
+
+def lm_fit(model: CpmgModel, target, *, rho0, t2b0, truth=None, tol=1e-3, max_evals=40,
+           rho_bounds=(0.5, 200.0), t2b_bounds=(50.0, 1e5), log=None):
+    """Levenberg-Marquardt on theta = (log rho_l, log T2b) minimising the
+    squared decay misfit to `target`, with the exact Jacobian from one GPU
+    walk per evaluation (CpmgModel.decay_and_jacobian).  The damping mu
+    follows the gain ratio (Nielsen's update: on a successful step mu *=
+    max(1/3, 1 - (2 r - 1)^3), else mu *= nu, nu *= 2), starting from 1e-3
+    max diag(J^T J).  Steps are clipped to the bounds.  Stops when the RMS
+    error against `truth` (the noise-free decay, if given) falls below tol
+    at an accepted step.  Returns the dict of gradient_fit."""
+    target = np.asarray(target, float)
+    truth_np = None if truth is None else np.asarray(truth, float)
+    lo = np.concatenate([np.full(len(rho0), np.log(rho_bounds[0])), [np.log(t2b_bounds[0])]])
+    hi = np.concatenate([np.full(len(rho0), np.log(rho_bounds[1])), [np.log(t2b_bounds[1])]])
+    th = np.clip(np.concatenate([np.log(rho0), [np.log(t2b0)]]), lo, hi)
+    hist = []
+    t0 = time.perf_counter()
+
+    def evaluate(theta):
+        S, J = model.decay_and_jacobian(np.exp(theta[:-1]), float(np.exp(theta[-1])))
+        r = S - target
+        err = rms(S, truth_np) if truth_np is not None else float(np.sqrt(np.mean(r ** 2)))
+        hist.append(dict(t=time.perf_counter() - t0, loss=float(np.mean(r ** 2)), rms_truth=err,
+                         rho=np.exp(theta[:-1]).copy(), t2b=float(np.exp(theta[-1]))))
+        if log:
+            log(f"lm eval {len(hist)} at {hist[-1]['t']:.3f} s: rms vs truth {err:.3e} loss {hist[-1]['loss']:.3e}")
+        return r, J, err
+
+    r, J, err = evaluate(th)
+    best = hist[-1] if err <= tol else None
+    A = J.T @ J
+    mu, nu = 1e-3 * float(np.max(np.diag(A))), 2.0
+    while best is None and len(hist) < max_evals:
+        A = J.T @ J
+        g = J.T @ r
+        h = np.linalg.solve(A + mu * np.eye(len(th)), -g)
+        th_new = np.clip(th + h, lo, hi)
+        h = th_new - th
+        r_new, J_new, err_new = evaluate(th_new)
+        F, F_new = 0.5 * float(r @ r), 0.5 * float(r_new @ r_new)
+        pred = 0.5 * float(h @ (mu * h - g))
+        gain = (F - F_new) / pred if pred > 0 else -1.0
+        if gain > 0:
+            th, r, J = th_new, r_new, J_new
+            mu *= max(1.0 / 3.0, 1.0 - (2.0 * gain - 1.0) ** 3)
+            nu = 2.0
+            if err_new <= tol:
+                best = hist[-1]
+        else:
+            mu *= nu
+            nu *= 2.0
+    done = best is not None
+    if best is None:
+        best = min(hist, key=lambda e: e["rms_truth"])
+    return dict(converged=done, evals=len(hist), seconds=best["t"] if done else hist[-1]["t"], rho=best["rho"],
+                t2b=best["t2b"], rms_truth=best["rms_truth"], history=hist)
 
 
 def ga_offspring_count(population: int, offspring_proportion: float) -> int:
