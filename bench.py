@@ -13,7 +13,8 @@ Beside the timed runs the line carries the walk kernel's roofline (VALU
 issue, the binding roof, with HBM, L2-request and random-gather-probe
 figures), the CPU reference loop timed on the host cores, and under "fit"
 the metric's second half: the relaxivity fit's GPU wall time to 1e-3 RMS
-decay error, gradient fit against the genetic algorithm (--no-fit skips it).
+decay error, least-squares and gradient fits against the genetic algorithm
+(--no-fit skips it).
 
 --workload selects the other measured configurations (one JSON line each):
   small   128^3 CPMG, 10k walkers, 2000 echoes, plus the 128-bin Laplace
@@ -23,8 +24,9 @@ decay error, gradient fit against the genetic algorithm (--no-fit skips it).
   batch   64 uneven CPMG instances (64^3..256^3, 10k..500k walkers, 4000
           echoes) in one persistent launch vs 64 sequential launches;
   fit     8 per-label relaxivities + T2bulk on a 256^3 Voronoi-labelled pack,
-          1M walkers, 4000 echoes: gradient fit (backward kernel) vs the
-          genetic algorithm, GPU wall time to 1e-3 RMS decay error.
+          1M walkers, 4000 echoes: least-squares fit (Jacobian forward)
+          and gradient fit (backward kernel) vs the genetic algorithm, GPU
+          wall time to 1e-3 RMS decay error.
 """
 from __future__ import annotations
 
@@ -639,7 +641,8 @@ def run_fit(args):
     """256^3 sphere pack (r 6-12, porosity 0.25), solid labelled 0-7 by a
     64-site Voronoi partition, rho_l U(5, 40), T2b 2800 ms, 1M walkers, 4000
     echoes x 3 steps (voxel 1 um); target = forward(truth) + 0.5% Gaussian
-    noise; gradient fit (checkpoint every 500 echoes) vs the genetic
+    noise; least-squares fit (Levenberg-Marquardt with the walk's
+    Jacobian) and gradient fit (checkpoint every 500 echoes) vs the genetic
     algorithm (population 64, each batch of individuals scored on one
     trajectory), GPU wall time until the fitted decay is within 1e-3 RMS of
     the noise-free decay.  Returns the result dict."""
