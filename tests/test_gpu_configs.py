@@ -223,6 +223,13 @@ def test_labelled_fit_256_1m_walkers_4000_echoes(ctx):
     got = m.backward_delta(g)
     ref = np.array([math.fsum(ef * lh[l]) / delta[l] for l in range(8)])
     assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref).max()), (got, ref)
+    # the Jacobian forward at full size: the same sums bit for bit, and its
+    # last row dM(E)/d log delta_l = delta_l dM(E)/d delta_l = the
+    # replay's gradient for L = M(E) times delta_l, = sum_w E_w h_wl
+    Mj, Jj = (t.cpu().numpy() for t in m.forward_jacobian(delta))
+    assert np.array_equal(Mj, Mf)
+    assert np.all(np.abs(Jj[:, -1] - ref * delta) <= 1e-12 * np.abs(ref * delta).max()), (Jj[:, -1], ref * delta)
+    assert np.all(Jj[:, 0] == 0.0)
     # central differences with common random numbers, h = 1e-3 relative on rho
     c = np.random.default_rng(1).standard_normal(E + 1)
     m.forward_sums(delta)
