@@ -59,8 +59,10 @@ def log(msg):
 class ClockSampler:
     """The GPU's GFX clock over a timed region, read from the system
     management library (amdsmi; no HIP calls) by a thread every `period`
-    seconds; samples taken while the GFX engine is less than 50% busy are
-    dropped.  `result()` is None where amdsmi is missing or fails."""
+    seconds.  The median is over the samples that read the GFX engine at
+    least 50% busy, or over all samples when none does (the activity figure
+    averages over a longer window than a short timed region).  `result()`
+    is None where amdsmi is missing or fails."""
 
     def __init__(self, period=0.01):
         self.period, self.samples, self.error = period, [], None
@@ -78,9 +80,9 @@ class ClockSampler:
             try:
                 c = self._smi.amdsmi_get_clock_info(self._h, self._smi.AmdSmiClkType.GFX)
                 a = self._smi.amdsmi_get_gpu_activity(self._h)
-                if isinstance(c.get("clk"), (int, float)) and isinstance(a.get("gfx_activity"), (int, float)) \
-                        and a["gfx_activity"] >= 50:
-                    self.samples.append(float(c["clk"]))
+                if isinstance(c.get("clk"), (int, float)):
+                    act = a.get("gfx_activity")
+                    self.samples.append((float(c["clk"]), float(act) if isinstance(act, (int, float)) else -1.0))
             except Exception as e:  # noqa: BLE001
                 self.error = str(e)
                 return
@@ -108,9 +110,12 @@ class ClockSampler:
     def result(self):
         if not self.samples:
             return None
-        v = sorted(self.samples)
+        busy = [c for c, act in self.samples if act >= 50]
+        v = sorted(busy or [c for c, _ in self.samples])
         return {"median_MHz": v[len(v) // 2], "min_MHz": v[0], "max_MHz": v[-1], "samples": len(v),
-                "source": "amdsmi GFX clock, sampled every 10 ms over the timed steps (GFX activity >= 50%)"}
+                "busy_samples": len(busy),
+                "source": "amdsmi GFX clock, sampled every 10 ms over the timed steps"
+                          + (" (GFX activity >= 50%)" if busy else " (all samples: none read the engine busy)")}
 
 
 def main():
@@ -225,7 +230,10 @@ def main():
         t1 = time.perf_counter()
     clock = sampler.result()
     if clock:
-        log(f"GFX clock over the timed steps: median {clock['median_MHz']:.0f} MHz ({clock['samples']} samples)")
+        log(f"GFX clock over the timed steps: median {clock['median_MHz']:.0f} MHz ({clock['samples']} samples, "
+            f"{clock['busy_samples']} busy)")
+    else:
+        log(f"GFX clock not sampled ({sampler.error or 'no samples'})")
     kern_ms, launches = ctx.get_timing(reset=True)
     ctx.set_timing(False)
 
