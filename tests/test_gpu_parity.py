@@ -456,6 +456,37 @@ def test_pfgse_matches_oracle(ctx, bc):
     assert np.array_equal(g["msd"], o["msd"])  # integer-valued sums: exact
 
 
+@pytest.mark.parametrize("dims", [(32, 32, 32), (30, 21, 17)])
+@pytest.mark.parametrize("kind", ["arithmetic", "random"])
+def test_pfgse_mirror_reflection_flags_match_oracle(ctx, dims, kind):
+    """Mirror PFGSE from starts in every mirror image: the walk folds each
+    walker into the first octant with its reflection flags (8 flag
+    combinations; periods with and without the wrap fix-up), two energy
+    sets; walkers bit-exact, M within 1e-11 M(0), msd exact."""
+    print(f"[test] pfgse mirror flags dims={dims} kind={kind}", flush=True)
+    W, H, D = dims
+    solid, seeds, x, y, z, _ = _case(W, H, D, 4, 0.35, 2500, 51)
+    x, y, z = _mirror_images(solid, x, y, z, 53)
+    img = R.Image.from_solid(ctx, solid)
+    steps_inc = np.array([2, 7, 15, 30, 46], np.int64)
+    if kind == "arithmetic":
+        kx = np.linspace(0.0, 1.1, 16)
+    else:
+        kx = np.random.default_rng(3).uniform(0.0, 1.1, 16)
+    ky = 0.02 * np.arange(16)
+    kz = np.zeros(16)
+    sd = np.array([1.0, 0.985])
+    g = R.pfgse_run(ctx, img, x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=ky, kz=kz, voxel=0.9, bc="mirror",
+                    set_delta=sd)
+    o = O.pfgse(O.bitblock_pack(solid), dims, x, y, z, seeds, steps_inc=steps_inc, kx=kx, ky=ky, kz=kz,
+                voxel=0.9, bc="mirror", set_delta=sd)
+    for k in ("x", "y", "z", "seed", "hits"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.all(np.abs(g["M"] - o["M"]) <= 1e-11 * np.abs(o["M"][..., :1]))
+    assert np.array_equal(g["msd"], o["msd"])
+    img.close()
+
+
 def test_pfgse_k_set_per_diffusion_time(ctx):
     """k_max scaled per diffusion time (k sqrt(D0 Delta) = const)."""
     print("[test] pfgse k per delta", flush=True)
