@@ -48,7 +48,10 @@ SIMDS, CLOCK_HZ, CYCLES_PER_VALU = 1024, 2.4e9, 4
 VALU_WAVE_INSTR_PER_S = SIMDS * CLOCK_HZ / CYCLES_PER_VALU
 # committed rocprofv3 counters of the headline walk kernel (VALU per wave-step,
 # L2 requests and HBM bytes per launch), applied when the launch shape matches
-COUNTERS_FILE = os.path.join(ROOT, "profiles", "walk_kernel_cpmg512_counters.json")
+# committed counters of the walk kernel per image form: 16-byte (3) and
+# 8-byte (6) register chunk caches
+COUNTERS_FILES = {3: os.path.join(ROOT, "profiles", "walk_kernel_cpmg512_counters.json"),
+                  6: os.path.join(ROOT, "profiles", "walk_kernel_cpmg512_chunk8_counters.json")}
 PROBE_TABLES = {"2MiB_L2": 2 << 20, "128MiB_MALL": 128 << 20, "4GiB_HBM": 4 << 30}
 
 
@@ -60,9 +63,11 @@ class ClockSampler:
     """The GPU's GFX clock over a timed region, read from the system
     management library (amdsmi; no HIP calls) by a thread every `period`
     seconds.  The median is over the samples that read the GFX engine at
-    least 50% busy, or over all samples when none does (the activity figure
-    averages over a longer window than a short timed region).  `result()`
-    is None where amdsmi is missing or fails."""
+    least 50% busy, or, when none does (the activity figure averages over a
+    longer window than a short timed region), over the samples clocked at
+    1 GHz or more (a loaded engine runs well above that; an idle or stale
+    reading sits at a few hundred MHz).  `result()` is None where amdsmi is
+    missing or fails, or no sample qualifies."""
 
     def __init__(self, period=0.01):
         self.period, self.samples, self.error = period, [], None
@@ -111,11 +116,16 @@ class ClockSampler:
         if not self.samples:
             return None
         busy = [c for c, act in self.samples if act >= 50]
-        v = sorted(busy or [c for c, _ in self.samples])
+        v = sorted(busy or [c for c, _ in self.samples if c >= 1000.0])
+        if not v:
+            self.error = (f"{len(self.samples)} samples, none read the GFX engine busy or clocked at 1 GHz or more "
+                          f"(median {sorted(c for c, _ in self.samples)[len(self.samples) // 2]:.0f} MHz)")
+            return None
         return {"median_MHz": v[len(v) // 2], "min_MHz": v[0], "max_MHz": v[-1], "samples": len(v),
                 "busy_samples": len(busy),
                 "source": "amdsmi GFX clock, sampled every 10 ms over the timed steps"
-                          + (" (GFX activity >= 50%)" if busy else " (all samples: none read the engine busy)")}
+                          + (" (GFX activity >= 50%)" if busy else
+                             " (no sample read the engine busy: the samples at 1 GHz or more)")}
 
 
 def main():
@@ -248,11 +258,13 @@ def main():
     # same kernel code: boundary, walkers per lane, image form, and for mirror walks the
     # direction-table form (flag-keyed tables with 1024-thread blocks)
     prof = None
-    if os.path.exists(COUNTERS_FILE):
-        with open(COUNTERS_FILE) as f:
+    counters_file = COUNTERS_FILES.get(form)
+    if counters_file and os.path.exists(counters_file):
+        with open(counters_file) as f:
             prof = json.load(f)
+        prof["file"] = os.path.relpath(counters_file, ROOT)
         prof_tpb = int(str(prof.get("launch", "0,0")).split(",")[0])
-        if prof.get("bc") != args.bc or int(prof.get("walkers_per_lane", 0)) != k or form != 3 or \
+        if prof.get("bc") != args.bc or int(prof.get("walkers_per_lane", 0)) != k or \
                 (args.bc == "mirror" and (prof_tpb >= 1024) != (tpb >= 1024)):
             prof = None
     # gather probe: random 16-byte loads from tables held in L2, in the MALL and in HBM
@@ -318,11 +330,12 @@ def main():
                                f"{spe} steps, {args.bc} BC, voxel {voxel} um, rho {rho} um/s",
                    "walker_steps_per_step": walker_steps,
                    "launch": {"threads_per_block": tpb, "walkers_per_lane": k,
-                              "image_form": {3: "unfolded + register chunk cache", 0: "folded bitBlock bricks"}.get(
-                                  form, f"mode {form}"),
+                              "image_form": {3: "unfolded + 16-byte register chunk cache",
+                                             6: "unfolded + 8-byte register chunk cache (mirror)",
+                                             0: "folded bitBlock bricks"}.get(form, f"mode {form}"),
                               "echoes_between_resorts": resort}},
         "roofline": roofline(args, prof, probe, kernel_steps_per_s, steps_per_launch, avg_kernel_s, launches, k,
-                             tpb, clock),
+                             tpb, clock, form),
         "cpu_baseline": cpu,
         "checks": {"decay_monotone": decay_ok, "subsample_bit_exact_vs_cpu": parity},
     }
@@ -334,21 +347,23 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def walk_kernel_label(bc, k, threads):
-    form = {"mirror": "unfolded mirror image, walkers folded into its first octant with their reflection flags "
-                      "at every launch",
-            "periodic": "unfolded periodic image",
-            "noflux": "periodic unfolded image with the noflux face test and turn-around"}[bc]
-    # mirror walks of blocks of 512 threads or more read flag-keyed byte tables
-    table = ("flag-keyed byte-key tables" if bc == "mirror" and threads >= 512 else
+def walk_kernel_label(bc, k, threads, form=3):
+    image = {"mirror": "unfolded mirror image, walkers folded into its first octant with their reflection flags "
+                       "at every launch",
+             "periodic": "unfolded periodic image",
+             "noflux": "periodic unfolded image with the noflux face test and turn-around"}[bc]
+    # mirror walks of 1024-thread blocks read flag-keyed byte tables
+    table = ("flag-keyed byte-key tables" if bc == "mirror" and threads >= 1024 else
              "byte-key direction table" + (" per reflection-flag combination" if bc == "mirror" else ""))
-    return (f"zwalk_kernel<CPMG, {k} walker(s) per lane, 16-byte register chunk cache by indexed buffer loads, "
-            f"{table}, direction drawn a step ahead> on the {form}")
+    chunk = "8-byte (4 x 4 x 4 voxel)" if (form == 6 and bc == "mirror") else "16-byte (8 x 4 x 4 voxel)"
+    return (f"zwalk_kernel<CPMG, {k} walker(s) per lane, {chunk} register chunk cache by indexed buffer loads, "
+            f"{table}, direction drawn a step ahead> on the {image}")
 
 
-def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, launches, k, threads, clock=None):
+def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, launches, k, threads, clock=None,
+             form=3):
     """Roofline of the dominant kernel.  The binding roof is VALU issue: the
-    walk keeps each walker's 16-byte image chunk in registers and the L2
+    walk keeps each walker's 16- or 8-byte image chunk in registers and the L2
     serves nearly every reload, so HBM is far from busy; its ceiling in
     walker-steps/s is the VALU issue rate x 64 lanes / VALU instructions per
     wave-step (committed SQ counters).  Secondary figures: the algorithmic
@@ -356,7 +371,7 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
     gathers), the measured HBM bytes per launch, and the walk's L2 request
     rate against the random-gather probe's."""
     out = {"bound": "valu", "achieved": steps_per_s, "peak": None, "unit": "walker-steps/s", "frac": None,
-           "traffic": None, "kernel": walk_kernel_label(args.bc, k, threads), "avg_kernel_ms": avg_kernel_s * 1e3,
+           "traffic": None, "kernel": walk_kernel_label(args.bc, k, threads, form), "avg_kernel_ms": avg_kernel_s * 1e3,
            "launches": launches, "walker_steps_per_launch": steps_per_launch,
            "peak_basis": f"{SIMDS} SIMDs x {CLOCK_HZ / 1e9} GHz / {CYCLES_PER_VALU} cycles per wave64 VALU "
                          "instruction x 64 lanes / VALU instructions per wave-step"}
@@ -367,7 +382,7 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
         vps = float(prof["valu_per_wave_step"])
         peak = VALU_WAVE_INSTR_PER_S * 64.0 / vps
         out.update(peak=peak, frac=steps_per_s / peak, valu_per_wave_step=vps,
-                   valu_busy_measured=prof.get("valu_busy"), counters=os.path.relpath(COUNTERS_FILE, ROOT))
+                   valu_busy_measured=prof.get("valu_busy"), counters=prof["file"])
         if clock:
             # the same ceiling at the GFX clock this run ran at (sustained
             # load runs below the 2.4 GHz peak), sampled over its timed steps

@@ -101,7 +101,9 @@ rwnmr_status rwnmr_ctx_set_locality(rwnmr_ctx *ctx, int32_t enable, uint32_t res
  * direction tables, byte-key form).  5: as 3 for every walk except CPMG with
  * one walker per lane, which loads the chunks of both possible next
  * candidates a step ahead (two loads per walker-step) and draws directions
- * two steps ahead: for latency-bound runs of few walkers.  Results are
+ * two steps ahead: for latency-bound runs of few walkers.  6: as 3, with
+ * mirror walks keeping an 8-byte chunk (4 x 4 x 4 voxels: 3 VALU per step
+ * fewer, a quarter more chunk loads); other walks run as 3.  Results are
  * identical in every mode. */
 rwnmr_status rwnmr_ctx_set_unfolded(rwnmr_ctx *ctx, int32_t mode);
 
@@ -146,9 +148,10 @@ typedef struct {
  * shape, threads per block 64..1024 x walkers per lane 1/2/4 (walk kernel
  * time); (2) the image form the walk reads, the unfolded Z-order image with
  * the per-walker register chunk cache against the folded brick-ordered
- * bitBlock words (rwnmr_ctx_set_unfolded 3 / 0; walk kernel time), and for
- * at most 65,536 walkers with one walker per lane the speculative walk (5);
- * (3) for
+ * bitBlock words (rwnmr_ctx_set_unfolded 3 / 0; walk kernel time), for
+ * mirror walks the 8-byte chunk against the 16-byte one (6 / 3, each timed
+ * twice in mirrored order), and for at most 65,536 walkers with one walker
+ * per lane the speculative walk (5); (3) for
  * runs that use the locality ordering, the echoes per launch between walker
  * re-sorts, trial_echoes / 4, / 2 or trial_echoes (whole-run time, re-sorts
  * included).  best_* report step 1's shape and the fastest walk time; the

@@ -46,12 +46,17 @@ def _run(cmd):
 # probe compiles that binding; a toolchain without it builds the raw form.
 _PROBE_SRC = """#include <hip/hip_runtime.h>
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
 __device__ v4u load4(__amdgpu_buffer_rsrc_t r, int vindex, int voffset, int soffset, int aux)
     __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+__device__ v2u load2(__amdgpu_buffer_rsrc_t r, int vindex, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
 __global__ void probe(unsigned int *w, v4u *out, int i)
 {
     __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(w, (short)16, 64, 0x00020000);
-    out[threadIdx.x] = load4(r, i, 0, 0, 0);
+    __amdgpu_buffer_rsrc_t r8 = __builtin_amdgcn_make_buffer_rsrc(w, (short)8, 128, 0x00020000);
+    v2u h = load2(r8, i, 0, 0, 0);
+    out[threadIdx.x] = load4(r, i, 0, 0, 0) + (v4u){h.x, h.y, 0u, 0u};
 }
 """
 

@@ -86,7 +86,9 @@ class Context:
         ahead, through the 4 KiB byte-key table (3, default) or the 64 KiB
         wide direction table (4); 5 as 3, with CPMG of one walker per lane
         loading both possible next chunks a step ahead (latency-bound runs
-        of few walkers)."""
+        of few walkers); 6 as 3, with mirror walks keeping an 8-byte (4 x 4
+        x 4 voxel) chunk (3 VALU per step fewer, a quarter more chunk
+        loads)."""
         check(load().rwnmr_ctx_set_unfolded(self.handle, int(mode)), self.handle)
 
     def set_timing(self, enable: bool = True):

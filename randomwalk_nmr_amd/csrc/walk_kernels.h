@@ -42,7 +42,8 @@ struct WalkArgs {
     uint32_t *error_flag;            // device word, bit 0: walker started outside the image
     const uint32_t *zwords;          // unfolded image (periodic / mirror fast path), else null
     ZLayout zl;                      // its dilated address layout
-    uint32_t zcache;                 // chunk cache in registers: 0 none, 1 16-byte (word select), 2/3 16-byte, pipelined
+    uint32_t zcache;                 // chunk cache in registers: 0 none, 1 16-byte (word select), 2/3 16-byte, pipelined,
+                                     // 4 speculative walk, 5 8-byte pipelined (mirror)
     uint32_t znoflux;                // 1: noflux walk on the periodic unfolded image (zl of that image)
     uint32_t zmflags;                // 1: mirror walk started in the first octant with reflection flags
     const uint8_t *zlabels;          // solid labels, one byte per voxel in the unfolded image's address order

@@ -338,6 +338,7 @@ __device__ __forceinline__ void zchunk_fill_n(const char *words, uint32_t c, uin
 constexpr int32_t kZChunkRsrcWord3 = 0x00020000;
 typedef __amdgpu_buffer_rsrc_t zrsrc_t;
 typedef uint32_t zv4u __attribute__((ext_vector_type(4)));
+typedef uint32_t zv2u __attribute__((ext_vector_type(2)));
 
 #if defined(RWNMR_RAW_BUFFER_LOAD)
 // Raw buffer load (a documented clang builtin) at byte offset cid * 16: one
@@ -345,6 +346,11 @@ typedef uint32_t zv4u __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ zv4u zchunk_load(zrsrc_t rsrc, uint32_t cid)
 {
     return __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int32_t)(cid << 4), 0, 0);
+}
+// 8-byte chunks (64 addresses) at byte offset cid * 8
+__device__ __forceinline__ zv2u zchunk_load8(zrsrc_t rsrc, uint32_t cid)
+{
+    return __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int32_t)(cid << 3), 0, 0);
 }
 #else
 // Indexed (struct) buffer load: the chunk id itself is the index, so no
@@ -357,12 +363,40 @@ __device__ __forceinline__ zv4u zchunk_load(zrsrc_t rsrc, uint32_t cid)
 {
     return zbuffer_load_v4(rsrc, (int32_t)cid, 0, 0, 0);
 }
+// 8-byte chunks (64 addresses): a descriptor of stride 8 (zchunk_rsrc8)
+__device__ zv2u zbuffer_load_v2(zrsrc_t rsrc, int32_t vindex, int32_t voffset, int32_t soffset, int32_t aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
+__device__ __forceinline__ zv2u zchunk_load8(zrsrc_t rsrc, uint32_t cid)
+{
+    return zbuffer_load_v2(rsrc, (int32_t)cid, 0, 0, 0);
+}
 #endif
 
 __device__ __forceinline__ zrsrc_t zchunk_rsrc(const uint32_t *words, const ZLayout &zl)
 {
     const int32_t chunks = (int32_t)(zl.bits > 7 ? (1u << (zl.bits - 7)) : 1u);   // 128 addresses each
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(words), (short)16, chunks, kZChunkRsrcWord3);
+}
+
+// The same over the image's 8-byte chunks (4 x 4 x 4 voxels, 64 addresses;
+// stride 8, num_records = that chunk count).
+__device__ __forceinline__ zrsrc_t zchunk_rsrc8(const uint32_t *words, const ZLayout &zl)
+{
+    const int32_t chunks = (int32_t)(zl.bits > 6 ? (1u << (zl.bits - 6)) : 1u);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(words), (short)8, chunks, kZChunkRsrcWord3);
+}
+
+// 8-byte chunk cache: QV.x / QV.y hold the chunk of address c (QB its id,
+// c >> 6), read when c left the chunk held.
+__device__ __forceinline__ void zchunk_fill_buf8(const zrsrc_t &rsrc, uint32_t c, uint32_t &QB, uint4 &QV)
+{
+    const uint32_t cid = c >> 6;
+    if (cid != QB) {
+        const zv2u v = zchunk_load8(rsrc, cid);
+        QV.x = v.x;
+        QV.y = v.y;
+    }
+    QB = cid;
 }
 
 __device__ __forceinline__ void zchunk_fill_buf(const zrsrc_t &rsrc, uint32_t c, uint32_t &QB, uint4 &QV)
@@ -469,7 +503,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     constexpr bool LH_LDS = EMODE == 2 && K == 1 && NS == 1;
     constexpr int NL = (EMODE == 2 && !LH_LDS && NS == 1) ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
-    const zrsrc_t zrsrc = zchunk_rsrc(a.zwords, a.zl);
+    const zrsrc_t zrsrc = CACHE == 64 ? zchunk_rsrc8(a.zwords, a.zl) : zchunk_rsrc(a.zwords, a.zl);
     const uint64_t base = tile * blockDim.x * K + threadIdx.x;
     const uint64_t n = a.n;
     const uint32_t echoes = a.echoes, spe = a.steps_per_echo;
@@ -634,7 +668,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 if (CACHE == 128)
                     zchunk_fill_buf(zrsrc, C[k], QB[k], QV[k]);
                 else
-                    zchunk_fill_n<64>(words, C[k], QB[k], QV[k]);
+                    zchunk_fill_buf8(zrsrc, C[k], QB[k], QV[k]);
             });
             static_for<K>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;

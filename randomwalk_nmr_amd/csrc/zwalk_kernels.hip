@@ -461,7 +461,9 @@ static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t bl
 // a.zcache: 0 no chunk cache; 1 128-bit chunk with word select; 2 and 3 a
 // 128-bit chunk with the shift test and the direction drawn one step ahead,
 // through the byte-key table (2) or, for power-of-two periods, the wide
-// direction table (3; the 6-entry table otherwise).
+// direction table (3; the 6-entry table otherwise); 4 the speculative CPMG
+// walk; 5 as 2 with a 64-bit (4 x 4 x 4 voxel) chunk, mirror walks with
+// reflection flags (others run 2).
 template <bool WRAPFIX, bool NOFLUX, bool MFLAGS>
 static hipError_t launch_z_spec(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
@@ -490,6 +492,18 @@ static hipError_t launch_z_mode(const WalkArgs &a, const LaunchShape &ls, uint32
     }
     if (a.znoflux)   // noflux on the periodic image: pipelined byte-key walk only
         return a.zcache >= 2 ? launch_z<EMODE, false, 128, 2, true>(a, ls, blocks, st) : hipErrorInvalidValue;
+    if (a.zmflags && a.zcache == 5) {   // mirror, 8-byte chunk cache
+        if (EMODE < 2 && ls.threads >= 1024)
+            return a.zl.wrapfix ? launch_z<EMODE, true, 64, 3, false, true>(a, ls, blocks, st)
+                                : launch_z<EMODE, false, 64, 3, false, true>(a, ls, blocks, st);
+        return a.zl.wrapfix ? launch_z<EMODE, true, 64, 2, false, true>(a, ls, blocks, st)
+                            : launch_z<EMODE, false, 64, 2, false, true>(a, ls, blocks, st);
+    }
+    if (a.zcache == 5) {
+        WalkArgs b = a;
+        b.zcache = 2;
+        return launch_z_mode<EMODE>(b, ls, blocks, st);
+    }
     if (a.zmflags && a.zcache >= 2) {   // mirror, walkers folded to the first octant with reflection flags
         // CPMG and map walks of 1024-thread blocks read the flag-keyed
         // tables (58 KiB of LDS: two blocks per CU, 8 waves per SIMD)

@@ -21,8 +21,9 @@ def test_autotuned_context_stays_exact(n):
         tpb, k, ms = ctx.autotune(img, x, y, z, seeds, steps_per_echo=3, bc="mirror", trial_echoes=40, delta=delta)
         t2, k2, form, resort = ctx.get_tuning()
         assert (t2, k2) == (tpb, k) and tpb in (64, 128, 256, 512, 1024) and k in (1, 2, 4) and ms > 0
-        # the speculative walk (5) is timed for runs of at most 65,536 walkers
-        assert form in ((0, 3, 5) if n <= 65536 else (0, 3))
+        # the speculative walk (5) is timed for runs of at most 65,536
+        # walkers; mirror walks also time the 8-byte chunk cache (6)
+        assert form in ((0, 3, 5, 6) if n <= 65536 else (0, 3, 6))
         # 70,000 walkers use the locality ordering, so the re-sort interval was tuned too
         assert resort in ((10, 20, 40) if n >= 65536 else (1000,))
         g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=60, steps_per_echo=3, bc="mirror", delta=delta)

@@ -36,10 +36,12 @@ def _same(got, want):
 
 
 @pytest.mark.parametrize("bc", ["mirror", "noflux", "periodic"])
-def test_graph_replays_equal_eager_runs(ctx, bc):
+@pytest.mark.parametrize("unfolded", [3, 6])
+def test_graph_replays_equal_eager_runs(ctx, bc, unfolded):
     import torch
-    print(f"[test] cpmg graph bc={bc}", flush=True)
+    print(f"[test] cpmg graph bc={bc} unfolded={unfolded}", flush=True)
     solid, _ = R.sphere_pack(48, 40, 36, 3.0, 7.0, 0.3, 21)
+    ctx.set_unfolded(unfolded)
     img = R.Image.from_solid(ctx, solid)
     delta = R.penalty(15.0, 1.0, 2.5)
     E = 300
@@ -64,6 +66,7 @@ def test_graph_replays_equal_eager_runs(ctx, bc):
             torch.cuda.synchronize()
     finally:
         g.close()
+        ctx.set_unfolded(3)
     img.close()
 
 

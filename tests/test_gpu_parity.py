@@ -81,14 +81,15 @@ def test_locality_order_and_brick_layout_match_oracle(ctx, bc, layout, shape):
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("dims", [(32, 16, 8), (16, 16, 16), (30, 21, 17), (9, 1, 5), (2, 3, 64)])
-@pytest.mark.parametrize("unfolded", [5, 4, 3, 2, 1, 0])
+@pytest.mark.parametrize("unfolded", [6, 5, 4, 3, 2, 1, 0])
 @pytest.mark.parametrize("shape", [(256, 1), (512, 2), (256, 4)])
 def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
     """Noflux / periodic / mirror CPMG and map walks on the unfolded image
     with and without the per-walker chunk cache (power-of-two periods: plain
     dilated wrap; others: wrap fix; noflux: face test and turn-around on the
     periodic image, folded kernel when a dimension is 1), the speculative
-    two-candidate CPMG walk (form 5, one walker per lane) and the folded
+    two-candidate CPMG walk (form 5, one walker per lane), the 8-byte chunk
+    cache of mirror walks (form 6; other walks run form 3) and the folded
     kernel: walkers bit-exact, echo sums to 1e-12, for every launch shape."""
     print(f"[test] unfolded={unfolded} bc={bc} dims={dims} launch={shape}", flush=True)
     W, H, D = dims
@@ -121,7 +122,7 @@ def test_unfolded_and_folded_walks_match_oracle(ctx, bc, dims, unfolded, shape):
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("spe", [1, 600])
-@pytest.mark.parametrize("unfolded", [5, 4, 3, 2])
+@pytest.mark.parametrize("unfolded", [6, 5, 4, 3, 2])
 def test_unfolded_long_echoes_and_locality_match_oracle(ctx, bc, spe, unfolded):
     """Steps per echo above the packed-displacement flush interval (511) and
     the Z-order re-sort of the unfolded walk on a power-of-two image."""
@@ -158,19 +159,21 @@ def _mirror_images(solid, x, y, z, seed):
 
 @pytest.mark.parametrize("dims", [(32, 16, 8), (30, 21, 17)])
 @pytest.mark.parametrize("shape", [(256, 1), (512, 1), (1024, 1), (512, 2)])
-def test_mirror_reflection_flags_match_oracle(ctx, dims, shape):
+@pytest.mark.parametrize("unfolded", [3, 6])
+def test_mirror_reflection_flags_match_oracle(ctx, dims, shape, unfolded):
     """Mirror CPMG and map walks from starts in every mirror image, so each
     launch folds its walkers with all 8 reflection-flag combinations, for
     the launch shapes of both direction-table forms (blocks of 512 threads
     or more read the flag-keyed tables) and periods with and without the
     wrap fix-up; a locality re-sort every 5 echoes makes later launches
     start from the walkers' moved positions."""
-    print(f"[test] mirror flags dims={dims} launch={shape}", flush=True)
+    print(f"[test] mirror flags dims={dims} launch={shape} unfolded={unfolded}", flush=True)
     W, H, D = dims
     solid, seeds, x, y, z, pen = _case(W, H, D, 3.5, 0.35, 3001, 41)
     x, y, z = _mirror_images(solid, x, y, z, 43)
     ctx.set_launch(*shape)
     ctx.set_locality(True, resort_every=5, min_walkers=0)
+    ctx.set_unfolded(unfolded)
     try:
         img = R.Image.from_solid(ctx, solid)
         g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=23, steps_per_echo=7, bc="mirror", penalty=pen)
@@ -178,6 +181,7 @@ def test_mirror_reflection_flags_match_oracle(ctx, dims, shape):
     finally:
         ctx.set_locality(True, resort_every=1000, min_walkers=65536)
         ctx.set_launch(256, 1)
+        ctx.set_unfolded(3)
     o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=23, steps_per_echo=7, bc="mirror", penalty=pen)
     _assert_walkers(g, o)
     _assert_echoes(g["echo_sums"], o["echo_sums"])
@@ -230,7 +234,7 @@ def test_cpmg_labels_match_oracle(ctx):
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 @pytest.mark.parametrize("dims", [(28, 28, 28), (32, 16, 8)])
-@pytest.mark.parametrize("unfolded", [4, 3, 2, 1, 0])
+@pytest.mark.parametrize("unfolded", [6, 4, 3, 2, 1, 0])
 def test_cpmg_labels_unfolded_and_folded_match_oracle(ctx, bc, dims, unfolded):
     """Per-label CPMG on the unfolded image (4-bit label volume) and on the
     folded kernel: walkers, energies and per-label hit counts bit-exact."""

@@ -81,13 +81,16 @@ def test_noalloc_matches_cpmg_run_twice():
         # call 2: fewer walkers, another launch shape and boundary; rwnmr_cpmg_run above built
         # the unfolded images it uses, so this call walks them
         ctx.set_launch(256, 1)
-        for bc in ("mirror", "noflux"):
+        for bc, form in (("mirror", 3), ("noflux", 3), ("mirror", 6)):
+            # image form 6: the 8-byte chunk cache of mirror walks
+            ctx.set_unfolded(form)
             base = _device_walkers(torch, solid, 23, 3000, with_hits=False)
             a, b = _clone(base), _clone(base)
             sb = _run(ctx, img, "run", b, 200, 3, bc, delta)
-            print(f"[test] noalloc call 2: 3000 walkers, 200 echoes, {bc}, 256x1", flush=True)
+            print(f"[test] noalloc call 2: 3000 walkers, 200 echoes, {bc}, 256x1, form {form}", flush=True)
             sa = _run(ctx, img, "noalloc", a, 200, 3, bc, delta)
             _compare(a, b, sa, sb)
+        ctx.set_unfolded(3)
         img.close()
     finally:
         ctx.close()
