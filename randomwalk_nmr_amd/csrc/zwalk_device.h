@@ -454,6 +454,26 @@ __device__ __forceinline__ uint32_t zlabel_at(const WalkArgs &a, uint32_t c)
     return a.zlabels[zoctant(c, a)];
 }
 
+// The coordinate held in the bits m of address v (the axis' dilated bits
+// gathered to the low end).  m is wave-uniform, so the loop is scalar: 2
+// VALU per bit.
+__device__ __forceinline__ uint32_t zcompact(uint32_t v, uint32_t m)
+{
+    uint32_t u = 0u, j = 0u;
+    for (uint32_t mm = m; mm; mm &= mm - 1u, ++j)
+        u |= ((v >> __builtin_ctz(mm)) & 1u) << j;
+    return u;
+}
+
+// Address displacements recover the unbounded moves when every period of
+// the layout exceeds twice the flush interval: a walker moves at most
+// kZFlush voxels per axis between flushes, less than half a period, so the
+// coordinate change modulo the period is the move itself.
+__host__ __device__ inline bool zaddress_displacement_ok(const ZLayout &zl)
+{
+    return zl.period[0] > 2u * kZFlush && zl.period[1] > 2u * kZFlush && zl.period[2] > 2u * kZFlush;
+}
+
 // One tile (blockDim.x * K consecutive walkers from tile * blockDim.x * K)
 // walked for a.echoes x a.steps_per_echo steps on the unfolded image
 // a.zwords / a.zl, with the direction table already in LDS.  EMODE 0 = hit
@@ -487,8 +507,14 @@ __device__ __forceinline__ uint32_t zlabel_at(const WalkArgs &a, uint32_t c)
 // to a.jac_partials + tile, row stride part_stride, label stride (a.echoes +
 // 1) part_stride, through the 8-value reduce-scatter (lds8: the staging of
 // stage_echo_sets8).
+// ADISP (mirror walks with reflection flags, every period above 2 kZFlush:
+// zaddress_displacement_ok): no packed displacement per move; each flush
+// recovers the walker's unbounded moves from its address change since the
+// last flush (per axis, the coordinate change taken modulo the period into
+// (-period / 2, period / 2), negated along reflected axes), one VALU per
+// move fewer.
 template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false, bool MFLAGS = false,
-          int NS = 1, bool JAC = false>
+          int NS = 1, bool JAC = false, bool ADISP = false>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
                                            const double *ldelta = nullptr, const uint8_t *keys = nullptr,
@@ -499,6 +525,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     static_assert(!NOFLUX || CACHE >= 64, "noflux: pipelined chunk-cache walk only");
     static_assert(TAB != 3 || (MFLAGS && CACHE >= 64), "flag-keyed tables: pipelined mirror walk only");
     static_assert(NS == 1 || (NS == 8 && EMODE == 2 && K == 1 && CACHE >= 64), "label sets: 8 sets, K == 1");
+    static_assert(!ADISP || (MFLAGS && !NOFLUX && CACHE >= 64), "address displacements: pipelined mirror walk");
     // label-set walks count no per-label hits (a.label_hits is null)
     constexpr bool LH_LDS = EMODE == 2 && K == 1 && NS == 1;
     constexpr int NL = (EMODE == 2 && !LH_LDS && NS == 1) ? kMaxLabels : 1;
@@ -510,6 +537,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     const int n_labels = a.n_labels;
 
     uint32_t A[K], C[K], Hc[K];
+    uint32_t A0[K];   // ADISP: the address at the last flush
     uint32_t FO[K];   // MFLAGS: the walker's direction table offset (flags x kZFlagStride)
     uint32_t QB[K];   // chunk cache: chunk id (address >> 6 or >> 7) held in QV
     uint4 QV[K];
@@ -558,6 +586,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
             FO[k] = 0u;
         }
         P[k] = 0;
+        A0[k] = A[k];
         PW[k] = 0u;
         PEND[k] = false;
         if (NS > 1)
@@ -586,11 +615,30 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
         static_for<K>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             int32_t dx, dy, dz;
-            zunpack(P[k], dx, dy, dz);
+            if constexpr (ADISP) {
+                const uint32_t flags = FO[k] / (TAB == 3 ? kZKeyEntries : kZFlagStride);
+                int32_t d[3];
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int32_t per = (int32_t)a.zl.period[ax];
+                    int32_t s = (int32_t)zcompact(A[k], a.zl.m[ax]) - (int32_t)zcompact(A0[k], a.zl.m[ax]);
+                    if (s > per / 2)
+                        s -= per;
+                    else if (s < -(per / 2))
+                        s += per;
+                    d[ax] = ((flags >> ax) & 1u) ? -s : s;
+                }
+                dx = d[0];
+                dy = d[1];
+                dz = d[2];
+                A0[k] = A[k];
+            } else {
+                zunpack(P[k], dx, dy, dz);
+                P[k] = 0;
+            }
             X[k] += dx;
             Y[k] += dy;
             Z[k] += dz;
-            P[k] = 0;
         });
     };
     const char *tab_bytes = reinterpret_cast<const char *>(tab);
@@ -703,7 +751,10 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                     E[k] *= DL[k];
                 } else {
                     A[k] = C[k];
-                    P[k] += DP[k];
+                    if (!ADISP)
+                        P[k] += DP[k];
+                    else   // a move of one copy: kept a branch, not turned into selects
+                        asm volatile("" : "+v"(A[k]));
                 }
                 return;
             }
@@ -714,12 +765,14 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                     PEND[k] = true;
                 } else {
                     A[k] = C[k];
-                    P[k] += DP[k];
+                    if (!ADISP)
+                        P[k] += DP[k];
                 }
                 return;
             }
             A[k] = wall ? A[k] : C[k];
-            P[k] += wall ? 0 : DP[k];
+            if (!ADISP)
+                P[k] += wall ? 0 : DP[k];
             Hc[k] += wall;
             if (EMODE == 1 && wall)
                 E[k] *= DL[k];

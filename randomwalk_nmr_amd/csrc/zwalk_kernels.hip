@@ -11,7 +11,8 @@
 
 namespace rwnmr {
 
-template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false, bool MFLAGS = false>
+template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false, bool MFLAGS = false,
+          bool ADISP = false>
 __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kEchoCols];
@@ -45,9 +46,9 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
         zbuild_table_wide(small, tab);
         __syncthreads();
     }
-    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS>(a, tile, a.partials ? a.partials + tile : nullptr,
-                                                              a.partial_stride, lds, tab, fix, ldelta, key_bytes,
-                                                              EMODE == 2 && K == 1 ? zlh : nullptr);
+    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS, 1, false, ADISP>(
+        a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride, lds, tab, fix, ldelta, key_bytes,
+        EMODE == 2 && K == 1 ? zlh : nullptr);
 }
 
 // Speculative CPMG walk (zwalk_tile_spec): one walker per lane, for runs
@@ -447,6 +448,15 @@ hipError_t launch_zkeys(const int32_t *x, const int32_t *y, const int32_t *z, ui
 template <int EMODE, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false, bool MFLAGS = false>
 static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t blocks, hipStream_t st)
 {
+    // mirror CPMG of one walker per lane on images whose periods all exceed
+    // twice the flush interval: unbounded moves from address displacements
+    if constexpr (EMODE == 1 && MFLAGS && TAB == 3 && !NOFLUX) {
+        if (ls.k == 1 && zaddress_displacement_ok(a.zl)) {
+            hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS, true>), dim3(blocks),
+                               dim3(ls.threads), 0, st, a);
+            return hipGetLastError();
+        }
+    }
     // EMODE 2 with one walker per lane keeps the per-label hit counts in LDS
     const size_t lh_bytes = EMODE == 2 ? (size_t)kLhStride * ls.threads * sizeof(uint32_t) : 0;
     switch (ls.k) {

@@ -189,6 +189,35 @@ def test_mirror_reflection_flags_match_oracle(ctx, dims, shape, unfolded):
         assert np.array_equal(m[k], o[k]), k
 
 
+@pytest.mark.parametrize("dims", [(512, 512, 512), (512, 520, 512)])
+@pytest.mark.parametrize("unfolded", [3, 6])
+def test_mirror_address_displacements_match_oracle(ctx, dims, unfolded):
+    """Mirror CPMG of 1024-thread blocks on images whose mirror periods all
+    exceed twice the flush interval (1022 steps): the unbounded moves come
+    from the walkers' address changes between flushes (power-of-two periods,
+    and a period of 1040 with the wrap fix-up), walkers starting in mirror
+    images from -2 to 2 per axis, re-sorts splitting the run into launches:
+    walkers bit-exact, echo sums to 1e-12."""
+    print(f"[test] mirror address displacements dims={dims} unfolded={unfolded}", flush=True)
+    W, H, D = dims
+    solid, seeds, x, y, z, pen = _case(W, H, D, 8.0, 0.25, 3000, 47)
+    x, y, z = _mirror_images(solid, x, y, z, 49)
+    ctx.set_launch(1024, 1)
+    ctx.set_locality(True, resort_every=300, min_walkers=0)
+    ctx.set_unfolded(unfolded)
+    try:
+        img = R.Image.from_solid(ctx, solid)
+        g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=700, steps_per_echo=3, bc="mirror", penalty=pen)
+        img.close()
+    finally:
+        ctx.set_locality(True, resort_every=1000, min_walkers=65536)
+        ctx.set_launch(256, 1)
+        ctx.set_unfolded(3)
+    o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=700, steps_per_echo=3, bc="mirror", penalty=pen)
+    _assert_walkers(g, o)
+    _assert_echoes(g["echo_sums"], o["echo_sums"])
+
+
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
 def test_map_matches_oracle(ctx, bc):
     print(f"[test] map bc={bc}", flush=True)
