@@ -265,6 +265,7 @@ def main():
         prof["file"] = os.path.relpath(counters_file, ROOT)
         prof_tpb = int(str(prof.get("launch", "0,0")).split(",")[0])
         if prof.get("bc") != args.bc or int(prof.get("walkers_per_lane", 0)) != k or \
+                int(prof.get("image_side", S)) != S or \
                 (args.bc == "mirror" and (prof_tpb >= 1024) != (tpb >= 1024)):
             prof = None
     # gather probe: random 16-byte loads from tables held in L2, in the MALL and in HBM
@@ -347,7 +348,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def walk_kernel_label(bc, k, threads, form=3):
+def walk_kernel_label(bc, k, threads, form=3, side=512):
     image = {"mirror": "unfolded mirror image, walkers folded into its first octant with their reflection flags "
                        "at every launch",
              "periodic": "unfolded periodic image",
@@ -356,8 +357,11 @@ def walk_kernel_label(bc, k, threads, form=3):
     table = ("flag-keyed byte-key tables" if bc == "mirror" and threads >= 1024 else
              "byte-key direction table" + (" per reflection-flag combination" if bc == "mirror" else ""))
     chunk = "8-byte (4 x 4 x 4 voxel)" if (form == 6 and bc == "mirror") else "16-byte (8 x 4 x 4 voxel)"
+    # mirror periods of 2 x 512 exceed twice the 511-step flush interval
+    disp = (", unbounded moves from address changes at each flush" if bc == "mirror" and threads >= 1024 and k == 1
+            and side >= 512 else "")
     return (f"zwalk_kernel<CPMG, {k} walker(s) per lane, {chunk} register chunk cache by indexed buffer loads, "
-            f"{table}, direction drawn a step ahead> on the {image}")
+            f"{table}, direction drawn a step ahead{disp}> on the {image}")
 
 
 def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, launches, k, threads, clock=None,
@@ -371,7 +375,7 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
     gathers), the measured HBM bytes per launch, and the walk's L2 request
     rate against the random-gather probe's."""
     out = {"bound": "valu", "achieved": steps_per_s, "peak": None, "unit": "walker-steps/s", "frac": None,
-           "traffic": None, "kernel": walk_kernel_label(args.bc, k, threads, form), "avg_kernel_ms": avg_kernel_s * 1e3,
+           "traffic": None, "kernel": walk_kernel_label(args.bc, k, threads, form, args.size), "avg_kernel_ms": avg_kernel_s * 1e3,
            "launches": launches, "walker_steps_per_launch": steps_per_launch,
            "peak_basis": f"{SIMDS} SIMDs x {CLOCK_HZ / 1e9} GHz / {CYCLES_PER_VALU} cycles per wave64 VALU "
                          "instruction x 64 lanes / VALU instructions per wave-step"}
