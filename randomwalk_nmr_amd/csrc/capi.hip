@@ -155,6 +155,9 @@ struct rwnmr_ctx {
     // CPMG walk; 6 as 3 with an 8-byte (4 x 4 x 4 voxel) chunk for mirror
     // walks (others run 3)
     int unfolded = 3;
+    // resident 1024-thread blocks of the flag-keyed mirror walk (two per CU):
+    // the tile count of balanced tiles (zbalanced_tile_walkers)
+    uint32_t balanced_slots = 0;
 };
 
 // Unfolded image of one boundary condition (dilated addresses).
@@ -558,6 +561,19 @@ constexpr uint64_t kMaxPartialBytes = 1ull << 30;
 // autotuner also times the speculative walk (form 5) for these
 constexpr uint64_t kSpecMaxWalkers = 1024ull * 64ull;
 
+// Tiles (echo-partial columns) of a walk launch over a.n walkers: balanced
+// tiles (zbalanced_tile_walkers, which set a.tile_walkers) or the usual grid
+// of blockDim x K walkers per tile (walk_grid_blocks).
+uint32_t walk_columns(const rwnmr_ctx *c, WalkArgs &a, int emode, const LaunchShape &ls)
+{
+    a.tile_walkers = 0;
+    const uint64_t tw = rwnmr::zbalanced_tile_walkers(a, emode, ls, c->balanced_slots);
+    if (tw == 0)
+        return rwnmr::walk_grid_blocks(a.n, ls, a.shift);
+    a.tile_walkers = (uint32_t)tw;
+    return (uint32_t)((a.n + tw - 1) / tw);
+}
+
 // Walk in position order: every `resort_every` echoes the walkers are sorted
 // by the Morton code of their image block and gathered into context arrays,
 // the walk runs with XCD-striped tiles, and at the end the final state is
@@ -571,7 +587,8 @@ rwnmr_status walk_local(rwnmr_ctx *c, const rwnmr_image *img, const WalkArgs &a,
     if (n > 0xFFFFFFFFull)
         return fail(c, RWNMR_ERR_INVALID, "locality ordering supports up to 2^32 walkers");
     const LaunchShape ls = c->shape;
-    const uint64_t per_tile = (uint64_t)ls.threads * ls.k;
+    // a's tiles (balanced tiles when walk_columns set them)
+    const uint64_t per_tile = a.tile_walkers ? (uint64_t)a.tile_walkers : (uint64_t)ls.threads * ls.k;
     const uint32_t tiles = (uint32_t)((n + per_tile - 1) / per_tile);
     // keys: the unfolded word address (zl.bits - 5 bits) or the 30-bit Morton code of the block
     const int key_bits = a.zwords ? (int)std::max<uint32_t>(1u, a.zl.bits > 5 ? a.zl.bits - 5 : 1u) : 30;
@@ -762,7 +779,8 @@ rwnmr_status cpmg_common(rwnmr_ctx *c, const rwnmr_image *img, rwnmr_walkers *w,
         return fail(c, RWNMR_ERR_INVALID, "Jacobian forward: needs the unfolded labelled image");
     // the Jacobian forward holds one walker per lane
     const LaunchShape shape = jac ? LaunchShape{c->shape.threads, 1u} : c->shape;
-    const uint32_t blocks = rwnmr::walk_grid_blocks(w->n, shape, (int)shift);
+    // tiles, one echo-partial column each (balanced tiles where they apply)
+    const uint32_t blocks = jac ? rwnmr::walk_grid_blocks(w->n, shape, (int)shift) : walk_columns(c, a, emode, shape);
     a.partial_stride = blocks;
     double *sums = nullptr;
     if (echo_sums) {
@@ -903,6 +921,9 @@ rwnmr_status rwnmr_ctx_create(rwnmr_ctx **out, int32_t device)
     c->host_flag[1] = 0;
     // the no-allocation entry point's scratch on this device (the device is current)
     c->scratch_err = scratch_address(device, &c->scratch);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->balanced_slots = 2u * (uint32_t)cus;
     *out = c;
     return RWNMR_OK;
 }

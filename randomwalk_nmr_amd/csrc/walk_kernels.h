@@ -24,6 +24,8 @@ struct WalkArgs {
     BrickLayout layout;              // device word order of the blocks
     uint32_t tile_map;               // 0: block b walks tile b; 1: tiles striped per XCD
     uint32_t n_tiles;                // tiles of the launch (tile_map == 1)
+    uint32_t tile_walkers;           // 0: a tile is blockDim x K walkers; else the walkers of a tile, at most
+                                     // blockDim (balanced tiles, zbalanced_tile_walkers)
     int32_t shift;                   // log2(voxel division)
     int32_t *x, *y, *z;
     uint64_t *seed;

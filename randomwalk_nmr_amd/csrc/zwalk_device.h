@@ -465,17 +465,8 @@ __device__ __forceinline__ uint32_t zcompact(uint32_t v, uint32_t m)
     return u;
 }
 
-// Address displacements recover the unbounded moves when every period of
-// the layout exceeds twice the flush interval: a walker moves at most
-// kZFlush voxels per axis between flushes, less than half a period, so the
-// coordinate change modulo the period is the move itself.
-__host__ __device__ inline bool zaddress_displacement_ok(const ZLayout &zl)
-{
-    return zl.period[0] > 2u * kZFlush && zl.period[1] > 2u * kZFlush && zl.period[2] > 2u * kZFlush;
-}
-
-// One tile (blockDim.x * K consecutive walkers from tile * blockDim.x * K)
-// walked for a.echoes x a.steps_per_echo steps on the unfolded image
+// One tile (blockDim.x * K consecutive walkers from tile * blockDim.x * K,
+// or a.tile_walkers from tile * a.tile_walkers with balanced tiles) walked for a.echoes x a.steps_per_echo steps on the unfolded image
 // a.zwords / a.zl, with the direction table already in LDS.  EMODE 0 = hit
 // counting (map), 1 = CPMG with per-walker or uniform delta, 2 = per-label
 // delta (ldelta: the label deltas in LDS; labels from a.zlabels; per-label
@@ -531,8 +522,13 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     constexpr int NL = (EMODE == 2 && !LH_LDS && NS == 1) ? kMaxLabels : 1;
     const char *words = reinterpret_cast<const char *>(a.zwords);
     const zrsrc_t zrsrc = CACHE == 64 ? zchunk_rsrc8(a.zwords, a.zl) : zchunk_rsrc(a.zwords, a.zl);
-    const uint64_t base = tile * blockDim.x * K + threadIdx.x;
+    // the tile's walkers: blockDim x K from tile x blockDim x K, or with
+    // balanced tiles a.tile_walkers (at most blockDim, K == 1) from tile x
+    // a.tile_walkers
+    const uint64_t tw = a.tile_walkers ? (uint64_t)a.tile_walkers : (uint64_t)blockDim.x * K;
+    const uint64_t base = tile * tw + threadIdx.x;
     const uint64_t n = a.n;
+    const uint64_t end = (tile + 1) * tw < n ? (tile + 1) * tw : n;
     const uint32_t echoes = a.echoes, spe = a.steps_per_echo;
     const int n_labels = a.n_labels;
 
@@ -556,7 +552,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     static_for<K>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         const uint64_t id = base + (uint64_t)k * blockDim.x;
-        act[k] = id < n;
+        act[k] = id < end;
         const uint64_t j = act[k] ? id : 0;
         X[k] = act[k] ? a.x[j] : 0;
         Y[k] = act[k] ? a.y[j] : 0;

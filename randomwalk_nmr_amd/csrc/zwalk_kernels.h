@@ -13,6 +13,39 @@ namespace rwnmr {
 // component must stay within +-511).
 static constexpr uint32_t kZFlush = 511;
 
+// Address displacements recover the unbounded moves when every period of
+// the layout exceeds twice the flush interval: a walker moves at most
+// kZFlush voxels per axis between flushes, less than half a period, so the
+// coordinate change modulo the period is the move itself.
+__host__ __device__ inline bool zaddress_displacement_ok(const ZLayout &zl)
+{
+    return zl.period[0] > 2u * kZFlush && zl.period[1] > 2u * kZFlush && zl.period[2] > 2u * kZFlush;
+}
+
+// True when a CPMG launch of shape ls runs the mirror walk with flag-keyed
+// tables and address displacements (zwalk_kernel<1, 1, ..., TAB 3, ...,
+// ADISP>): one walker per lane, 1024-thread blocks (two per CU).
+inline bool zadisp_walk(const WalkArgs &a, int emode, const LaunchShape &ls)
+{
+    return a.zwords && a.zmflags && !a.znoflux && (a.zcache == 2 || a.zcache == 3 || a.zcache == 5) &&
+           emode == 1 && ls.k == 1 && ls.threads >= 1024 && a.shift == 0 && zaddress_displacement_ok(a.zl);
+}
+
+// Balanced tiles of that walk: with more walkers than one round of `slots`
+// resident blocks holds, tiles of equal walker counts (at most blockDim)
+// that fill whole rounds, so no CU idles through a part-empty last round
+// (1M walkers on 512 slots: 1024 tiles of 977 walkers, two full rounds,
+// instead of 977 tiles of 1024, the second round 465 of 512).  Returns the
+// walkers per tile, or 0 for the usual tiles of blockDim x K walkers.
+inline uint64_t zbalanced_tile_walkers(const WalkArgs &a, int emode, const LaunchShape &ls, uint32_t slots)
+{
+    const uint64_t round = (uint64_t)slots * ls.threads;
+    if (!zadisp_walk(a, emode, ls) || slots == 0 || a.n <= round)
+        return 0;
+    const uint64_t rounds = (a.n + round - 1) / round;
+    return (a.n + rounds * slots - 1) / (rounds * slots);
+}
+
 // Words of the unfolded bit volume for layout zl (at least one 16-byte chunk).
 inline uint64_t zimage_words(const ZLayout &zl) { return zl.bits > 7 ? (1ull << (zl.bits - 5)) : 4ull; }
 

@@ -606,7 +606,14 @@ hipError_t launch_zwalk(const WalkArgs &a, int emode, const LaunchShape &ls, hip
         ls.threads == 0)
         return hipErrorInvalidValue;
     const uint64_t per_block = (uint64_t)ls.threads * ls.k;
-    const uint64_t tiles = (a.n + per_block - 1) / per_block;
+    uint64_t tiles = (a.n + per_block - 1) / per_block;
+    if (a.tile_walkers) {
+        // balanced tiles: only for the address-displacement mirror walk, one
+        // lane per walker of the tile
+        if (!zadisp_walk(a, emode, ls) || a.tile_walkers > ls.threads)
+            return hipErrorInvalidValue;
+        tiles = (a.n + a.tile_walkers - 1) / a.tile_walkers;
+    }
     if (tiles > 0x7FFFFFF0ull || (a.tile_map && a.n_tiles != tiles))
         return hipErrorInvalidValue;
     const uint32_t blocks = a.tile_map ? (uint32_t)(((tiles + kXcds - 1) / kXcds) * kXcds) : (uint32_t)tiles;

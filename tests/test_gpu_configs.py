@@ -79,22 +79,27 @@ def test_cpmg_512_1m_walkers_10000_echoes(ctx):
     assert np.all(np.abs(g["echo_sums"] - o["echo_sums"]) <= ECHO_RTOL * np.abs(o["echo_sums"]))
     # the whole run again in 1024-thread blocks (flag-keyed tables; on this
     # image, whose periods exceed twice the flush interval, the unbounded
-    # moves come from address displacements), with both chunk caches: the
-    # same walkers bit for bit
-    for form in (3, 6):
-        print(f"[test] cpmg 512^3 full run, 1024 x 1, image form {form}", flush=True)
+    # moves come from address displacements; 1M walkers are more than one
+    # round of resident blocks, so the tiles are balanced: 977 walkers in
+    # each of two full rounds of blocks), with both chunk caches,
+    # position-ordered and (form 6) in the caller's order: the same walkers
+    # bit for bit
+    for form, local in ((3, True), (6, True), (6, False)):
+        print(f"[test] cpmg 512^3 full run, 1024 x 1, image form {form}, locality {local}", flush=True)
         X2, Y2, Z2, SD2 = _dev(torch, x), _dev(torch, y), _dev(torch, z), _dev(torch, seeds)
         EN2 = torch.ones(N, dtype=torch.float64, device="cuda:0")
         HT2 = torch.zeros(N, dtype=torch.int32, device="cuda:0")
         SUMS2 = torch.zeros(E + 1, dtype=torch.float64, device="cuda:0")
         ctx.set_launch(1024, 1)
         ctx.set_unfolded(form)
+        ctx.set_locality(local)
         try:
             R.cpmg_run(ctx, img, X2, Y2, Z2, SD2, echoes=E, steps_per_echo=spe, bc="mirror", delta=delta,
                        energy=EN2, hits=HT2, echo_sums=SUMS2)
         finally:
             ctx.set_launch(256, 1)
             ctx.set_unfolded(3)
+            ctx.set_locality(True)
         for A, B in ((X, X2), (Y, Y2), (Z, Z2), (SD, SD2), (HT, HT2)):
             assert torch.equal(A, B)
         assert _bits_equal(en, EN2.cpu().numpy())
