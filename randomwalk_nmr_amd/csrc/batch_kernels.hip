@@ -55,6 +55,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchArgs b)
         a.layout = I.layout;
         a.tile_map = 0;
         a.n_tiles = 0;
+        a.tile_walkers = 0;   // tiles of blockDim x K walkers
         a.shift = 0;
         a.x = I.x;
         a.y = I.y;
