@@ -306,26 +306,6 @@ __device__ __forceinline__ uint32_t zchunk_bit(uint32_t c, const uint4 &QV)
     return (zchunk_word(c, QV) << (c & 31u)) >> 31;
 }
 
-// Chunk caches of CW bits (64: 4 x 4 x 4 voxels, 128: 8 x 4 x 4) read by
-// chunk id (c >> log2 CW, the load address is the id times the chunk bytes)
-// and tested with one 64-bit shift, whose hardware shift count is c & 63.
-template <int CW>
-__device__ __forceinline__ void zchunk_fill_n(const char *words, uint32_t c, uint32_t &QB, uint4 &QV)
-{
-    static_assert(CW == 64 || CW == 128, "chunk of 64 or 128 bits");
-    const uint32_t cid = c >> (CW == 64 ? 6 : 7);
-    if (cid != QB) {
-        if (CW == 64) {
-            const uint2 v = *reinterpret_cast<const uint2 *>(words + (cid << 3));
-            QV.x = v.x;
-            QV.y = v.y;
-        } else {
-            QV = *reinterpret_cast<const uint4 *>(words + (cid << 4));
-        }
-        QB = cid;
-    }
-}
-
 // Buffer descriptor over the unfolded image's 16-byte chunks (base address,
 // stride 16, num_records = chunk count, built by the compiler's
 // make_buffer_rsrc builtin) for the chunk reloads; ids at or past the chunk
@@ -466,8 +446,9 @@ __device__ __forceinline__ uint32_t zcompact(uint32_t v, uint32_t m)
 }
 
 // One tile (blockDim.x * K consecutive walkers from tile * blockDim.x * K,
-// or a.tile_walkers from tile * a.tile_walkers with balanced tiles) walked for a.echoes x a.steps_per_echo steps on the unfolded image
-// a.zwords / a.zl, with the direction table already in LDS.  EMODE 0 = hit
+// or a.tile_walkers from tile * a.tile_walkers with balanced tiles) walked
+// for a.echoes x a.steps_per_echo steps on the unfolded image a.zwords /
+// a.zl, with the direction table already in LDS.  EMODE 0 = hit
 // counting (map), 1 = CPMG with per-walker or uniform delta, 2 = per-label
 // delta (ldelta: the label deltas in LDS; labels from a.zlabels; per-label
 // hit counts in a.label_hits).  Echo partials go to part[row * part_stride]
