@@ -485,8 +485,11 @@ __device__ __forceinline__ uint32_t zcompact(uint32_t v, uint32_t m)
 // last flush (per axis, the coordinate change taken modulo the period into
 // (-period / 2, period / 2), negated along reflected axes), one VALU per
 // move fewer.
+// HITS false (CPMG, a.hits null): no wall-hit count, as the reference's
+// CPMG kernels (NMR_cpmg_cuda.cpp:380-386: a hit only multiplies the
+// energy), one VALU per step fewer.
 template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB = 0, bool NOFLUX = false, bool MFLAGS = false,
-          int NS = 1, bool JAC = false, bool ADISP = false>
+          int NS = 1, bool JAC = false, bool ADISP = false, bool HITS = true>
 __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, double *part, uint64_t part_stride,
                                            double (*lds)[kEchoCols], const uint4 *tab, const uint2 *fix,
                                            const double *ldelta = nullptr, const uint8_t *keys = nullptr,
@@ -498,6 +501,7 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
     static_assert(TAB != 3 || (MFLAGS && CACHE >= 64), "flag-keyed tables: pipelined mirror walk only");
     static_assert(NS == 1 || (NS == 8 && EMODE == 2 && K == 1 && CACHE >= 64), "label sets: 8 sets, K == 1");
     static_assert(!ADISP || (MFLAGS && !NOFLUX && CACHE >= 64), "address displacements: pipelined mirror walk");
+    static_assert(HITS || (EMODE == 1 && CACHE >= 64), "uncounted hits: pipelined CPMG walk");
     // label-set walks count no per-label hits (a.label_hits is null)
     constexpr bool LH_LDS = EMODE == 2 && K == 1 && NS == 1;
     constexpr int NL = (EMODE == 2 && !LH_LDS && NS == 1) ? kMaxLabels : 1;
@@ -724,7 +728,8 @@ __device__ __forceinline__ void zwalk_tile(const WalkArgs &a, uint64_t tile, dou
                 // two short divergent paths (4 VALU in all) instead of the
                 // selects and unconditional multiply the compiler would make
                 if (__builtin_expect(wall != 0u, 0)) {
-                    Hc[k] += 1u;
+                    if (HITS)
+                        Hc[k] += 1u;
                     E[k] *= DL[k];
                 } else {
                     A[k] = C[k];

@@ -12,7 +12,7 @@
 namespace rwnmr {
 
 template <int EMODE, int K, bool WRAPFIX, int CACHE, int TAB, bool NOFLUX = false, bool MFLAGS = false,
-          bool ADISP = false>
+          bool ADISP = false, bool HITS = true>
 __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkArgs a)
 {
     __shared__ double lds[kEchoChunk][kEchoCols];
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(max_threads_for_k(K)) void zwalk_kernel(const WalkA
         zbuild_table_wide(small, tab);
         __syncthreads();
     }
-    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS, 1, false, ADISP>(
+    zwalk_tile<EMODE, K, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS, 1, false, ADISP, HITS>(
         a, tile, a.partials ? a.partials + tile : nullptr, a.partial_stride, lds, tab, fix, ldelta, key_bytes,
         EMODE == 2 && K == 1 ? zlh : nullptr);
 }
@@ -452,8 +452,13 @@ static hipError_t launch_z(const WalkArgs &a, const LaunchShape &ls, uint32_t bl
     // twice the flush interval: unbounded moves from address displacements
     if constexpr (EMODE == 1 && MFLAGS && TAB == 3 && !NOFLUX) {
         if (ls.k == 1 && zaddress_displacement_ok(a.zl)) {
-            hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS, true>), dim3(blocks),
-                               dim3(ls.threads), 0, st, a);
+            // hit counts only when the caller asked for them
+            if (a.hits)
+                hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS, true>), dim3(blocks),
+                                   dim3(ls.threads), 0, st, a);
+            else
+                hipLaunchKernelGGL((zwalk_kernel<EMODE, 1, WRAPFIX, CACHE, TAB, NOFLUX, MFLAGS, true, false>),
+                                   dim3(blocks), dim3(ls.threads), 0, st, a);
             return hipGetLastError();
         }
     }

@@ -83,12 +83,14 @@ def test_cpmg_512_1m_walkers_10000_echoes(ctx):
     # round of resident blocks, so the tiles are balanced: 977 walkers in
     # each of two full rounds of blocks), with both chunk caches,
     # position-ordered and (form 6) in the caller's order: the same walkers
-    # bit for bit
-    for form, local in ((3, True), (6, True), (6, False)):
-        print(f"[test] cpmg 512^3 full run, 1024 x 1, image form {form}, locality {local}", flush=True)
+    # bit for bit (runs without a hit array count no hits: the headline's
+    # kernel variant)
+    for form, local, count in ((3, True, True), (3, True, False), (6, True, True), (6, False, False)):
+        print(f"[test] cpmg 512^3 full run, 1024 x 1, image form {form}, locality {local}, hits {count}",
+              flush=True)
         X2, Y2, Z2, SD2 = _dev(torch, x), _dev(torch, y), _dev(torch, z), _dev(torch, seeds)
         EN2 = torch.ones(N, dtype=torch.float64, device="cuda:0")
-        HT2 = torch.zeros(N, dtype=torch.int32, device="cuda:0")
+        HT2 = torch.zeros(N, dtype=torch.int32, device="cuda:0") if count else None
         SUMS2 = torch.zeros(E + 1, dtype=torch.float64, device="cuda:0")
         ctx.set_launch(1024, 1)
         ctx.set_unfolded(form)
@@ -100,7 +102,7 @@ def test_cpmg_512_1m_walkers_10000_echoes(ctx):
             ctx.set_launch(256, 1)
             ctx.set_unfolded(3)
             ctx.set_locality(True)
-        for A, B in ((X, X2), (Y, Y2), (Z, Z2), (SD, SD2), (HT, HT2)):
+        for A, B in ((X, X2), (Y, Y2), (Z, Z2), (SD, SD2)) + (((HT, HT2),) if count else ()):
             assert torch.equal(A, B)
         assert _bits_equal(en, EN2.cpu().numpy())
         assert np.all(np.abs(SUMS2.cpu().numpy() - sums) <= ECHO_RTOL * np.abs(sums))

@@ -197,7 +197,10 @@ def test_mirror_address_displacements_match_oracle(ctx, dims, unfolded):
     from the walkers' address changes between flushes (power-of-two periods,
     and a period of 1040 with the wrap fix-up), walkers starting in mirror
     images from -2 to 2 per axis, re-sorts splitting the run into launches:
-    walkers bit-exact, echo sums to 1e-12."""
+    walkers bit-exact, echo sums to 1e-12.  The same walk on device arrays
+    with no hit array (the kernel then counts no hits, as the reference's
+    CPMG kernels) gives the same positions, seeds and energies."""
+    import torch
     print(f"[test] mirror address displacements dims={dims} unfolded={unfolded}", flush=True)
     W, H, D = dims
     solid, seeds, x, y, z, pen = _case(W, H, D, 8.0, 0.25, 3000, 47)
@@ -208,6 +211,11 @@ def test_mirror_address_displacements_match_oracle(ctx, dims, unfolded):
     try:
         img = R.Image.from_solid(ctx, solid)
         g = R.cpmg_run(ctx, img, x, y, z, seeds, echoes=700, steps_per_echo=3, bc="mirror", penalty=pen)
+        dev = [torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0") for a in (x, y, z, seeds.view(np.int64), pen)]
+        en = torch.ones(len(x), dtype=torch.float64, device="cuda:0")
+        sums = torch.zeros(701, dtype=torch.float64, device="cuda:0")
+        R.cpmg_run(ctx, img, dev[0], dev[1], dev[2], dev[3], echoes=700, steps_per_echo=3, bc="mirror",
+                   penalty=dev[4], energy=en, echo_sums=sums)
         img.close()
     finally:
         ctx.set_locality(True, resort_every=1000, min_walkers=65536)
@@ -216,6 +224,11 @@ def test_mirror_address_displacements_match_oracle(ctx, dims, unfolded):
     o = O.walk(O.bitblock_pack(solid), dims, x, y, z, seeds, echoes=700, steps_per_echo=3, bc="mirror", penalty=pen)
     _assert_walkers(g, o)
     _assert_echoes(g["echo_sums"], o["echo_sums"])
+    for k, T in (("x", dev[0]), ("y", dev[1]), ("z", dev[2])):
+        assert np.array_equal(T.cpu().numpy(), o[k]), k
+    assert np.array_equal(dev[3].cpu().numpy().view(np.uint64), o["seed"])
+    assert np.array_equal(en.cpu().numpy().view(np.uint64), np.asarray(o["energy"]).view(np.uint64))
+    _assert_echoes(sums.cpu().numpy(), o["echo_sums"])
 
 
 @pytest.mark.parametrize("bc", ["noflux", "periodic", "mirror"])
