@@ -387,13 +387,24 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
         peak = VALU_WAVE_INSTR_PER_S * 64.0 / vps
         out.update(peak=peak, frac=steps_per_s / peak, valu_per_wave_step=vps,
                    valu_busy_measured=prof.get("valu_busy"), counters=prof["file"])
+        # the lowest GFX clock at which the counted VALU instructions per
+        # step issue at the achieved rate
+        min_clock = CLOCK_HZ * steps_per_s / peak
+        sec["clock_implied_min_GHz"] = min_clock / 1e9
         if clock:
             # the same ceiling at the GFX clock this run ran at (sustained
-            # load runs below the 2.4 GHz peak), sampled over its timed steps
+            # load runs below the 2.4 GHz peak), sampled over its timed steps;
+            # a sample below the implied minimum is a reading that did not
+            # track the engine (seen as a constant 1.6 GHz on some boxes) and
+            # is reported but not used
             f = clock["median_MHz"] * 1e6
-            sec["valu_at_measured_clock"] = {"clock_GHz": f / 1e9, "peak": peak * f / CLOCK_HZ,
-                                             "frac": steps_per_s / (peak * f / CLOCK_HZ), "clock": clock,
-                                             "counter_run_valu_busy": prof.get("valu_busy")}
+            if f >= min_clock:
+                sec["valu_at_measured_clock"] = {"clock_GHz": f / 1e9, "peak": peak * f / CLOCK_HZ,
+                                                 "frac": steps_per_s / (peak * f / CLOCK_HZ), "clock": clock,
+                                                 "counter_run_valu_busy": prof.get("valu_busy")}
+            else:
+                sec["clock_sample_rejected"] = {"clock": clock, "reason": f"below the {min_clock / 1e9:.3f} GHz "
+                                                "that the achieved rate at the counted VALU per step requires"}
         # HBM bytes per launch scale with the launch's walker-steps
         scale = steps_per_launch / float(prof["walker_steps_per_launch"])
         hbm = float(prof["hbm_bytes_per_launch"]) * scale
