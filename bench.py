@@ -76,7 +76,7 @@ class ClockSampler:
             import amdsmi
             amdsmi.amdsmi_init()
             self._smi = amdsmi
-            self._h = amdsmi.amdsmi_get_processor_handles()[0]   # the box's one GPU
+            self._h = amdsmi.amdsmi_get_processor_handles()[0]   # device 0, the GPU the bench runs on
         except Exception as e:  # noqa: BLE001 - the clock is a secondary figure
             self.error = str(e)
 
@@ -395,7 +395,7 @@ def roofline(args, prof, probe, steps_per_s, steps_per_launch, avg_kernel_s, lau
             # the same ceiling at the GFX clock this run ran at (sustained
             # load runs below the 2.4 GHz peak), sampled over its timed steps;
             # a sample below the implied minimum is a reading that did not
-            # track the engine (seen as a constant 1.6 GHz on some boxes) and
+            # track the engine (seen as a constant 1.6 GHz on some systems) and
             # is reported but not used
             f = clock["median_MHz"] * 1e6
             if f >= min_clock:

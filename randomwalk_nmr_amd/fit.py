@@ -2,7 +2,7 @@
 a least-squares fit (Levenberg-Marquardt) with the exact Jacobian from one
 walk (rwnmr_cpmg_forward_jacobian), a differentiable CPMG decay
 (torch.autograd.Function over rwnmr_cpmg_forward_ckpt / rwnmr_cpmg_backward)
-driving a gradient fit, and a genetic algorithm kept as fallback and judge.
+driving a gradient fit, and a genetic algorithm kept as fallback and comparator.
 
 The reference fits relaxivity with an (unbuilt) MPI island GA
 (B/GA/ga_core.cpp:101-131 run, :133-166 reproduction, :231-357 roulette and
